@@ -1,0 +1,1348 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement (C++17, FP64) of ChannelFlow.jl's CPU/FFTW CNRK2 step path.
+// It is the checker for the HIP implementation: only tests/, the smoke()
+// entry point and bench.py's cpu_baseline leg may load it.  Nothing in the
+// product library links or calls this file.
+//
+// Every function cites the reference file:line it restates.  Where the
+// reference calls the un-vendored ChebyshevHelmoltzSolvers.jl (CHS v0.1.0,
+// git-tree-sha1 27b8d9f9a5b858c0c4e4bce6fb94155ba23020b6), the published
+// Chebyshev-tau equations (docs/src/spatial.md:436-475) are solved by a
+// quasi-tridiagonal even/odd elimination of this file's own design; any exact
+// solver of that square system agrees with CHS to roundoff.
+//
+// Pinning (how each part is checked, see tests/test_oracle_*.py):
+//   transforms        test_ffts.jl known answers (exact coefficients, Nyquist,
+//                     3/2 dealiasing, top Chebyshev degree) + scipy DCT-I
+//   operators         test_operators.jl analytic fields
+//   nonlinear forms   test_nonlinear.jl analytic fields
+//   Helmholtz (CHS)   test_helmoltz.jl cubic + dense LU of the tau system
+//   influence/Stokes  test_helmoltz.jl / test_batchedinfluence.jl /
+//                     test_stokes.jl manufactured solutions + dense system
+//   step              validation/results/decay.csv, waleffe.csv, ts_*.csv
+//   diagnostics       test_postprocessing.jl closed forms
+//
+// Array conventions (reference layouts, column-major):
+//   spectral (k,l,n): index k + Nxh*(l + Nz*n), interleaved complex128
+//   physical (x,z,y): index x + Nxp*(z + Nzp*y), float64
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cplx;
+static const long double PI_L = 3.141592653589793238462643383279502884L;
+
+static thread_local std::string g_error;
+
+// ---------------------------------------------------------------------------
+// Mixed-radix complex FFT (own implementation; replaces FFTW/cuFFT).
+// Decimation in time, recursive, twiddles computed in long double.
+// ---------------------------------------------------------------------------
+struct FFTPlan {
+    int n = 0;
+    std::vector<int> factors;
+    std::vector<cplx> tw;  // exp(-2*pi*i*j/n), j = 0..n-1
+    explicit FFTPlan(int n_) : n(n_) {
+        int m = n;
+        while (m % 4 == 0) { factors.push_back(4); m /= 4; }
+        while (m % 2 == 0) { factors.push_back(2); m /= 2; }
+        for (int p = 3; p * p <= m; p += 2)
+            while (m % p == 0) { factors.push_back(p); m /= p; }
+        if (m > 1) factors.push_back(m);
+        tw.resize(n);
+        for (int j = 0; j < n; ++j) {
+            long double a = -2.0L * PI_L * (long double)j / (long double)n;
+            tw[j] = cplx((double)cosl(a), (double)sinl(a));
+        }
+    }
+    // out[k] = sum_j in[j*stride] * exp(sign*2*pi*i*j*k/n)
+    void rec(const cplx* in, int istride, cplx* out, int len, int fi, int sign, cplx* scratch) const {
+        if (len == 1) { out[0] = in[0]; return; }
+        int p = factors[fi];
+        int m = len / p;
+        int tstep = n / len;
+        for (int q = 0; q < p; ++q)
+            rec(in + q * istride, istride * p, out + q * m, m, fi + 1, sign, scratch);
+        // combine: X[k + r*m] = sum_q W_len^{q(k+r m)} Y_q[k]
+        for (int k = 0; k < m; ++k) {
+            cplx* s = scratch;
+            for (int q = 0; q < p; ++q) {
+                cplx w = tw[(size_t)q * k * tstep % n];
+                if (sign > 0) w = std::conj(w);
+                s[q] = out[q * m + k] * w;
+            }
+            if (p == 2) {
+                out[k] = s[0] + s[1];
+                out[k + m] = s[0] - s[1];
+            } else if (p == 4) {
+                cplx t0 = s[0] + s[2], t1 = s[0] - s[2];
+                cplx t2 = s[1] + s[3], t3 = s[1] - s[3];
+                cplx jt3 = (sign < 0) ? cplx(t3.imag(), -t3.real()) : cplx(-t3.imag(), t3.real());
+                out[k] = t0 + t2;
+                out[k + m] = t1 + jt3;
+                out[k + 2 * m] = t0 - t2;
+                out[k + 3 * m] = t1 - jt3;
+            } else {
+                for (int r = 0; r < p; ++r) {
+                    cplx acc = 0.0;
+                    for (int q = 0; q < p; ++q) {
+                        cplx w = tw[(size_t)((q * r) % p) * (n / p) % n];
+                        if (sign > 0) w = std::conj(w);
+                        acc += s[q] * w;
+                    }
+                    out[k + r * m] = acc;
+                }
+            }
+        }
+    }
+    void exec(const cplx* in, cplx* out, int sign) const {
+        std::vector<cplx> scratch(std::max(8, n));
+        rec(in, 1, out, n, 0, sign, scratch.data());
+    }
+};
+
+struct PlanCache {
+    std::vector<std::unique_ptr<FFTPlan>> plans;
+    const FFTPlan& get(int n) {
+        for (auto& p : plans) if (p->n == n) return *p;
+        plans.emplace_back(new FFTPlan(n));
+        return *plans.back();
+    }
+};
+
+// Real forward transform (FFTW rfft convention, unnormalised, e^{-i}):
+// X[k] = sum_j x[j] e^{-2 pi i jk/N}, k = 0..N/2.
+static void rfft_line(const FFTPlan& half, const FFTPlan& full, int N, const double* x, cplx* X,
+                      std::vector<cplx>& a, std::vector<cplx>& b) {
+    if (N % 2 == 0 && N >= 2) {
+        int M = N / 2;
+        a.resize(M); b.resize(M);
+        for (int j = 0; j < M; ++j) a[j] = cplx(x[2 * j], x[2 * j + 1]);
+        half.exec(a.data(), b.data(), -1);
+        for (int k = 0; k <= M; ++k) {
+            cplx Zk = b[k % M], Zc = std::conj(b[(M - k) % M]);
+            cplx w = full.tw[k % N];
+            X[k] = 0.5 * (Zk + Zc) - 0.5 * cplx(0, 1) * w * (Zk - Zc);
+        }
+    } else {
+        a.resize(N); b.resize(N);
+        for (int j = 0; j < N; ++j) a[j] = x[j];
+        full.exec(a.data(), b.data(), -1);
+        for (int k = 0; k <= N / 2; ++k) X[k] = b[k];
+    }
+}
+
+// Real backward transform (FFTW brfft convention, unnormalised, e^{+i}):
+// x[j] = sum_{k=0}^{N-1} C[k] e^{+2 pi i jk/N}, C Hermitian-extended from
+// X[0..N/2]; imaginary parts of DC and (even N) Nyquist are ignored.
+static void brfft_line(const FFTPlan& half, const FFTPlan& full, int N, const cplx* Xin, double* x,
+                       std::vector<cplx>& a, std::vector<cplx>& b) {
+    if (N % 2 == 0 && N >= 2) {
+        int M = N / 2;
+        a.resize(M); b.resize(M);
+        cplx X0 = cplx(Xin[0].real(), 0.0);
+        cplx XM = cplx(Xin[M].real(), 0.0);
+        for (int k = 0; k < M; ++k) {
+            cplx Xk = (k == 0) ? X0 : Xin[k];
+            cplx Xm = (k == 0) ? XM : Xin[M - k];
+            cplx Xc = std::conj(Xm);
+            cplx w = std::conj(full.tw[k]);
+            a[k] = (Xk + Xc) + cplx(0, 1) * w * (Xk - Xc);
+        }
+        half.exec(a.data(), b.data(), +1);
+        for (int j = 0; j < M; ++j) { x[2 * j] = b[j].real(); x[2 * j + 1] = b[j].imag(); }
+    } else {
+        a.resize(N); b.resize(N);
+        a[0] = cplx(Xin[0].real(), 0.0);
+        for (int k = 1; k <= N / 2; ++k) { a[k] = Xin[k]; a[N - k] = std::conj(Xin[k]); }
+        full.exec(a.data(), b.data(), +1);
+        for (int j = 0; j < N; ++j) x[j] = b[j].real();
+    }
+}
+
+// DCT-I (FFTW REDFT00, unnormalised) of a complex sequence of length P+1 via
+// a complex FFT of its even extension (length 2P).
+static void redft00_line(const FFTPlan& ext, int P, const cplx* x, cplx* X, std::vector<cplx>& a,
+                         std::vector<cplx>& b) {
+    if (P == 0) { X[0] = x[0]; return; }
+    a.resize(2 * P); b.resize(2 * P);
+    for (int j = 0; j <= P; ++j) a[j] = x[j];
+    for (int j = 1; j < P; ++j) a[2 * P - j] = x[j];
+    ext.exec(a.data(), b.data(), -1);
+    for (int k = 0; k <= P; ++k) X[k] = b[k];
+}
+
+// ---------------------------------------------------------------------------
+// Grid (src/grids.jl:22-74) and index helpers (src/indexing.jl:16-28)
+// ---------------------------------------------------------------------------
+struct Grid {
+    int Nx, Ny, Nz;
+    double Lx, Lz;
+    int Nxh, Nxp, Nzp;
+    Grid(int nx, int ny, int nz, double lx, double lz) : Nx(nx), Ny(ny), Nz(nz), Lx(lx), Lz(lz) {
+        Nxp = (3 * Nx + 1) / 2;  // cld(3n, 2), grids.jl:63
+        Nzp = (3 * Nz + 1) / 2;
+        Nxh = Nx / 2 + 1;        // spectralsize, grids.jl:71-74
+    }
+    size_t S() const { return (size_t)Nxh * Nz * Ny; }
+    size_t B() const { return (size_t)Nxh * Nz; }
+    size_t phys() const { return (size_t)Nxp * Nzp * Ny; }
+    int lmode(int iz) const { return iz <= (Nz >> 1) ? iz : iz - Nz; }  // indexing.jl:16-17
+    double alpha() const { return 2.0 * M_PI / Lx; }
+    double beta() const { return 2.0 * M_PI / Lz; }
+    bool xnyq(int ix) const { return (Nx % 2 == 0) && ix == Nx / 2; }
+    bool znyq(int iz) const { return (Nz % 2 == 0) && iz == Nz / 2; }
+};
+
+// Lobatto nodes, upper wall first (grids.jl:28)
+static double lobatto(int j, int Ny) {
+    long double a = PI_L * (long double)j / (long double)(Ny - 1);
+    return (double)cosl(a);
+}
+
+// ---------------------------------------------------------------------------
+// Transforms (src/ffts.jl, src/transforms/chebyshev.jl)
+// ---------------------------------------------------------------------------
+struct Transforms {
+    Grid g;
+    PlanCache cache;
+    const FFTPlan *xhalf = nullptr, *xfull = nullptr, *zplan = nullptr, *yext = nullptr;
+    std::vector<cplx> padded;  // (Nxp/2+1, Nzp, Ny)
+    explicit Transforms(const Grid& g_) : g(g_) {
+        xfull = &cache.get(g.Nxp);
+        xhalf = &cache.get(std::max(1, g.Nxp / 2));
+        zplan = &cache.get(g.Nzp);
+        yext = &cache.get(std::max(1, 2 * (g.Ny - 1)));
+        padded.assign((size_t)(g.Nxp / 2 + 1) * g.Nzp * g.Ny, 0.0);
+    }
+    int nkp() const { return g.Nxp / 2 + 1; }
+
+    // plan_cheb / plan_icheb + mul! (chebyshev.jl:45-66): DCT-I along n of a
+    // (B, Ny) matrix; INVERSE doubles both endpoint inputs first.
+    void cheb(const cplx* src, cplx* dst, bool inverse, size_t B) {
+        int Ny = g.Ny, P = Ny - 1;
+#pragma omp parallel
+        {
+            std::vector<cplx> x(Ny), X(Ny), a, b;
+#pragma omp for schedule(static)
+            for (long s = 0; s < (long)B; ++s) {
+                for (int n = 0; n < Ny; ++n) x[n] = src[s + B * n];
+                if (inverse) { x[0] *= 2.0; x[P] *= 2.0; }
+                redft00_line(*yext, P, x.data(), X.data(), a, b);
+                for (int n = 0; n < Ny; ++n) dst[s + B * n] = X[n];
+            }
+        }
+    }
+
+    // ForwardFFT! (ffts.jl:65-82): rfft x, fft z, truncate, DCT-I,
+    // normalize_forward! (ffts.jl:254-269).
+    void forward(const double* u, cplx* U) {
+        const Grid& G = g;
+        int nkp_ = nkp();
+        // rfft along x for every (z, y) line
+#pragma omp parallel
+        {
+            std::vector<cplx> a, b, X(nkp_);
+#pragma omp for schedule(static)
+            for (long line = 0; line < (long)G.Nzp * G.Ny; ++line) {
+                rfft_line(*xhalf, *xfull, G.Nxp, u + (size_t)line * G.Nxp, X.data(), a, b);
+                for (int k = 0; k < nkp_; ++k) padded[(size_t)line * nkp_ + k] = X[k];
+            }
+        }
+        // fft along z for retained k; copy_from_padded (ffts.jl:277-284)
+        int npos = (G.Nz >> 1) + 1, nneg = G.Nz - npos;
+        std::vector<cplx> resolved(G.S());
+#pragma omp parallel
+        {
+            std::vector<cplx> a(G.Nzp), b(G.Nzp);
+#pragma omp for collapse(2) schedule(static)
+            for (long y = 0; y < G.Ny; ++y)
+                for (long k = 0; k < G.Nxh; ++k) {
+                    for (int z = 0; z < G.Nzp; ++z) a[z] = padded[(size_t)k + nkp_ * (z + (size_t)G.Nzp * y)];
+                    zplan->exec(a.data(), b.data(), -1);
+                    for (int l = 0; l < npos; ++l) resolved[k + G.Nxh * (l + (size_t)G.Nz * y)] = b[l];
+                    for (int l = 0; l < nneg; ++l)
+                        resolved[k + G.Nxh * (npos + l + (size_t)G.Nz * y)] = b[G.Nzp - nneg + l];
+                }
+        }
+        cheb(resolved.data(), U, false, G.B());
+        normalize_forward(U, 1.0 / ((double)G.Nxp * G.Nzp * (G.Ny - 1)));
+    }
+
+    void normalize_forward(cplx* U, double normalization) {
+        const Grid& G = g;
+        for (int iy = 0; iy < G.Ny; ++iy) {
+            double endpoint = (iy == 0 || iy == G.Ny - 1) ? 0.5 : 1.0;
+            for (int iz = 0; iz < G.Nz; ++iz)
+                for (int ix = 0; ix < G.Nxh; ++ix) {
+                    size_t i = ix + (size_t)G.Nxh * (iz + (size_t)G.Nz * iy);
+                    U[i] = (G.xnyq(ix) || G.znyq(iz)) ? cplx(0.0) : U[i] * (normalization * endpoint);
+                }
+        }
+    }
+
+    // InverseFFT! (ffts.jl:151-166): icheb (endpoint x2), embed x0.5
+    // (copy_to_padded!, 242-251), zero_nyquist! (306-312), brfft.
+    void inverse(const cplx* U, double* u) {
+        const Grid& G = g;
+        int nkp_ = nkp();
+        std::vector<cplx> resolved(G.S());
+        cheb(U, resolved.data(), true, G.B());
+        int npos = (G.Nz >> 1) + 1, nneg = G.Nz - npos;
+        std::fill(padded.begin(), padded.end(), cplx(0.0));
+#pragma omp parallel
+        {
+            std::vector<cplx> a(G.Nzp), b(G.Nzp);
+#pragma omp for collapse(2) schedule(static)
+            for (long y = 0; y < G.Ny; ++y)
+                for (long k = 0; k < G.Nxh; ++k) {
+                    std::fill(a.begin(), a.end(), cplx(0.0));
+                    if (!G.xnyq((int)k)) {
+                        for (int l = 0; l < npos; ++l)
+                            if (!G.znyq(l)) a[l] = 0.5 * resolved[k + G.Nxh * (l + (size_t)G.Nz * y)];
+                        for (int l = 0; l < nneg; ++l)
+                            a[G.Nzp - nneg + l] = 0.5 * resolved[k + G.Nxh * (npos + l + (size_t)G.Nz * y)];
+                    }
+                    zplan->exec(a.data(), b.data(), +1);
+                    for (int z = 0; z < G.Nzp; ++z) padded[(size_t)k + nkp_ * (z + (size_t)G.Nzp * y)] = b[z];
+                }
+        }
+#pragma omp parallel
+        {
+            std::vector<cplx> a, b;
+#pragma omp for schedule(static)
+            for (long line = 0; line < (long)G.Nzp * G.Ny; ++line)
+                brfft_line(*xhalf, *xfull, G.Nxp, &padded[(size_t)line * nkp_], u + (size_t)line * G.Nxp, a, b);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Spectral operators (src/fields/operators.jl)
+// ---------------------------------------------------------------------------
+// ddx1! operators.jl:14-23
+static void ddx1(const Grid& g, cplx* out, const cplx* U, bool add) {
+    double a = g.alpha();
+    for (int n = 0; n < g.Ny; ++n)
+        for (int iz = 0; iz < g.Nz; ++iz)
+            for (int ix = 0; ix < g.Nxh; ++ix) {
+                size_t i = ix + (size_t)g.Nxh * (iz + (size_t)g.Nz * n);
+                cplx v = cplx(0, 1) * (double)ix * a * U[i];
+                out[i] = add ? out[i] + v : v;
+            }
+}
+// ddx3! operators.jl:74-83
+static void ddx3(const Grid& g, cplx* out, const cplx* U, bool add) {
+    double b = g.beta();
+    for (int n = 0; n < g.Ny; ++n)
+        for (int iz = 0; iz < g.Nz; ++iz) {
+            int l = g.lmode(iz);
+            for (int ix = 0; ix < g.Nxh; ++ix) {
+                size_t i = ix + (size_t)g.Nxh * (iz + (size_t)g.Nz * n);
+                cplx v = cplx(0, 1) * (double)l * b * U[i];
+                out[i] = add ? out[i] + v : v;
+            }
+        }
+}
+// _batchdiff! batchedinfluence.jl:32-43 (also ddx2! non-aliased path,
+// operators.jl:38-42); (B, Ny) matrix, B contiguous.  Generic over element.
+template <typename T>
+static void batchdiff(T* out, const T* a, size_t B, int Ny) {
+    for (size_t s = 0; s < B; ++s) out[s + B * (Ny - 1)] = T(0.0);
+    if (Ny >= 2)
+        for (size_t s = 0; s < B; ++s) out[s + B * (Ny - 2)] = 2.0 * (Ny - 1) * a[s + B * (Ny - 1)];
+    for (int j = Ny - 3; j >= 0; --j)  // 0-based j
+        for (size_t s = 0; s < B; ++s) out[s + B * j] = out[s + B * (j + 2)] + 2.0 * (j + 1) * a[s + B * (j + 1)];
+    for (size_t s = 0; s < B; ++s) out[s] *= 0.5;
+}
+static void ddx2(const Grid& g, cplx* out, const cplx* U, bool add) {
+    size_t B = g.B();
+    if (!add) { batchdiff(out, U, B, g.Ny); return; }
+    std::vector<cplx> tmp(g.S());
+    batchdiff(tmp.data(), U, B, g.Ny);
+    for (size_t i = 0; i < g.S(); ++i) out[i] += tmp[i];
+}
+// laplacian! operators.jl:96-125 (non-aliased fused recurrence)
+static void laplacian(const Grid& g, cplx* b, const cplx* a) {
+    double a2 = g.alpha() * g.alpha(), b2 = g.beta() * g.beta();
+    int Ny = g.Ny;
+    size_t B = g.B();
+    for (int j = Ny; j >= 1; --j)  // 1-based as in the reference
+        for (int iz = 0; iz < g.Nz; ++iz) {
+            int l = g.lmode(iz);
+            for (int ix = 0; ix < g.Nxh; ++ix) {
+                size_t s = ix + (size_t)g.Nxh * iz;
+                double k2 = a2 * ix * ix + b2 * (double)l * l;
+                cplx d2 = 0.0;
+                auto A = [&](int jj) { return a[s + B * (jj - 1)]; };
+                auto Bv = [&](int jj) { return b[s + B * (jj - 1)]; };
+                if (j + 2 <= Ny)
+                    d2 = 4.0 * j * (j + 1) * A(j + 2) + (2.0 * (j + 1) / (j + 2)) * (Bv(j + 2) + k2 * A(j + 2));
+                if (j + 4 <= Ny) d2 -= ((double)j / (j + 2)) * (Bv(j + 4) + k2 * A(j + 4));
+                b[s + B * (j - 1)] = (j == 1 ? d2 / 2.0 : d2) - k2 * A(j);
+            }
+        }
+}
+// curl! operators.jl:172-190
+static void curl(const Grid& g, cplx* const O[3], const cplx* const U[3]) {
+    ddx2(g, O[0], U[2], false);
+    ddx2(g, O[2], U[0], false);
+    double al = g.alpha(), be = g.beta();
+    const cplx I(0, 1);
+    for (int iy = 0; iy < g.Ny; ++iy)
+        for (int iz = 0; iz < g.Nz; ++iz) {
+            int l = g.lmode(iz);
+            for (int ix = 0; ix < g.Nxh; ++ix) {
+                size_t i = ix + (size_t)g.Nxh * (iz + (size_t)g.Nz * iy);
+                O[0][i] -= I * (double)l * be * U[1][i];
+                O[1][i] = I * ((double)l * be * U[0][i] - (double)ix * al * U[2][i]);
+                O[2][i] = I * (double)ix * al * U[1][i] - O[2][i];
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev utilities standing in for CHS (chebcoeffs, diff, diff2!, wall
+// derivative functionals) and _bulkmean (meanmode.jl:13-19).
+// ---------------------------------------------------------------------------
+// chebcoeffs of Lobatto values: DCT-I / P with halved endpoints (test/helpers.jl:5-11)
+static void chebcoeffs(int Ny, const double* values, double* coeffs) {
+    int P = Ny - 1;
+    std::vector<double> out(Ny);
+    for (int k = 0; k <= P; ++k) {
+        long double acc = values[0] + ((k % 2) ? -values[P] : values[P]);
+        for (int j = 1; j < P; ++j) acc += 2.0L * values[j] * cosl(PI_L * (long double)j * k / P);
+        out[k] = (double)(acc / P);
+    }
+    out[0] *= 0.5; out[P] *= 0.5;
+    std::copy(out.begin(), out.end(), coeffs);
+}
+template <typename T>
+static void diff1(T* out, const T* a, int Ny) { batchdiff(out, a, 1, Ny); }
+template <typename T>
+static T diff_right(const T* a, int Ny) {  // phi'(+1) = sum n^2 a_n
+    T acc = T(0.0);
+    for (int n = 0; n < Ny; ++n) acc += (double)n * n * a[n];
+    return acc;
+}
+template <typename T>
+static T diff_left(const T* a, int Ny) {  // phi'(-1) = sum (-1)^(n+1) n^2 a_n
+    T acc = T(0.0);
+    for (int n = 0; n < Ny; ++n) acc += ((n % 2) ? 1.0 : -1.0) * (double)n * n * a[n];
+    return acc;
+}
+template <typename T>
+static T bulkmean(const T* a, int Ny) {  // _bulkmean, meanmode.jl:13-19
+    T v = a[0];
+    for (int n = 2; n <= Ny - 1; n += 2) v += a[n] / (1.0 - (double)n * n);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev-tau Helmholtz solver (restates CHS HelmoltzSolver semantics,
+// docs/src/spatial.md:436-475): rows 0..P-2 of (t0 D^2 - t1) a = f plus two
+// wall rows, Dirichlet phi(+1), phi(-1) or Neumann phi'(+1), phi'(-1).
+// Solved by eliminating the quasi-tridiagonal even/odd systems obtained from
+// the two-fold Chebyshev integration identity.
+// ---------------------------------------------------------------------------
+struct Helmholtz {
+    int P = 0;
+    double t0 = 1, t1 = 0;
+    bool neumann = false;
+    // per parity p (0 even, 1 odd): unknowns x_j = a_{2j+p}, j = 0..J_p
+    int J[2];
+    std::vector<double> lo[2], up[2], s[2], alpha[2], gam[2], w[2];
+    double sumwg[2];
+
+    static double cfac(int m) { return m == 0 ? 2.0 : 1.0; }
+    Helmholtz() {}
+    Helmholtz(int P_, double t0_, double t1_, bool neu) { setup(P_, t0_, t1_, neu); }
+    void setup(int P_, double t0_, double t1_, bool neu) {
+        P = P_; t0 = t0_; t1 = t1_; neumann = neu;
+        for (int p = 0; p < 2; ++p) {
+            int Jp = (P - p) / 2;  // largest j with 2j+p <= P
+            J[p] = Jp;
+            lo[p].assign(Jp + 1, 0.0); up[p].assign(Jp + 1, 0.0); s[p].assign(Jp + 1, 0.0);
+            alpha[p].assign(Jp + 2, 0.0); gam[p].assign(Jp + 1, 0.0); w[p].assign(Jp + 1, 0.0);
+            std::vector<double> d(Jp + 1, 0.0);
+            for (int j = 1; j <= Jp; ++j) {
+                int n = 2 * j + p;
+                double en = (n <= P - 2) ? 1.0 : 0.0, en2 = (n + 2 <= P - 2) ? 1.0 : 0.0;
+                lo[p][j] = -t1 * cfac(n - 2) / (4.0 * n * (n - 1));
+                d[j] = t0 + t1 * en / (2.0 * ((double)n * n - 1.0));
+                up[p][j] = -t1 * en2 / (4.0 * n * (n + 1));
+            }
+            alpha[p][Jp + 1] = 0.0;
+            for (int j = Jp; j >= 1; --j) {
+                s[p][j] = 1.0 / (d[j] + up[p][j] * alpha[p][j + 1]);
+                alpha[p][j] = -lo[p][j] * s[p][j];
+            }
+            gam[p][0] = 1.0;
+            for (int j = 1; j <= Jp; ++j) gam[p][j] = alpha[p][j] * gam[p][j - 1];
+            sumwg[p] = 0.0;
+            for (int j = 0; j <= Jp; ++j) {
+                int n = 2 * j + p;
+                w[p][j] = neumann ? (double)n * n : 1.0;
+                sumwg[p] += w[p][j] * gam[p][j];
+            }
+        }
+    }
+    // Solve; rhs/out length P+1, T = double or cplx.  gp, gm: wall data at
+    // y=+1, y=-1 (values, or derivatives for Neumann).
+    template <typename T>
+    void solve(T* out, const T* f, T gp, T gm) const {
+        std::vector<T> rhs(P + 1), beta(P + 2), delta(P + 1);
+        auto F = [&](int m) -> T { return (m >= 0 && m <= P - 2) ? f[m] : T(0.0); };
+        for (int p = 0; p < 2; ++p) {
+            int Jp = J[p];
+            // boundary datum for this parity
+            T g;
+            if (!neumann) g = (p == 0) ? 0.5 * (gp + gm) : 0.5 * (gp - gm);
+            else g = (p == 0) ? 0.5 * (gp - gm) : 0.5 * (gp + gm);
+            for (int j = 1; j <= Jp; ++j) {
+                int n = 2 * j + p;
+                rhs[j] = cfac(n - 2) * F(n - 2) / (4.0 * n * (n - 1)) - F(n) / (2.0 * ((double)n * n - 1.0)) +
+                         F(n + 2) / (4.0 * n * (n + 1));
+            }
+            beta[Jp + 1] = T(0.0);
+            for (int j = Jp; j >= 1; --j) beta[j] = s[p][j] * (rhs[j] - up[p][j] * beta[j + 1]);
+            delta[0] = T(0.0);
+            for (int j = 1; j <= Jp; ++j) delta[j] = alpha[p][j] * delta[j - 1] + beta[j];
+            T swd = T(0.0);
+            for (int j = 0; j <= Jp; ++j) swd += w[p][j] * delta[j];
+            T x0 = (g - swd) / sumwg[p];
+            for (int j = 0; j <= Jp; ++j) out[2 * j + p] = gam[p][j] * x0 + delta[j];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// InfluenceModeSolver (src/solvers/influence.jl:21-316) — scalar oracle for
+// the batched solver, exactly as in the reference.
+// ---------------------------------------------------------------------------
+struct InfluenceMode {
+    int Ny, P;
+    double k, l, nu, lambda, shift;
+    Helmholtz pres, vel;
+    std::vector<double> pp, pm, vp, vm, p0, v0;
+    double Ainv[4];
+    double sigma0[2];
+    bool ok = true;
+    InfluenceMode(int Ny_, double k_, double l_, double nu_, double lam_) : Ny(Ny_), P(Ny_ - 1), k(k_), l(l_), nu(nu_), lambda(lam_) {
+        double kappa2 = k * k + l * l;
+        shift = lambda + nu * kappa2;
+        pres.setup(P, 1.0, kappa2, false);
+        vel.setup(P, nu, shift, false);
+        std::vector<double> work(Ny, 0.0), zero(Ny, 0.0);
+        pp.assign(Ny, 0); pm.assign(Ny, 0); vp.assign(Ny, 0); vm.assign(Ny, 0); p0.assign(Ny, 0); v0.assign(Ny, 0);
+        pres.solve(pp.data(), zero.data(), 1.0, 0.0);
+        diff1(work.data(), pp.data(), Ny);
+        vel.solve(vp.data(), work.data(), 0.0, 0.0);
+        pres.solve(pm.data(), zero.data(), 0.0, 1.0);
+        diff1(work.data(), pm.data(), Ny);
+        vel.solve(vm.data(), work.data(), 0.0, 0.0);
+        double A = diff_right(vp.data(), Ny), B = diff_right(vm.data(), Ny);
+        double C = diff_left(vp.data(), Ny), D = diff_left(vm.data(), Ny);
+        double scale = std::max(std::max(std::fabs(A), std::fabs(B)), std::max(std::fabs(C), std::fabs(D)));
+        if (!(std::isfinite(scale) && scale > 0)) ok = false;
+        A /= scale; B /= scale; C /= scale; D /= scale;
+        double det = A * D - B * C;
+        if (!(std::fabs(det) > 2.220446049250313e-16 * std::max(std::fabs(A * D), std::fabs(B * C)))) ok = false;
+        Ainv[0] = D / det / scale; Ainv[1] = -B / det / scale; Ainv[2] = -C / det / scale; Ainv[3] = A / det / scale;
+        std::fill(work.begin(), work.end(), 0.0);
+        work[P - 1] = work[P] = 1.0;
+        std::vector<double> dw(Ny);
+        diff1(dw.data(), work.data(), Ny);
+        pres.solve(p0.data(), dw.data(), 0.0, 0.0);
+        diff1(work.data(), p0.data(), Ny);
+        vel.solve(v0.data(), work.data(), 0.0, 0.0);
+        sigma0[0] = sigma0[1] = 0.0;
+        correct(p0.data(), v0.data());
+        diff1(work.data(), p0.data(), Ny);
+        sigma0[0] = shift * v0[P - 1] + work[P - 1];
+        sigma0[1] = shift * v0[P];
+    }
+    void correct(double* p, double* v) const {  // _influence_correction! influence.jl:161-177
+        double bp = -diff_right(v, Ny), bm = -diff_left(v, Ny);
+        double dp = Ainv[0] * bp + Ainv[1] * bm, dm = Ainv[2] * bp + Ainv[3] * bm;
+        for (int n = 0; n < Ny; ++n) {
+            p[n] += dp * pp[n] + dm * pm[n];
+            v[n] += dp * vp[n] + dm * vm[n];
+        }
+    }
+    void solve_pv(double* p, double* v, const double* r, const double* Ry) const {  // influence.jl:282-316
+        std::vector<double> work(Ny);
+        pres.solve(p, r, 0.0, 0.0);
+        diff1(work.data(), p, Ny);
+        for (int n = 0; n < Ny; ++n) work[n] -= Ry[n];
+        vel.solve(v, work.data(), 0.0, 0.0);
+        correct(p, v);
+        diff1(work.data(), p, Ny);
+        double sNm1 = (shift * v[P - 1] + work[P - 1] - Ry[P - 1]) / (1.0 - sigma0[0]);
+        double sN = (shift * v[P] + work[P] - Ry[P]) / (1.0 - sigma0[1]);
+        for (int n = 0; n <= P; ++n) {
+            p[n] += ((n % 2 == 0) ? sNm1 : sN) * p0[n];
+            v[n] += ((n % 2 == 0) ? sN : sNm1) * v0[n];
+        }
+    }
+    void solve(cplx* u, cplx* v, cplx* w, cplx* p, const cplx* Rx, const cplx* Ry, const cplx* Rz) const {  // 205-253
+        std::vector<double> pc(Ny), vc(Ny), r(Ny), ry(Ny);
+        for (int im = 0; im < 2; ++im) {
+            for (int n = 0; n < Ny; ++n) ry[n] = im ? Ry[n].imag() : Ry[n].real();
+            diff1(r.data(), ry.data(), Ny);
+            for (int n = 0; n < Ny; ++n)
+                r[n] += im ? k * Rx[n].real() + l * Rz[n].real() : -k * Rx[n].imag() - l * Rz[n].imag();
+            solve_pv(pc.data(), vc.data(), r.data(), ry.data());
+            for (int n = 0; n < Ny; ++n) {
+                if (im) { p[n] += cplx(0, pc[n]); v[n] += cplx(0, vc[n]); }
+                else { p[n] = pc[n]; v[n] = vc[n]; }
+            }
+        }
+        for (int c = 0; c < 2; ++c) {
+            cplx* out = c == 0 ? u : w;
+            const cplx* src = c == 0 ? Rx : Rz;
+            double kk = c == 0 ? k : l;
+            for (int n = 0; n < Ny; ++n) {
+                pc[n] = -kk * p[n].imag() - src[n].real();
+                vc[n] = kk * p[n].real() - src[n].imag();
+            }
+            vel.solve(ry.data(), pc.data(), 0.0, 0.0);
+            vel.solve(r.data(), vc.data(), 0.0, 0.0);
+            for (int n = 0; n < Ny; ++n) out[n] = cplx(ry[n], r[n]);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// BatchedInfluenceSolver (src/solvers/batchedinfluence.jl:17-159)
+// Matrices are (B, Ny) with B contiguous, as spectralmatrix views.
+// ---------------------------------------------------------------------------
+struct BatchedInfluence {
+    int Ny, P;
+    size_t B;
+    double nu, lambda;
+    std::vector<double> k, l, shift;
+    std::vector<Helmholtz> ph, vh;              // per system
+    std::vector<double> pp, pm, vp, vm, p0, v0;  // (B, Ny)
+    std::vector<double> ia, ib, ic, id;         // influence inverse, row-major
+    std::vector<double> sig1, sig2;
+    bool ok = true;
+    std::string err;
+
+    BatchedInfluence(int Ny_, const std::vector<double>& k_, const std::vector<double>& l_, double nu_, double lam_)
+        : Ny(Ny_), P(Ny_ - 1), B(k_.size()), nu(nu_), lambda(lam_), k(k_), l(l_) {
+        shift.resize(B);
+        ph.resize(B); vh.resize(B);
+        pp.assign(B * Ny, 0); pm.assign(B * Ny, 0); vp.assign(B * Ny, 0); vm.assign(B * Ny, 0);
+        p0.assign(B * Ny, 0); v0.assign(B * Ny, 0);
+        ia.resize(B); ib.resize(B); ic.resize(B); id.resize(B); sig1.resize(B); sig2.resize(B);
+        std::vector<int> bad(B, 0);
+#pragma omp parallel
+        {
+            std::vector<double> c(Ny), work(Ny), zero(Ny, 0.0), dw(Ny), a(Ny), b(Ny), x(Ny), y(Ny), q0(Ny), r0(Ny);
+#pragma omp for schedule(static)
+            for (long s = 0; s < (long)B; ++s) {
+                double kap2 = k[s] * k[s] + l[s] * l[s];
+                shift[s] = lambda + nu * kap2;
+                ph[s].setup(P, 1.0, kap2 == 0.0 ? 1.0 : kap2, false);  // batchedinfluence.jl:60-62
+                vh[s].setup(P, nu, shift[s], false);
+                // responses p+, v+, p-, v- (64-74)
+                ph[s].solve(a.data(), zero.data(), 1.0, 0.0);
+                diff1(work.data(), a.data(), Ny);
+                vh[s].solve(b.data(), work.data(), 0.0, 0.0);
+                ph[s].solve(x.data(), zero.data(), 0.0, 1.0);
+                diff1(work.data(), x.data(), Ny);
+                vh[s].solve(y.data(), work.data(), 0.0, 0.0);
+                for (int n = 0; n < Ny; ++n) {
+                    pp[s + B * n] = a[n]; vp[s + B * n] = b[n]; pm[s + B * n] = x[n]; vm[s + B * n] = y[n];
+                }
+                // influence inverse (76-88)
+                double A = diff_right(b.data(), Ny), Bq = diff_right(y.data(), Ny);
+                double C = diff_left(b.data(), Ny), D = diff_left(y.data(), Ny);
+                double scale = std::max(std::max(std::fabs(A), std::fabs(Bq)), std::max(std::fabs(C), std::fabs(D)));
+                A /= scale; Bq /= scale; C /= scale; D /= scale;
+                double det = A * D - Bq * C;
+                if (!(std::isfinite(det) && std::fabs(det) > 2.220446049250313e-16 * std::max(std::fabs(A * D), std::fabs(Bq * C))))
+                    bad[s] = 1;
+                ia[s] = D / (det * scale); ib[s] = -Bq / (det * scale); ic[s] = -C / (det * scale); id[s] = A / (det * scale);
+                // auxiliary tau problem (93-105)
+                std::fill(work.begin(), work.end(), 0.0);
+                work[Ny - 2] = work[Ny - 1] = 1.0;
+                diff1(dw.data(), work.data(), Ny);
+                ph[s].solve(q0.data(), dw.data(), 0.0, 0.0);
+                diff1(dw.data(), q0.data(), Ny);
+                vh[s].solve(r0.data(), dw.data(), 0.0, 0.0);
+                // _batchinfluence!(solver, p0, v0) on this row
+                double upper = 0.0, lower = 0.0;
+                for (int j = 1; j < Ny; ++j) {
+                    double term = (double)j * j * r0[j];
+                    upper -= term;
+                    lower -= ((j % 2 == 1) ? 1.0 : -1.0) * term;
+                }
+                double dp = ia[s] * upper + ib[s] * lower, dm = ic[s] * upper + id[s] * lower;
+                for (int n = 0; n < Ny; ++n) {
+                    q0[n] += dp * a[n] + dm * x[n];
+                    r0[n] += dp * b[n] + dm * y[n];
+                }
+                diff1(dw.data(), q0.data(), Ny);
+                sig1[s] = shift[s] * r0[Ny - 2] + dw[Ny - 2];
+                sig2[s] = shift[s] * r0[Ny - 1];
+                for (int n = 0; n < Ny; ++n) { p0[s + B * n] = q0[n]; v0[s + B * n] = r0[n]; }
+            }
+        }
+        for (size_t s = 0; s < B; ++s)
+            if (bad[s]) { ok = false; err = "singular influence matrix for system " + std::to_string(s + 1); break; }
+    }
+
+    // solve! (129-146) with _batchinfluence! (101-120) and _batchtau! (148-159)
+    void solve(cplx* u, cplx* v, cplx* w, cplx* p, const cplx* Rx, const cplx* Ry, const cplx* Rz) const {
+        const cplx I(0, 1);
+#pragma omp parallel
+        {
+            std::vector<cplx> ry(Ny), r(Ny), pc(Ny), vc(Ny), q(Ny), uu(Ny), ww(Ny), rx(Ny), rz(Ny);
+#pragma omp for schedule(static)
+            for (long s = 0; s < (long)B; ++s) {
+                for (int n = 0; n < Ny; ++n) { rx[n] = Rx[s + B * n]; ry[n] = Ry[s + B * n]; rz[n] = Rz[s + B * n]; }
+                diff1(r.data(), ry.data(), Ny);
+                for (int n = 0; n < Ny; ++n) r[n] += I * k[s] * rx[n] + I * l[s] * rz[n];
+                ph[s].solve(pc.data(), r.data(), cplx(0.0), cplx(0.0));
+                diff1(q.data(), pc.data(), Ny);
+                for (int n = 0; n < Ny; ++n) q[n] -= ry[n];
+                vh[s].solve(vc.data(), q.data(), cplx(0.0), cplx(0.0));
+                // influence
+                cplx upper = 0.0, lower = 0.0;
+                for (int j = 1; j < Ny; ++j) {
+                    cplx term = (double)j * j * vc[j];
+                    upper -= term;
+                    lower -= ((j % 2 == 1) ? 1.0 : -1.0) * term;
+                }
+                cplx dp = ia[s] * upper + ib[s] * lower, dm = ic[s] * upper + id[s] * lower;
+                for (int n = 0; n < Ny; ++n) {
+                    pc[n] += dp * pp[s + B * n] + dm * pm[s + B * n];
+                    vc[n] += dp * vp[s + B * n] + dm * vm[s + B * n];
+                }
+                diff1(q.data(), pc.data(), Ny);
+                // tau
+                cplx s1 = (shift[s] * vc[Ny - 2] + q[Ny - 2] - ry[Ny - 2]) / (1.0 - sig1[s]);
+                cplx s2 = (shift[s] * vc[Ny - 1] + q[Ny - 1] - ry[Ny - 1]) / (1.0 - sig2[s]);
+                for (int n = 0; n < Ny; ++n) {
+                    pc[n] += ((n % 2 == 0) ? s1 : s2) * p0[s + B * n];
+                    vc[n] += ((n % 2 == 0) ? s2 : s1) * v0[s + B * n];
+                }
+                for (int n = 0; n < Ny; ++n) r[n] = I * k[s] * pc[n] - rx[n];
+                vh[s].solve(uu.data(), r.data(), cplx(0.0), cplx(0.0));
+                for (int n = 0; n < Ny; ++n) r[n] = I * l[s] * pc[n] - rz[n];
+                vh[s].solve(ww.data(), r.data(), cplx(0.0), cplx(0.0));
+                for (int n = 0; n < Ny; ++n) {
+                    u[s + B * n] = uu[n]; v[s + B * n] = vc[n]; w[s + B * n] = ww[n]; p[s + B * n] = pc[n];
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// MeanModeSolver (src/solvers/meanmode.jl:38-150)
+// ---------------------------------------------------------------------------
+struct MeanMode {
+    int Ny;
+    double nu, lambda;
+    Helmholtz vel;
+    std::vector<double> response;
+    double response_mean = 0;
+    bool ok = true;
+    MeanMode(int Ny_, double nu_, double lam_) : Ny(Ny_), nu(nu_), lambda(lam_) {
+        vel.setup(Ny - 1, nu, lambda, false);
+        response.assign(Ny, 0.0);
+        std::vector<double> work(Ny, 0.0);
+        work[0] = 1.0;
+        vel.solve(response.data(), work.data(), 0.0, 0.0);
+        response_mean = bulkmean(response.data(), Ny);
+        if (!(std::isfinite(response_mean) && response_mean != 0.0)) ok = false;
+    }
+    // kind 0: pressure gradient (g0,g1); kind 1: bulk velocity target (g0,g1)
+    void solve(cplx* u, cplx* v, cplx* w, cplx* p, const cplx* Rx, const cplx* Ry, const cplx* Rz, int kind,
+               double c0, double c1, double* gout) const {
+        int P = Ny - 1;
+        for (int n = 1; n <= P; ++n) {  // meanmode.jl:121-126 (1-based n+1 -> index n)
+            cplx lower = (n == 1) ? Ry[0] : Ry[n - 1] / 2.0;
+            cplx upper = (n + 1 < P) ? Ry[n + 1] / 2.0 : cplx(0.0);
+            p[n] = (lower - upper) / (double)n;
+        }
+        p[0] = 0.0;
+        p[0] = -bulkmean(p, Ny);
+        for (int n = 0; n < Ny; ++n) v[n] = 0.0;
+        std::vector<double> work(Ny), sol(Ny);
+        for (int i = 0; i < 2; ++i) {
+            cplx* out = i == 0 ? u : w;
+            const cplx* src = i == 0 ? Rx : Rz;
+            for (int part = 0; part < 2; ++part) {
+                for (int n = 0; n < Ny; ++n) work[n] = -(part == 0 ? src[n].real() : src[n].imag());
+                vel.solve(sol.data(), work.data(), 0.0, 0.0);
+                for (int n = 0; n < Ny; ++n) {
+                    if (part == 0) out[n] = sol[n];
+                    else out[n] += cplx(0, 1) * sol[n];
+                }
+            }
+            double target = i == 0 ? c0 : c1;
+            double gradient = (kind == 0) ? target : (target - bulkmean(out, Ny).real()) / response_mean;
+            for (int n = 0; n < Ny; ++n) out[n] += gradient * response[n];
+            gout[i] = gradient;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// StokesSolver (src/solvers/stokes.jl:23-108)
+// ---------------------------------------------------------------------------
+static void zero_nyquist(const Grid& g, cplx* U) {  // ffts.jl:306-312
+    for (int n = 0; n < g.Ny; ++n)
+        for (int iz = 0; iz < g.Nz; ++iz)
+            for (int ix = 0; ix < g.Nxh; ++ix)
+                if (g.xnyq(ix) || g.znyq(iz)) U[ix + (size_t)g.Nxh * (iz + (size_t)g.Nz * n)] = 0.0;
+}
+
+struct Stokes {
+    Grid g;
+    std::unique_ptr<MeanMode> mean;
+    std::unique_ptr<BatchedInfluence> modes;
+    Stokes(const Grid& g_, double nu, double lambda) : g(g_) {
+        mean.reset(new MeanMode(g.Ny, nu, lambda));
+        std::vector<double> k(g.B()), l(g.B());
+        for (int iz = 0; iz < g.Nz; ++iz)
+            for (int ix = 0; ix < g.Nxh; ++ix) {
+                k[ix + (size_t)g.Nxh * iz] = g.alpha() * ix;
+                l[ix + (size_t)g.Nxh * iz] = g.beta() * g.lmode(iz);
+            }
+        modes.reset(new BatchedInfluence(g.Ny, k, l, nu, lambda));
+    }
+    // kind: 0 = pressuregradient (c0,c1), 1 = bulkvelocity total (c0,c1)
+    void solve(cplx* const U[3], cplx* P, const cplx* const R[3], int kind, double c0, double c1,
+               const double* basecoeffs, double* gout) const {
+        double t0 = c0, t1 = c1;
+        if (kind == 1 && basecoeffs) t0 = c0 - bulkmean(basecoeffs, g.Ny);
+        modes->solve(U[0], U[1], U[2], P, R[0], R[1], R[2]);
+        size_t B = g.B();
+        int Ny = g.Ny;
+        std::vector<cplx> col[7];
+        for (auto& c : col) c.resize(Ny);
+        const cplx* src[7] = {U[0], U[1], U[2], P, R[0], R[1], R[2]};
+        for (int f = 0; f < 7; ++f)
+            for (int n = 0; n < Ny; ++n) col[f][n] = src[f][B * n];
+        mean->solve(col[0].data(), col[1].data(), col[2].data(), col[3].data(), col[4].data(), col[5].data(),
+                    col[6].data(), kind, t0, t1, gout);
+        cplx* dst[4] = {U[0], U[1], U[2], P};
+        for (int f = 0; f < 4; ++f)
+            for (int n = 0; n < Ny; ++n) dst[f][B * n] = col[f][n];
+        for (int f = 0; f < 4; ++f) zero_nyquist(g, dst[f]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// NonLinearTerm (src/nonlinear.jl)
+// ---------------------------------------------------------------------------
+enum Form { ROTATING = 0, CONVECTIVE = 1, DIVERGENCE = 2 };
+
+struct NonLinear {
+    Grid g;
+    Form form;
+    std::vector<double> base;  // Chebyshev coeffs of Ub
+    Transforms tr;
+    NonLinear(const Grid& g_, Form f, const std::vector<double>& b) : g(g_), form(f), base(b), tr(g_) {}
+
+    // (t, U, N, add) -> N (= or +=)
+    void eval(const cplx* const U[3], cplx* const N[3], bool add) {
+        size_t S = g.S(), Pn = g.phys();
+        int Ny = g.Ny;
+        size_t B = g.B();
+        std::vector<cplx> TMP[3];
+        for (int i = 0; i < 3; ++i) TMP[i].assign(U[i], U[i] + S);
+        for (int n = 0; n < Ny; ++n) TMP[0][B * n] += base[n];  // (1,1,:) += baseflow
+        std::vector<cplx> out[3];
+        for (int i = 0; i < 3; ++i) out[i].resize(S);
+        int sign = 1;
+        if (form == ROTATING) {  // nonlinear.jl:173-193
+            std::vector<cplx> Om[3];
+            for (int i = 0; i < 3; ++i) Om[i].resize(S);
+            cplx* O[3] = {Om[0].data(), Om[1].data(), Om[2].data()};
+            const cplx* T[3] = {TMP[0].data(), TMP[1].data(), TMP[2].data()};
+            curl(g, O, T);
+            std::vector<double> u[3], w[3], nn(Pn);
+            for (int i = 0; i < 3; ++i) {
+                u[i].resize(Pn); w[i].resize(Pn);
+                tr.inverse(T[i], u[i].data());
+                tr.inverse(O[i], w[i].data());
+            }
+            for (int c = 0; c < 3; ++c) {
+                int a = (c + 1) % 3, b = (c + 2) % 3;
+                for (size_t i = 0; i < Pn; ++i) nn[i] = u[a][i] * w[b][i] - u[b][i] * w[a][i];
+                tr.forward(nn.data(), out[c].data());
+            }
+            sign = 1;
+        } else if (form == CONVECTIVE) {  // nonlinear.jl:113-140
+            std::vector<double> u[3], gr(Pn), nn[3];
+            for (int i = 0; i < 3; ++i) {
+                u[i].resize(Pn);
+                tr.inverse(TMP[i].data(), u[i].data());
+                nn[i].assign(Pn, 0.0);
+            }
+            std::vector<cplx> G(S);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    if (j == 0) ddx1(g, G.data(), TMP[i].data(), false);
+                    else if (j == 1) ddx2(g, G.data(), TMP[i].data(), false);
+                    else ddx3(g, G.data(), TMP[i].data(), false);
+                    tr.inverse(G.data(), gr.data());
+                    // dot!: out[i] = u1*g[i,1] + u2*g[i,2] + u3*g[i,3]
+                    for (size_t q = 0; q < Pn; ++q) nn[i][q] += u[j][q] * gr[q];
+                }
+            for (int i = 0; i < 3; ++i) tr.forward(nn[i].data(), out[i].data());
+            sign = -1;
+        } else {  // DIVERGENCE nonlinear.jl:146-167
+            std::vector<double> u[3], uu(Pn);
+            for (int i = 0; i < 3; ++i) { u[i].resize(Pn); tr.inverse(TMP[i].data(), u[i].data()); }
+            std::vector<cplx> UU(S);
+            for (int i = 0; i < 3; ++i) {
+                std::fill(out[i].begin(), out[i].end(), cplx(0.0));
+                for (int j = 0; j < 3; ++j) {
+                    for (size_t q = 0; q < Pn; ++q) uu[q] = u[i][q] * u[j][q];
+                    tr.forward(uu.data(), UU.data());
+                    if (j == 0) ddx1(g, out[i].data(), UU.data(), false);
+                    else if (j == 1) ddx2(g, out[i].data(), UU.data(), true);
+                    else ddx3(g, out[i].data(), UU.data(), true);
+                }
+            }
+            sign = -1;
+        }
+        for (int i = 0; i < 3; ++i)
+            for (size_t q = 0; q < S; ++q) N[i][q] = add ? N[i][q] + (double)sign * out[i][q] : (double)sign * out[i][q];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// CNRK2 (src/timesteppers/cnrk2.jl)
+// ---------------------------------------------------------------------------
+static const double CNRK2_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+static const double CNRK2_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+static const double CNRK2_C[3] = {1.0 / 6.0, 5.0 / 24.0, 1.0 / 8.0};
+
+struct Problem {
+    Grid g;
+    double nu, dt;
+    int kind;  // 0 pressuregradient, 1 bulkvelocity
+    double c0, c1;
+    std::vector<double> base, curv;
+    std::unique_ptr<Stokes> solvers[3];
+    std::unique_ptr<NonLinear> nl;
+    std::vector<cplx> Q[3], N[3], R[3];
+    double last_grad[2] = {0, 0};
+    Problem(const Grid& g_, const double* basevals, double nu_, double dt_, int form, int kind_, double c0_, double c1_)
+        : g(g_), nu(nu_), dt(dt_), kind(kind_), c0(c0_), c1(c1_) {
+        base.resize(g.Ny); curv.resize(g.Ny);
+        chebcoeffs(g.Ny, basevals, base.data());  // cnrk2.jl:76
+        std::vector<double> d1(g.Ny);
+        diff1(d1.data(), base.data(), g.Ny);      // diff2! (77)
+        diff1(curv.data(), d1.data(), g.Ny);
+        for (int j = 0; j < 3; ++j) solvers[j].reset(new Stokes(g, nu, 1.0 / (CNRK2_C[j] * dt)));
+        nl.reset(new NonLinear(g, (Form)form, base));
+        for (int i = 0; i < 3; ++i) { Q[i].assign(g.S(), 0.0); N[i].assign(g.S(), 0.0); R[i].assign(g.S(), 0.0); }
+    }
+    bool ok() const {
+        for (int j = 0; j < 3; ++j) if (!solvers[j]->modes->ok || !solvers[j]->mean->ok) return false;
+        return true;
+    }
+    // _bulkpressuregradient cnrk2.jl:98-107
+    void bulkgrad(const cplx* const U[3], double out[2]) {
+        int Ny = g.Ny;
+        size_t B = g.B();
+        std::vector<cplx> u(Ny), n(Ny);
+        for (int i = 0; i < 2; ++i) {
+            int c = i == 0 ? 0 : 2;
+            for (int m = 0; m < Ny; ++m) { u[m] = U[c][B * m]; n[m] = N[c][B * m]; }
+            cplx shear = diff_right(u.data(), Ny) - diff_left(u.data(), Ny);
+            double bs = i == 0 ? nu * bulkmean(curv.data(), Ny) : 0.0;
+            out[i] = nu * shear.real() / 2.0 + bs + bulkmean(n.data(), Ny).real();
+        }
+    }
+    // step! cnrk2.jl:149-207 (forcing = NoForcing)
+    void step(cplx* const U[3], cplx* P) {
+        size_t S = g.S(), B = g.B();
+        double grads[2] = {kind == 0 ? c0 : 0.0, kind == 0 ? c1 : 0.0};
+        cplx* Np[3] = {N[0].data(), N[1].data(), N[2].data()};
+        cplx* Rp[3] = {R[0].data(), R[1].data(), R[2].data()};
+        for (int j = 0; j < 3; ++j) {
+            const cplx* Uc[3] = {U[0], U[1], U[2]};
+            nl->eval(Uc, Np, false);
+            double old[2] = {grads[0], grads[1]};
+            if (kind == 1) bulkgrad(Uc, old);
+            for (int i = 0; i < 3; ++i) {
+                if (j == 0) std::copy(N[i].begin(), N[i].end(), Q[i].begin());
+                else for (size_t q = 0; q < S; ++q) Q[i][q] = CNRK2_A[j] * Q[i][q] + N[i][q];
+            }
+            double lambda = 1.0 / (CNRK2_C[j] * dt);
+            double weight = CNRK2_B[j] / CNRK2_C[j];
+            for (int i = 0; i < 3; ++i) {
+                laplacian(g, R[i].data(), U[i]);
+                if (i == 0) ddx1(g, N[i].data(), P, false);
+                else if (i == 1) ddx2(g, N[i].data(), P, false);
+                else ddx3(g, N[i].data(), P, false);
+                for (size_t q = 0; q < S; ++q) R[i][q] = lambda * U[i][q] + nu * R[i][q] - N[i][q] + weight * Q[i][q];
+            }
+            for (int n = 0; n < g.Ny; ++n) R[0][B * n] += 2.0 * nu * curv[n];
+            R[0][0] -= old[0];
+            R[2][0] -= old[1];
+            const cplx* Rc[3] = {Rp[0], Rp[1], Rp[2]};
+            double gout[2];
+            solvers[j]->solve(U, P, Rc, kind, kind == 0 ? c0 : c0, kind == 0 ? c1 : c1, base.data(), gout);
+            if (kind == 0) { grads[0] = c0; grads[1] = c1; }
+            last_grad[0] = gout[0]; last_grad[1] = gout[1];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// pressure reconstruction (src/pressure.jl:176-285) and projection (117-135)
+// ---------------------------------------------------------------------------
+static void pressure_poisson(const Grid& g, const cplx* const A[3], const cplx* v, double nu, cplx* P) {
+    // div!(P, A) (operators.jl:270-276)
+    ddx1(g, P, A[0], false);
+    ddx2(g, P, A[1], true);
+    ddx3(g, P, A[2], true);
+    int Ny = g.Ny;
+    size_t B = g.B();
+#pragma omp parallel
+    {
+        std::vector<double> work(Ny), sol(Ny);
+        std::vector<cplx> rhs(Ny), normal(Ny);
+        Helmholtz h;
+#pragma omp for collapse(2) schedule(static)
+        for (long iz = 0; iz < g.Nz; ++iz)
+            for (long ix = 0; ix < g.Nxh; ++ix) {
+                if (ix == 0 && iz == 0) continue;
+                if (g.xnyq((int)ix) || g.znyq((int)iz)) continue;
+                double k = g.alpha() * ix, l = g.beta() * g.lmode((int)iz);
+                h.setup(Ny - 1, 1.0, k * k + l * l, true);
+                size_t s = ix + (size_t)g.Nxh * iz;
+                for (int n = 0; n < Ny; ++n) { rhs[n] = P[s + B * n]; normal[n] = A[1][s + B * n]; }
+                cplx upper = 0.0, lower = 0.0;
+                for (int j = 0; j < Ny; ++j) { upper += normal[j]; lower += ((j % 2) ? -1.0 : 1.0) * normal[j]; }
+                for (int n = 2; n <= Ny - 1; ++n) {
+                    cplx wall = nu * ((double)n * n * ((double)n * n - 1.0) / 3.0) * v[s + B * n];
+                    upper += wall;
+                    lower += (n % 2 == 0) ? wall : -wall;
+                }
+                for (int n = 0; n < Ny; ++n) work[n] = rhs[n].real();
+                h.solve(sol.data(), work.data(), upper.real(), lower.real());
+                for (int n = 0; n < Ny; ++n) rhs[n] = cplx(sol[n], rhs[n].imag());
+                for (int n = 0; n < Ny; ++n) work[n] = rhs[n].imag();
+                h.solve(sol.data(), work.data(), upper.imag(), lower.imag());
+                for (int n = 0; n < Ny; ++n) P[s + B * n] = cplx(rhs[n].real(), sol[n]);
+            }
+    }
+    // zero mode (pressure.jl:265-279)
+    std::vector<cplx> normal(Ny), mean(Ny);
+    for (int n = 0; n < Ny; ++n) normal[n] = A[1][B * n];
+    for (int n = 1; n <= Ny - 1; ++n) {
+        cplx lower = n == 1 ? normal[0] : normal[n - 1] / 2.0;
+        cplx upper = (n + 1 < Ny - 1) ? normal[n + 1] / 2.0 : cplx(0.0);
+        mean[n] = (lower - upper) / (double)n;
+    }
+    mean[0] = 0.0;
+    mean[0] = -bulkmean(mean.data(), Ny);
+    for (int n = 0; n < Ny; ++n) P[B * n] = mean[n];
+    zero_nyquist(g, P);
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (src/postprocessing.jl:21-128)
+// ---------------------------------------------------------------------------
+static double spectral_dot(const Grid& g, const cplx* U, const cplx* V) {
+    int Ny = g.Ny;
+    auto moment = [](int n) { return (n % 2 == 0) ? 1.0 / (1.0 - (double)n * n) : 0.0; };
+    std::vector<double> M((size_t)Ny * Ny);
+    for (int m = 0; m < Ny; ++m)
+        for (int n = 0; n < Ny; ++n) M[m + (size_t)Ny * n] = (moment(m + n) + moment(std::abs(m - n))) / 2.0;
+    size_t B = g.B();
+    double result = 0.0;
+    std::vector<cplx> work(Ny), vv(Ny);
+    for (int iz = 0; iz < g.Nz; ++iz)
+        for (int ix = 0; ix < g.Nxh; ++ix) {
+            if (g.xnyq(ix) || g.znyq(iz)) continue;
+            size_t s = ix + (size_t)g.Nxh * iz;
+            for (int n = 0; n < Ny; ++n) vv[n] = V[s + B * n];
+            for (int m = 0; m < Ny; ++m) {
+                cplx acc = 0.0;
+                for (int n = 0; n < Ny; ++n) acc += M[m + (size_t)Ny * n] * vv[n];
+                work[m] = acc;
+            }
+            cplx d = 0.0;
+            for (int n = 0; n < Ny; ++n) d += std::conj(U[s + B * n]) * work[n];
+            result += (ix == 0 ? 1.0 : 2.0) * d.real();
+        }
+    return result;
+}
+
+// ===========================================================================
+// C ABI for the Python test harness (tests/, smoke(), bench cpu_baseline).
+// Complex arrays are interleaved double pairs.  Return 0 on success.
+// ===========================================================================
+extern "C" {
+
+const char* orc_last_error() { return g_error.c_str(); }
+
+int orc_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+// -- raw transforms ---------------------------------------------------------
+int orc_redft00(int B, int Ny, const double* src, double* dst) {
+    Grid g(1, Ny, 1, 1.0, 1.0);
+    Transforms t(g);
+    t.cheb((const cplx*)src, (cplx*)dst, false, (size_t)B);
+    return 0;
+}
+int orc_cheb(int B, int Ny, int inverse, const double* src, double* dst) {
+    Grid g(1, Ny, 1, 1.0, 1.0);
+    Transforms t(g);
+    t.cheb((const cplx*)src, (cplx*)dst, inverse != 0, (size_t)B);
+    return 0;
+}
+int orc_forward(int Nx, int Ny, int Nz, double Lx, double Lz, const double* u, double* U) {
+    Grid g(Nx, Ny, Nz, Lx, Lz);
+    Transforms t(g);
+    t.forward(u, (cplx*)U);
+    return 0;
+}
+int orc_inverse(int Nx, int Ny, int Nz, double Lx, double Lz, const double* U, double* u) {
+    Grid g(Nx, Ny, Nz, Lx, Lz);
+    Transforms t(g);
+    t.inverse((const cplx*)U, u);
+    return 0;
+}
+int orc_normalize_forward(int Nx, int Ny, int Nz, double norm, double* U) {
+    Grid g(Nx, Ny, Nz, 1.0, 1.0);
+    Transforms t(g);
+    t.normalize_forward((cplx*)U, norm);
+    return 0;
+}
+
+// -- operators -----------------------------------------------------------------
+// op: 1 ddx1, 2 ddx2, 3 ddx3, 4 laplacian
+int orc_operator(int op, int Nx, int Ny, int Nz, double Lx, double Lz, const double* U, double* out, int add) {
+    Grid g(Nx, Ny, Nz, Lx, Lz);
+    const cplx* a = (const cplx*)U;
+    cplx* o = (cplx*)out;
+    switch (op) {
+        case 1: ddx1(g, o, a, add != 0); break;
+        case 2: ddx2(g, o, a, add != 0); break;
+        case 3: ddx3(g, o, a, add != 0); break;
+        case 4: laplacian(g, o, a); break;
+        default: g_error = "unknown operator"; return 1;
+    }
+    return 0;
+}
+int orc_curl(int Nx, int Ny, int Nz, double Lx, double Lz, const double* U3, double* O3) {
+    Grid g(Nx, Ny, Nz, Lx, Lz);
+    size_t S = g.S();
+    const cplx* U[3] = {(const cplx*)U3, (const cplx*)U3 + S, (const cplx*)U3 + 2 * S};
+    cplx* O[3] = {(cplx*)O3, (cplx*)O3 + S, (cplx*)O3 + 2 * S};
+    curl(g, O, U);
+    return 0;
+}
+
+// -- CHS stand-ins ---------------------------------------------------------------
+int orc_chebcoeffs(int Ny, const double* values, double* coeffs) { chebcoeffs(Ny, values, coeffs); return 0; }
+int orc_diff(int Ny, const double* a, double* out) { diff1(out, a, Ny); return 0; }
+int orc_helmholtz(int P, double t0, double t1, int neumann, const double* f, double* out, double gp, double gm) {
+    Helmholtz h(P, t0, t1, neumann != 0);
+    h.solve(out, f, gp, gm);
+    return 0;
+}
+// complex variant (f, out interleaved complex; gp, gm complex)
+int orc_helmholtz_c(int P, double t0, double t1, int neumann, const double* f, double* out, const double* gp,
+                    const double* gm) {
+    Helmholtz h(P, t0, t1, neumann != 0);
+    h.solve((cplx*)out, (const cplx*)f, cplx(gp[0], gp[1]), cplx(gm[0], gm[1]));
+    return 0;
+}
+
+// -- modal solvers -----------------------------------------------------------------
+// Scalar InfluenceModeSolver: u,v,w,p,Rx,Ry,Rz complex Ny vectors.
+// info (out, 18 doubles): Ainv[4], sigma0[2], shift, ok, then
+//   pp[0], pm[0]... (not used)
+int orc_influence_mode(int Ny, double k, double l, double nu, double lambda, const double* Rx, const double* Ry,
+                       const double* Rz, double* u, double* v, double* w, double* p, double* responses) {
+    if (!(Ny >= 5 && Ny % 2 == 1)) { g_error = "Gibson's tau correction requires odd Ny >= 5"; return 2; }
+    if (!(k * k + l * l > 0)) { g_error = "InfluenceModeSolver requires kappa2 > 0"; return 2; }
+    InfluenceMode m(Ny, k, l, nu, lambda);
+    if (!m.ok) { g_error = "the influence matrix is numerically singular"; return 2; }
+    m.solve((cplx*)u, (cplx*)v, (cplx*)w, (cplx*)p, (const cplx*)Rx, (const cplx*)Ry, (const cplx*)Rz);
+    if (responses) {
+        const std::vector<double>* r[6] = {&m.pp, &m.pm, &m.vp, &m.vm, &m.p0, &m.v0};
+        for (int i = 0; i < 6; ++i) std::copy(r[i]->begin(), r[i]->end(), responses + (size_t)i * Ny);
+    }
+    return 0;
+}
+// BatchedInfluenceSolver on (B, Ny) matrices.
+int orc_batched_influence(int Ny, int B, const double* k, const double* l, double nu, double lambda,
+                          const double* Rx, const double* Ry, const double* Rz, double* u, double* v, double* w,
+                          double* p) {
+    if (!(Ny >= 5 && Ny % 2 == 1)) { g_error = "tau correction requires odd Ny >= 5"; return 2; }
+    std::vector<double> kk(k, k + B), ll(l, l + B);
+    BatchedInfluence h(Ny, kk, ll, nu, lambda);
+    if (!h.ok) { g_error = h.err; return 2; }
+    h.solve((cplx*)u, (cplx*)v, (cplx*)w, (cplx*)p, (const cplx*)Rx, (const cplx*)Ry, (const cplx*)Rz);
+    return 0;
+}
+// MeanModeSolver. kind 0 pressure gradient, 1 bulk velocity; grads[2] out.
+int orc_mean_mode(int Ny, double nu, double lambda, const double* Rx, const double* Ry, const double* Rz, double* u,
+                  double* v, double* w, double* p, int kind, double c0, double c1, double* grads,
+                  double* response) {
+    if (Ny < 4) { g_error = "at least four Chebyshev coefficients are required"; return 2; }
+    if (!(std::isfinite(nu) && nu > 0)) { g_error = "nu must be finite and positive"; return 2; }
+    if (!(std::isfinite(lambda) && lambda >= 0)) { g_error = "lambda must be finite and nonnegative"; return 2; }
+    MeanMode m(Ny, nu, lambda);
+    if (!m.ok) { g_error = "the uniform-gradient response has no finite nonzero bulk mean"; return 2; }
+    m.solve((cplx*)u, (cplx*)v, (cplx*)w, (cplx*)p, (const cplx*)Rx, (const cplx*)Ry, (const cplx*)Rz, kind, c0, c1,
+            grads);
+    if (response) std::copy(m.response.begin(), m.response.end(), response);
+    return 0;
+}
+static int check_grid(int Nx, int Ny, int Nz, double Lx, double Lz) {
+    if (Ny % 2 == 0) { g_error = "Gibson's tau correction requires odd Ny"; return 2; }
+    if (!(Nx > 0 && Nz > 0)) { g_error = "Nx and Nz must be positive"; return 2; }
+    if (!(std::isfinite(Lx) && std::isfinite(Lz) && Lx > 0 && Lz > 0)) {
+        g_error = "Lx and Lz must be finite and positive"; return 2;
+    }
+    return 0;
+}
+// StokesSolver.solve!: U3 (3 fields), P, R3; basecoeffs may be NULL.
+int orc_stokes(int Nx, int Ny, int Nz, double Lx, double Lz, double nu, double lambda, const double* R3, double* U3,
+               double* P, int kind, double c0, double c1, const double* basecoeffs, double* grads) {
+    if (int rc = check_grid(Nx, Ny, Nz, Lx, Lz)) return rc;
+    Grid g(Nx, Ny, Nz, Lx, Lz);
+    Stokes st(g, nu, lambda);
+    if (!st.modes->ok) { g_error = st.modes->err; return 2; }
+    size_t S = g.S();
+    cplx* U[3] = {(cplx*)U3, (cplx*)U3 + S, (cplx*)U3 + 2 * S};
+    const cplx* R[3] = {(const cplx*)R3, (const cplx*)R3 + S, (const cplx*)R3 + 2 * S};
+    st.solve(U, (cplx*)P, R, kind, c0, c1, basecoeffs, grads);
+    return 0;
+}
+
+// -- problem handle: nonlinear term, step, projection, pressure --------------------
+void* orc_problem_create(int Nx, int Ny, int Nz, double Lx, double Lz, double nu, double dt, const double* basevals,
+                         int form, int kind, double c0, double c1) {
+    try {
+        if (check_grid(Nx, Ny, Nz, Lx, Lz)) return nullptr;
+        if (!(std::isfinite(dt) && dt > 0)) { g_error = "dt must be finite and positive"; return nullptr; }
+        Grid g(Nx, Ny, Nz, Lx, Lz);
+        Problem* p = new Problem(g, basevals, nu, dt, form, kind, c0, c1);
+        if (!p->ok()) { g_error = "singular influence matrix"; delete p; return nullptr; }
+        return p;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return nullptr;
+    }
+}
+void orc_problem_destroy(void* h) { delete (Problem*)h; }
+int orc_base_coeffs(void* h, double* base, double* curv) {
+    Problem* p = (Problem*)h;
+    std::copy(p->base.begin(), p->base.end(), base);
+    std::copy(p->curv.begin(), p->curv.end(), curv);
+    return 0;
+}
+// N = nonlinear(U) (+= if add)
+int orc_nonlinear(void* h, const double* U3, double* N3, int add) {
+    Problem* p = (Problem*)h;
+    size_t S = p->g.S();
+    const cplx* U[3] = {(const cplx*)U3, (const cplx*)U3 + S, (const cplx*)U3 + 2 * S};
+    cplx* N[3] = {(cplx*)N3, (cplx*)N3 + S, (cplx*)N3 + 2 * S};
+    p->nl->eval(U, N, add != 0);
+    return 0;
+}
+// nsteps CNRK2 steps in place on U3 (3 fields) and P.
+int orc_step(void* h, double* U3, double* P, int nsteps) {
+    Problem* p = (Problem*)h;
+    size_t S = p->g.S();
+    cplx* U[3] = {(cplx*)U3, (cplx*)U3 + S, (cplx*)U3 + 2 * S};
+    for (int s = 0; s < nsteps; ++s) p->step(U, (cplx*)P);
+    return 0;
+}
+int orc_last_gradient(void* h, double* g2) {
+    Problem* p = (Problem*)h;
+    g2[0] = p->last_grad[0]; g2[1] = p->last_grad[1];
+    return 0;
+}
+// project! (pressure.jl:117-135) in place on U3
+int orc_project(void* h, double* U3) {
+    Problem* p = (Problem*)h;
+    const Grid& g = p->g;
+    size_t S = g.S();
+    Stokes st(g, 1.0, 0.0);
+    if (!st.modes->ok) { g_error = st.modes->err; return 2; }
+    std::vector<cplx> R[3], phi(S);
+    cplx* U[3] = {(cplx*)U3, (cplx*)U3 + S, (cplx*)U3 + 2 * S};
+    for (int i = 0; i < 3; ++i) {
+        R[i].resize(S);
+        laplacian(g, R[i].data(), U[i]);
+        for (auto& x : R[i]) x = -x;
+    }
+    const cplx* Rc[3] = {R[0].data(), R[1].data(), R[2].data()};
+    double grads[2];
+    if (p->kind == 1) st.solve(U, phi.data(), Rc, 1, p->c0, p->c1, p->base.data(), grads);
+    else st.solve(U, phi.data(), Rc, 0, 0.0, 0.0, p->base.data(), grads);
+    return 0;
+}
+// pressure(U, problem) (pressure.jl:176-185)
+int orc_pressure(void* h, const double* U3, double* P) {
+    Problem* p = (Problem*)h;
+    const Grid& g = p->g;
+    size_t S = g.S();
+    std::vector<cplx> A[3];
+    for (int i = 0; i < 3; ++i) A[i].resize(S);
+    const cplx* U[3] = {(const cplx*)U3, (const cplx*)U3 + S, (const cplx*)U3 + 2 * S};
+    cplx* Ap[3] = {A[0].data(), A[1].data(), A[2].data()};
+    p->nl->eval(U, Ap, false);
+    const cplx* Ac[3] = {A[0].data(), A[1].data(), A[2].data()};
+    pressure_poisson(g, Ac, U[1], p->nu, (cplx*)P);
+    return 0;
+}
+// _pressure_poisson(A, v, nu) directly
+int orc_pressure_poisson(int Nx, int Ny, int Nz, double Lx, double Lz, double nu, const double* A3, const double* v,
+                         double* P) {
+    Grid g(Nx, Ny, Nz, Lx, Lz);
+    size_t S = g.S();
+    const cplx* A[3] = {(const cplx*)A3, (const cplx*)A3 + S, (const cplx*)A3 + 2 * S};
+    pressure_poisson(g, A, (const cplx*)v, nu, (cplx*)P);
+    return 0;
+}
+// spectral dot (postprocessing.jl:31-55) of scalar fields
+double orc_dot(int Nx, int Ny, int Nz, const double* U, const double* V) {
+    Grid g(Nx, Ny, Nz, 1.0, 1.0);
+    return spectral_dot(g, (const cplx*)U, (const cplx*)V);
+}
+
+}  // extern "C"

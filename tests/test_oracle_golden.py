@@ -1,0 +1,78 @@
+"""Oracle against the reference's committed whole-pipeline golden values.
+
+validation/results/decay.csv (ConvectiveForm), waleffe.csv (RotatingForm,
+the default hot path) and ts_*.csv were written by the reference's own
+validation runs; the oracle must reproduce their digits.
+"""
+import numpy as np
+import pytest
+
+import golden_cases as GC
+from oracle import oracle as O
+
+
+def _close(a, b, rtol, atol=0.0):
+    return abs(float(a) - float(b)) <= atol + rtol * abs(float(b))
+
+
+@pytest.mark.parametrize("name", ["Poiseuille", "Couette"])
+@pytest.mark.parametrize("n", [1, 3])
+def test_viscous_decay_csv(name, n):
+    ref = [r for r in GC.read_csv("decay.csv") if r["flow"] == name and int(r["n"]) == n]
+    errors = []
+    for dt in (0.5, 0.25, 0.125):
+        rows = GC.decay_case(O, name, n, dt)
+        for row in rows:
+            match = [r for r in ref if float(r["dt"]) == dt and float(r["t"]) == row["t"]]
+            assert len(match) == 1
+            r = match[0]
+            assert _close(row["mu"], r["mu"], 1e-15)
+            assert _close(row["energy_ratio"], r["energy_ratio"], 1e-13), (row["energy_ratio"], r["energy_ratio"])
+            assert _close(row["error"], r["error"], 1e-6), (row["error"], r["error"])
+        errors.append(rows[-1]["error"])
+    assert 3.5 < errors[0] / errors[1] < 4.5
+    assert 3.5 < errors[1] / errors[2] < 4.5
+
+
+@pytest.mark.parametrize("branch", ["LB", "UB"])
+def test_waleffe_csv_35(branch):
+    _waleffe(branch, 35, 32)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("branch", ["LB", "UB"])
+def test_waleffe_csv_49(branch):
+    _waleffe(branch, 49, 48)
+
+
+def _waleffe(branch, Ny, N):
+    ref = [r for r in GC.read_csv("waleffe.csv") if r["branch"] == branch and int(r["Ny"]) == Ny]
+    rows = GC.waleffe_case(O, branch, Ny, N)
+    assert len(rows) == len(ref) == 4
+    for row, r in zip(rows, ref):
+        assert float(r["dt"]) == row["dt"] and float(r["t"]) == row["t"]
+        assert _close(row["energy_initial"], r["energy_initial"], 1e-13)
+        assert _close(row["defect"], r["defect"], 1e-8)
+        assert _close(row["energy"], r["energy"], 1e-13), (row["energy"], r["energy"])
+        assert _close(row["dissipation"], r["dissipation"], 1e-12), (row["dissipation"], r["dissipation"])
+        assert _close(row["input"], r["input"], 1e-12), (row["input"], r["input"])
+        assert _close(row["drift"], r["drift"], 1e-6), (row["drift"], r["drift"])
+
+
+@pytest.mark.slow
+def test_tollmien_schlichting_csv():
+    hist = GC.read_csv("ts_history.csv")
+    conv = GC.read_csv("ts_convergence.csv")
+    errors = []
+    for dt in (0.4, 0.2, 0.1):
+        rows, c, _ = GC.ts_case(O, dt)
+        for row in rows:
+            r = [h for h in hist if float(h["dt"]) == dt and float(h["t"]) == row["t"]][0]
+            assert _close(row["growth"], r["growth"], 1e-9)
+            assert _close(row["frequency"], r["frequency"], 1e-9)
+            assert _close(row["energy_ratio"], r["energy_ratio"], 1e-9)
+        rc = [x for x in conv if float(x["dt"]) == dt][0]
+        assert _close(c["eigenvalue_real"], rc["eigenvalue_real"], 1e-9)
+        assert _close(c["error"], rc["error"], 1e-6), (c["error"], rc["error"])
+        errors.append(c["error"])
+    assert errors[0] / errors[1] > 3.5 and errors[1] / errors[2] > 3.5 and errors[-1] < 1e-5
