@@ -1,0 +1,376 @@
+"""channelflow — MI355X (gfx950) device backend for ChannelFlow.jl's CNRK2 step.
+
+Python mirror of the reference's public surface for the hot path
+(src/grids.jl, src/problem.jl, src/state.jl, src/initialization.jl,
+src/timesteppers/cnrk2.jl): ``Grid``, ``ChannelFlowProblem``, ``CouetteFlow``,
+``PoiseuilleFlow``, ``State``, ``zero_state``, ``random_state``, ``step_``
+(Julia's ``step!``), ``project_``, ``pressure``, ``FFT``/``IFFT`` and the
+energy diagnostics.  Every numerical operation runs in libcfhip.so through
+its C ABI; torch provides device memory and streams only.
+
+Storage follows the reference exactly: a spectral field is a complex128
+device tensor of shape (Ny, Nz, Nxh) (= (k, l, n) column-major), a vector
+field stacks three of them, a padded physical field is a float64 tensor of
+shape (Ny, Nzp, Nxp) (= (x, z, y) column-major).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
+
+__all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State", "zero_state", "random_state",
+           "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
+           "require_device", "RotatingForm"]
+
+RotatingForm = "rotating"
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def require_device():
+    """Load the HIP library and check that a GPU is visible; raise otherwise."""
+    _lib.load()
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise DeviceError("no HIP device visible")
+    return True
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------------------
+# Grid (src/grids.jl:22-96)
+# ---------------------------------------------------------------------------
+class Grid:
+    def __init__(self, Nx, Ny, Nz, Lx, Lz):
+        self.Nx, self.Ny, self.Nz = int(Nx), int(Ny), int(Nz)
+        self.Lx, self.Lz = float(Lx), float(Lz)
+        self.y = np.cos(np.pi * np.arange(self.Ny) / (self.Ny - 1))
+        self.Nxp = (3 * self.Nx + 1) // 2
+        self.Nzp = (3 * self.Nz + 1) // 2
+        self.Nxh = self.Nx // 2 + 1
+
+    def __eq__(self, other):
+        return isinstance(other, Grid) and (self.Nx, self.Ny, self.Nz, self.Lx, self.Lz) == (
+            other.Nx, other.Ny, other.Nz, other.Lx, other.Lz)
+
+    def physicalsize(self, padded=True):
+        return (self.Nxp, self.Nzp, self.Ny) if padded else (self.Nx, self.Nz, self.Ny)
+
+    def spectralsize(self):
+        return (self.Nxh, self.Nz, self.Ny)
+
+    @property
+    def spec_shape(self):
+        return (self.Ny, self.Nz, self.Nxh)
+
+    @property
+    def phys_shape(self):
+        return (self.Ny, self.Nzp, self.Nxp)
+
+    def points(self):
+        """Padded coordinates broadcast to (y, z, x) storage order (grids.jl:89-96)."""
+        x = self.Lx * np.arange(self.Nxp) / self.Nxp
+        z = self.Lz * np.arange(self.Nzp) / self.Nzp
+        return self.y[:, None, None], z[None, :, None], x[None, None, :]
+
+    def zeros_spec(self, n=None, device="cuda"):
+        torch = _torch()
+        shape = self.spec_shape if n is None else (n,) + self.spec_shape
+        return torch.zeros(shape, dtype=torch.complex128, device=device)
+
+    def zeros_phys(self, n=None, device="cuda"):
+        torch = _torch()
+        shape = self.phys_shape if n is None else (n,) + self.phys_shape
+        return torch.zeros(shape, dtype=torch.float64, device=device)
+
+
+# ---------------------------------------------------------------------------
+# State (src/state.jl)
+# ---------------------------------------------------------------------------
+class State:
+    def __init__(self, velocity, stagepressure):
+        self.velocity = velocity
+        self.stagepressure = stagepressure
+
+    def copy(self):
+        return State(self.velocity.clone(), self.stagepressure.clone())
+
+
+def velocity(state):
+    return state.velocity
+
+
+def stagepressure(state):
+    return state.stagepressure
+
+
+def zero_state(grid):
+    """zero_state (initialization.jl:16)."""
+    return State(grid.zeros_spec(3), grid.zeros_spec())
+
+
+# ---------------------------------------------------------------------------
+# Problem, scheme and nonlinear term (src/problem.jl, cnrk2.jl, nonlinear.jl)
+# ---------------------------------------------------------------------------
+class CNRK2:
+    """Device CNRK2 scheme: owns the libcfhip problem handle (factored
+    stage solvers, plans, workspaces)."""
+
+    def __init__(self, grid, baseflow_values, nu, dt, form, constraint, device=0):
+        require_device()
+        L = _lib.load()
+        cfg = _lib.cf_config()
+        cfg.Nx, cfg.Ny, cfg.Nz = grid.Nx, grid.Ny, grid.Nz
+        cfg.Lx, cfg.Lz = grid.Lx, grid.Lz
+        cfg.nu, cfg.dt = float(nu), float(dt)
+        self._base = np.ascontiguousarray(baseflow_values, dtype=np.float64)
+        if self._base.shape != (grid.Ny,):
+            raise DimensionMismatch("baseflow must have one value per wall-normal point")
+        cfg.baseflow = self._base.ctypes.data_as(C.POINTER(C.c_double))
+        cfg.form = _lib.FORMS[form]
+        kind, vals = constraint
+        cfg.constraint = kind
+        cfg.constraint_values[0], cfg.constraint_values[1] = vals
+        cfg.device = int(device)
+        h = C.c_void_p()
+        _lib.check(L.cf_problem_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.grid, self.nu, self.dt = grid, float(nu), float(dt)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            _lib.load().cf_problem_destroy(h)
+            self.h = None
+
+
+class NonLinearTerm:
+    """Rotational-form nonlinear operator N = FFT(V x curl V) (nonlinear.jl:173-193)."""
+
+    def __init__(self, scheme):
+        self.scheme = scheme
+
+    def __call__(self, t, U, N=None, add=False):
+        g = self.scheme.grid
+        out = g.zeros_spec(3) if (N is None or add) else N
+        _check_vec(g, U)
+        _check_vec(g, out)
+        _lib.check(_lib.load().cf_nonlinear(self.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(out[0]),
+                                            _ptr(out[1]), _ptr(out[2]), _stream()))
+        if N is not None and add:
+            N += out
+            return N
+        return out
+
+
+def _constraint(pressuregradient, bulkvelocity):
+    if pressuregradient is not None and bulkvelocity is not None:
+        raise ArgumentError("specify either pressuregradient or bulkvelocity")
+    if bulkvelocity is not None:
+        vals = tuple(float(v) for v in bulkvelocity)
+        kind = 1
+    else:
+        vals = tuple(float(v) for v in (pressuregradient if pressuregradient is not None else (0.0, 0.0)))
+        kind = 0
+    if not all(math.isfinite(v) for v in vals):
+        raise ArgumentError("mean-flow constraint values must be finite")
+    return kind, vals
+
+
+class ChannelFlowProblem:
+    """ChannelFlowProblem(grid, profile, nu, dt; form, forcing, pressuregradient|bulkvelocity)
+    (src/problem.jl:69-107)."""
+
+    def __init__(self, grid, profile, nu, dt, form=RotatingForm, forcing=None, pressuregradient=None,
+                 bulkvelocity=None, device=0):
+        if forcing is not None:
+            raise NotImplementedError("custom forcing is not available on the device path")
+        kind, vals = _constraint(pressuregradient, bulkvelocity)
+        self.grid = grid
+        self.baseflow = np.asarray(profile(grid.y) if callable(profile) else profile, dtype=np.float64)
+        self.baseflow = np.broadcast_to(self.baseflow, (grid.Ny,)).copy()
+        self.scheme = CNRK2(grid, self.baseflow, nu, dt, form, (kind, vals), device)
+        self.nlterm = NonLinearTerm(self.scheme)
+        self.forcing = None
+        self.constraint = {"bulkvelocity": vals} if kind == 1 else {"pressuregradient": vals}
+        self.form = form
+
+
+def CouetteFlow(grid, nu, dt, **kw):
+    """Base velocity Ub(y) = y (problem.jl:122-123)."""
+    return ChannelFlowProblem(grid, lambda y: y, nu, dt, **kw)
+
+
+def PoiseuilleFlow(grid, nu, dt, bulkvelocity=None, pressuregradient="default", **kw):
+    """Base velocity Ub(y) = 1 - y^2, default pressure gradient (-2 nu, 0) (problem.jl:137-144)."""
+    if isinstance(pressuregradient, str):
+        pressuregradient = None if bulkvelocity is not None else (-2.0 * nu, 0.0)
+    return ChannelFlowProblem(grid, lambda y: 1.0 - y ** 2, nu, dt, bulkvelocity=bulkvelocity,
+                              pressuregradient=pressuregradient, **kw)
+
+
+# ---------------------------------------------------------------------------
+# step! (cnrk2.jl:149-207)
+# ---------------------------------------------------------------------------
+def _check_spec(g, f):
+    torch = _torch()
+    if f.dtype != torch.complex128 or tuple(f.shape) != g.spec_shape:
+        raise DimensionMismatch("fields must have the resolved spectral size")
+    if not f.is_cuda or not f.is_contiguous():
+        raise ArgumentError("fields must be contiguous device tensors")
+
+
+def _check_vec(g, U):
+    if len(U) != 3:
+        raise DimensionMismatch("vector field must have three components")
+    for c in range(3):
+        _check_spec(g, U[c])
+
+
+def step_(scheme, nlterm, U, P, t, forcing=None, pressuregradient=None, bulkvelocity=None, nsteps=1, stream=None):
+    """Advance (U, P) in place by scheme.dt (nsteps times); return the new time."""
+    if forcing is not None:
+        raise NotImplementedError("custom forcing is not available on the device path")
+    if not isinstance(nlterm, NonLinearTerm) or nlterm.scheme is not scheme:
+        raise ArgumentError("nlterm must be the problem's nonlinear term")
+    kind, vals = _constraint(pressuregradient, bulkvelocity)
+    g = scheme.grid
+    _check_vec(g, U)
+    _check_spec(g, P)
+    arr = (C.c_double * 2)(*vals)
+    tout = C.c_double()
+    _lib.check(_lib.load().cf_advance(scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), float(t), int(nsteps),
+                                      kind, arr, C.byref(tout), _stream(stream)))
+    return tout.value
+
+
+def last_gradient(scheme):
+    out = (C.c_double * 2)()
+    _lib.check(_lib.load().cf_last_gradient(scheme.h, out, _stream()))
+    return (out[0], out[1])
+
+
+# ---------------------------------------------------------------------------
+# transforms, projection, pressure, initial states
+# ---------------------------------------------------------------------------
+def FFT(problem, u):
+    """Padded physical field(s) -> resolved spectrum (ffts.jl:65-82, 198-207)."""
+    g = problem.grid
+    torch = _torch()
+    single = u.dim() == 3
+    uu = u.unsqueeze(0) if single else u
+    out = g.zeros_spec(uu.shape[0])
+    for c in range(uu.shape[0]):
+        src = uu[c].contiguous()
+        if src.dtype != torch.float64 or tuple(src.shape) != g.phys_shape:
+            raise DimensionMismatch("forward transform requires padded physical input")
+        _lib.check(_lib.load().cf_forward(problem.scheme.h, _ptr(src), _ptr(out[c]), _stream()))
+    return out[0] if single else out
+
+
+def IFFT(problem, U):
+    """Resolved spectrum -> padded physical field(s) (ffts.jl:151-166, 218-227)."""
+    g = problem.grid
+    single = U.dim() == 3
+    UU = U.unsqueeze(0) if single else U
+    out = g.zeros_phys(UU.shape[0])
+    for c in range(UU.shape[0]):
+        src = UU[c].contiguous()
+        _check_spec(g, src)
+        _lib.check(_lib.load().cf_inverse(problem.scheme.h, _ptr(src), _ptr(out[c]), _stream()))
+    return out[0] if single else out
+
+
+def stokes_solve(problem, nu, lam, R, U=None, P=None, pressuregradient=None, bulkvelocity=None):
+    """solve!(StokesSolver(grid, nu, lambda), U, P, R; ...) (stokes.jl:81-108).  Returns (U, P, gradients)."""
+    g = problem.grid
+    U = g.zeros_spec(3) if U is None else U
+    P = g.zeros_spec() if P is None else P
+    _check_vec(g, R)
+    if pressuregradient is None and bulkvelocity is None:
+        kind, vals = -1, (0.0, 0.0)
+    else:
+        kind, vals = _constraint(pressuregradient, bulkvelocity)
+    arr = (C.c_double * 2)(*vals)
+    grads = (C.c_double * 2)()
+    _lib.check(_lib.load().cf_stokes_solve(problem.scheme.h, float(nu), float(lam), _ptr(R[0]), _ptr(R[1]), _ptr(R[2]),
+                                           _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), kind, arr, grads, _stream()))
+    return U, P, (grads[0], grads[1])
+
+
+def project_(U, problem):
+    """project!(U, problem) (pressure.jl:117-135), in place; returns U."""
+    _check_vec(problem.grid, U)
+    _lib.check(_lib.load().cf_project(problem.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _stream()))
+    return U
+
+
+def pressure(U, problem, t=0.0):
+    """pressure(U, problem) (pressure.jl:176-185)."""
+    g = problem.grid
+    _check_vec(g, U)
+    P = g.zeros_spec()
+    _lib.check(_lib.load().cf_pressure(problem.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), _stream()))
+    return P
+
+
+def gaussian_noise(grid, epsilon, seed):
+    """epsilon * N(0,1) samples on the padded grid for components u, v, w in the
+    reference's evaluation order (x fastest, then z, then y; physicalfield.jl:39-43),
+    from numpy's PCG64 stream (Julia's randn stream is not reproducible here)."""
+    return epsilon * np.random.default_rng(seed).standard_normal((3,) + grid.phys_shape)
+
+
+def random_state(problem, epsilon, seed=42):
+    """random_state (initialization.jl:32-51): Gaussian padded field -> FFT ->
+    project! -> pressure."""
+    if epsilon < 0:
+        raise ArgumentError("epsilon must be non-negative")
+    torch = _torch()
+    noise = torch.as_tensor(gaussian_noise(problem.grid, epsilon, seed), device="cuda")
+    U = FFT(problem, noise)
+    project_(U, problem)
+    return State(U, pressure(U, problem))
+
+
+# ---------------------------------------------------------------------------
+# diagnostics (postprocessing.jl:21-95), torch reductions on the device
+# ---------------------------------------------------------------------------
+def _mass_matrix(Ny, device):
+    torch = _torch()
+    n = np.arange(Ny)
+    mom = lambda k: np.where(k % 2 == 0, 1.0 / (1.0 - k.astype(float) ** 2 + (k == 1)), 0.0)
+    M = (mom(n[:, None] + n[None, :]) + mom(np.abs(n[:, None] - n[None, :]))) / 2
+    return torch.as_tensor(M, device=device, dtype=torch.complex128)
+
+
+def spectral_dot(grid, U, V):
+    torch = _torch()
+    M = _mass_matrix(grid.Ny, U.device)
+    w = torch.full((grid.Nz, grid.Nxh), 2.0, dtype=torch.float64, device=U.device)
+    w[:, 0] = 1.0
+    if grid.Nx % 2 == 0:
+        w[:, grid.Nx // 2] = 0.0
+    if grid.Nz % 2 == 0:
+        w[grid.Nz // 2, :] = 0.0
+    MV = torch.einsum("mn,nzx->mzx", M, V)
+    return float((torch.conj(U) * MV).real.sum(0).mul(w).sum())
+
+
+def kinetic_energy(grid, U):
+    return sum(spectral_dot(grid, U[i], U[i]) for i in range(3)) / 2

@@ -1,0 +1,91 @@
+"""ctypes binding of libcfhip.so (C ABI in include/channelflow_hip.h).
+
+The library is the only compute path: loading fails loudly when it is
+missing, and there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libcfhip.so")
+
+CF_OK = 0
+CF_ERR_ARGUMENT = 1
+CF_ERR_DIMENSION = 2
+CF_ERR_DEVICE = 3
+CF_ERR_UNSUPPORTED = 4
+
+FORMS = {"rotating": 0, "convective": 1, "divergence": 2}
+
+
+class cf_config(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int),
+        ("Lx", C.c_double), ("Lz", C.c_double),
+        ("nu", C.c_double), ("dt", C.c_double),
+        ("baseflow", C.POINTER(C.c_double)),
+        ("form", C.c_int), ("constraint", C.c_int),
+        ("constraint_values", C.c_double * 2),
+        ("device", C.c_int),
+    ]
+
+
+_lib = None
+
+# symbols declared in include/channelflow_hip.h
+SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_problem_sizes", "cf_step", "cf_advance",
+           "cf_last_gradient", "cf_forward", "cf_inverse", "cf_nonlinear", "cf_stokes_solve", "cf_project",
+           "cf_pressure"]
+
+
+def load(path=LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    vp, d, i = C.c_void_p, C.c_double, C.c_int
+    dp = C.POINTER(C.c_double)
+    L.cf_last_error.restype = C.c_char_p
+    L.cf_problem_create.argtypes = [C.POINTER(cf_config), C.POINTER(vp)]
+    L.cf_problem_destroy.argtypes = [vp]
+    L.cf_problem_sizes.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.cf_step.argtypes = [vp, vp, vp, vp, vp, d, i, dp, dp, vp]
+    L.cf_advance.argtypes = [vp, vp, vp, vp, vp, d, i, i, dp, dp, vp]
+    L.cf_last_gradient.argtypes = [vp, dp, vp]
+    L.cf_forward.argtypes = [vp, vp, vp, vp]
+    L.cf_inverse.argtypes = [vp, vp, vp, vp]
+    L.cf_nonlinear.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.cf_stokes_solve.argtypes = [vp, d, d, vp, vp, vp, vp, vp, vp, vp, i, dp, dp, vp]
+    L.cf_project.argtypes = [vp, vp, vp, vp, vp]
+    L.cf_pressure.argtypes = [vp, vp, vp, vp, vp, vp]
+    for name in SYMBOLS:
+        getattr(L, name).restype = C.c_int if name != "cf_last_error" else C.c_char_p
+    _lib = L
+    return L
+
+
+class ArgumentError(ValueError):
+    """Mirror of Julia's ArgumentError."""
+
+
+class DimensionMismatch(ValueError):
+    """Mirror of Julia's DimensionMismatch."""
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc == CF_OK:
+        return
+    msg = load().cf_last_error().decode()
+    if rc == CF_ERR_ARGUMENT:
+        raise ArgumentError(msg)
+    if rc == CF_ERR_DIMENSION:
+        raise DimensionMismatch(msg)
+    if rc == CF_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise DeviceError(msg)

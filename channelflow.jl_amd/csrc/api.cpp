@@ -1,0 +1,618 @@
+// C ABI of the MI355X CNRK2 library (include/channelflow_hip.h).
+//
+// A cf_problem owns the factored solver banks of the three CNRK2 stages
+// (src/timesteppers/cnrk2.jl:58-83), the transform plans and the device
+// workspaces.  One step (cnrk2.jl:149-207) is 13 kernel launches:
+//   prologue (y):            U -> (V, Omega) at the Lobatto points
+//   3 x [ z inverse -> x pass (u x w) -> z forward -> y stage ]
+// where the y stage does the forward Chebyshev transform, Q update, RHS
+// assembly, Stokes solve and the inverse transforms for the next stage.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/channelflow_hip.h"
+#include "cf_host.h"
+#include "cf_kernels.h"
+
+using namespace cf;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPTRY(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) return fail(CF_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static const double CNRK2_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+static const double CNRK2_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+static const double CNRK2_C[3] = {1.0 / 6.0, 5.0 / 24.0, 1.0 / 8.0};
+
+struct DevBank {
+    double* facV = nullptr;
+    double* mconst = nullptr;
+    double* resp = nullptr;
+    double resp_mean = 0, lambda = 0, nu = 0;
+    Bank view() const {
+        Bank b;
+        b.facV = facV;
+        b.mconst = mconst;
+        b.resp = resp;
+        b.resp_mean = resp_mean;
+        b.lambda = lambda;
+        b.nu = nu;
+        return b;
+    }
+};
+
+struct cf_problem {
+    int device = 0;
+    GridDims gd{};
+    double Lx = 0, Lz = 0, nu = 0, dt = 0;
+    int form = 0, constraint = 0;
+    double cvals[2] = {0, 0};
+    std::vector<double> base, curv;
+    double base_mean = 0, curv_mean = 0;
+    // device data
+    std::vector<void*> allocs;
+    double* d_base = nullptr;
+    double* d_curv = nullptr;
+    double* d_facP = nullptr;
+    DevBank stage[3];
+    std::vector<std::pair<std::pair<double, double>, DevBank>> extra;
+    FFTPlan pz{}, py{}, px{};
+    double2* d_twx = nullptr;
+    double2* six = nullptr;   // 6 S
+    double2* zbuf = nullptr;  // 6 Sz
+    double2* Q = nullptr;     // 3 S
+    double2* R = nullptr;     // 3 S
+    double* d_grad = nullptr;
+    int KT = 1, zld = 0, LPB = 1, xld = 0, yld = 0;
+};
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+static int dalloc(cf_problem* h, T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 16);
+    if (e != hipSuccess) return fail(CF_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+template <typename T>
+static int upload(cf_problem* h, T** p, const std::vector<T>& v) {
+    if (int rc = dalloc(h, p, v.size())) return rc;
+    HIPTRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int make_plan(cf_problem* h, int n, FFTPlan& plan) {
+    std::vector<int> radix;
+    plan_factors(n, radix);
+    if ((int)radix.size() > MAX_FACTORS) return fail(CF_ERR_UNSUPPORTED, "FFT length has too many factors");
+    plan.n = n;
+    plan.nf = (int)radix.size();
+    for (int i = 0; i < plan.nf; ++i) plan.radix[i] = radix[i];
+    if (n == 1) plan.nf = 0;
+    std::vector<double> tw;
+    twiddles(n, tw);
+    double* d = nullptr;
+    if (int rc = upload(h, &d, tw)) return rc;
+    plan.tw = (const double2*)d;
+    return 0;
+}
+
+static int make_bank(cf_problem* h, double nu, double lambda, DevBank& db) {
+    HostBank hb;
+    std::string err;
+    if (!build_bank(h->gd, nu, lambda, hb, err)) return fail(CF_ERR_ARGUMENT, err);
+    if (int rc = upload(h, &db.facV, hb.facV)) return rc;
+    if (int rc = upload(h, &db.mconst, hb.mconst)) return rc;
+    if (int rc = upload(h, &db.resp, hb.resp)) return rc;
+    db.resp_mean = hb.resp_mean;
+    db.lambda = lambda;
+    db.nu = nu;
+    return 0;
+}
+
+static int get_bank(cf_problem* h, double nu, double lambda, DevBank** out) {
+    for (int j = 0; j < 3; ++j)
+        if (h->stage[j].nu == nu && h->stage[j].lambda == lambda) {
+            *out = &h->stage[j];
+            return 0;
+        }
+    for (auto& e : h->extra)
+        if (e.first.first == nu && e.first.second == lambda) {
+            *out = &e.second;
+            return 0;
+        }
+    DevBank db;
+    if (int rc = make_bank(h, nu, lambda, db)) return rc;
+    h->extra.push_back({{nu, lambda}, db});
+    *out = &h->extra.back().second;
+    return 0;
+}
+
+static void free_all(cf_problem* h) {
+    for (void* p : h->allocs) hipFree(p);
+    h->allocs.clear();
+}
+
+extern "C" const char* cf_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
+    if (!cfg || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (cfg->Ny % 2 == 0 || cfg->Ny < 5) return fail(CF_ERR_ARGUMENT, "Gibson's tau correction requires odd Ny >= 5");
+    if (!(cfg->Nx > 0 && cfg->Nz > 0)) return fail(CF_ERR_ARGUMENT, "Nx and Nz must be positive");
+    if (!(std::isfinite(cfg->Lx) && std::isfinite(cfg->Lz) && cfg->Lx > 0 && cfg->Lz > 0))
+        return fail(CF_ERR_ARGUMENT, "Lx and Lz must be finite and positive");
+    if (!(std::isfinite(cfg->dt) && cfg->dt > 0)) return fail(CF_ERR_ARGUMENT, "dt must be finite and positive");
+    if (!(std::isfinite(cfg->nu) && cfg->nu > 0)) return fail(CF_ERR_ARGUMENT, "nu must be finite and positive");
+    if (!cfg->baseflow) return fail(CF_ERR_DIMENSION, "baseflow must have one value per wall-normal point");
+    if (cfg->constraint != CF_CONSTRAINT_PRESSURE_GRADIENT && cfg->constraint != CF_CONSTRAINT_BULK_VELOCITY)
+        return fail(CF_ERR_ARGUMENT, "unknown mean-flow constraint");
+    if (!(std::isfinite(cfg->constraint_values[0]) && std::isfinite(cfg->constraint_values[1])))
+        return fail(CF_ERR_ARGUMENT, "mean-flow constraint values must be finite");
+    if (cfg->form != CF_FORM_ROTATING) return fail(CF_ERR_UNSUPPORTED, "only the rotational form runs on the device");
+    if (y_group_lanes(cfg->Ny) == 0) return fail(CF_ERR_UNSUPPORTED, "Ny above 255 is not supported");
+
+    cf_problem* h = new cf_problem();
+    h->device = cfg->device;
+    DeviceGuard dg(h->device);
+    GridDims& gd = h->gd;
+    gd.Nx = cfg->Nx;
+    gd.Ny = cfg->Ny;
+    gd.Nz = cfg->Nz;
+    gd.Nxh = cfg->Nx / 2 + 1;
+    gd.Nxp = (3 * cfg->Nx + 1) / 2;
+    gd.Nzp = (3 * cfg->Nz + 1) / 2;
+    gd.P = cfg->Ny - 1;
+    gd.B = (long long)gd.Nxh * gd.Nz;
+    gd.S = gd.B * gd.Ny;
+    gd.Sz = (long long)gd.Nxh * gd.Nzp * gd.Ny;
+    gd.alpha = 2.0 * M_PI / cfg->Lx;
+    gd.beta = 2.0 * M_PI / cfg->Lz;
+    h->Lx = cfg->Lx;
+    h->Lz = cfg->Lz;
+    h->nu = cfg->nu;
+    h->dt = cfg->dt;
+    h->form = cfg->form;
+    h->constraint = cfg->constraint;
+    h->cvals[0] = cfg->constraint_values[0];
+    h->cvals[1] = cfg->constraint_values[1];
+    int Ny = gd.Ny;
+    for (int j = 0; j < Ny; ++j)
+        if (!std::isfinite(cfg->baseflow[j])) {
+            delete h;
+            return fail(CF_ERR_ARGUMENT, "baseflow values must be finite");
+        }
+    h->base.resize(Ny);
+    h->curv.resize(Ny);
+    host_chebcoeffs(Ny, cfg->baseflow, h->base.data());  // cnrk2.jl:76
+    std::vector<double> d1(Ny);
+    host_diff(d1.data(), h->base.data(), Ny);  // diff2!, cnrk2.jl:77
+    host_diff(h->curv.data(), d1.data(), Ny);
+    h->base_mean = host_bulkmean(h->base.data(), Ny);
+    h->curv_mean = host_bulkmean(h->curv.data(), Ny);
+
+    int rc = 0;
+    auto bail = [&](int code) {
+        free_all(h);
+        delete h;
+        return code;
+    };
+    if ((rc = upload(h, &h->d_base, h->base))) return bail(rc);
+    if ((rc = upload(h, &h->d_curv, h->curv))) return bail(rc);
+    std::vector<double> facP;
+    build_pressure_factors(gd, facP);
+    if ((rc = upload(h, &h->d_facP, facP))) return bail(rc);
+    for (int j = 0; j < 3; ++j)
+        if ((rc = make_bank(h, h->nu, 1.0 / (CNRK2_C[j] * h->dt), h->stage[j]))) return bail(rc);
+    // plans
+    if ((rc = make_plan(h, gd.Nzp, h->pz))) return bail(rc);
+    if ((rc = make_plan(h, 2 * gd.P, h->py))) return bail(rc);
+    int M = (gd.Nxp % 2 == 0) ? gd.Nxp / 2 : gd.Nxp;
+    if ((rc = make_plan(h, M, h->px))) return bail(rc);
+    {
+        std::vector<double> tw;
+        twiddles(gd.Nxp, tw);
+        double* d = nullptr;
+        if ((rc = upload(h, &d, tw))) return bail(rc);
+        h->d_twx = (double2*)d;
+    }
+    // launch geometry (LDS <= 64 KB per block)
+    const size_t LDS_MAX = 64 * 1024;
+    h->yld = 2 * gd.P;
+    h->zld = gd.Nzp + 1;
+    h->KT = (int)std::max<size_t>(1, std::min<size_t>(16, LDS_MAX / (2 * (size_t)h->zld * sizeof(double2))));
+    h->KT = std::min(h->KT, gd.Nxh);
+    h->xld = M + 1;
+    {
+        size_t per_line = 2 * (size_t)h->xld * sizeof(double2) + 6 * (size_t)gd.Nxp * sizeof(double);
+        h->LPB = (int)std::max<size_t>(1, std::min<size_t>(8, LDS_MAX / per_line));
+    }
+    if (y_lds_bytes(gd.Ny, h->yld) > LDS_MAX) return bail(fail(CF_ERR_UNSUPPORTED, "wall-normal tile exceeds LDS"));
+    // workspaces
+    if ((rc = dalloc(h, &h->six, 6 * (size_t)gd.S))) return bail(rc);
+    if ((rc = dalloc(h, &h->zbuf, 6 * (size_t)gd.Sz))) return bail(rc);
+    if ((rc = dalloc(h, &h->Q, 3 * (size_t)gd.S))) return bail(rc);
+    if ((rc = dalloc(h, &h->R, 3 * (size_t)gd.S))) return bail(rc);
+    if ((rc = dalloc(h, &h->d_grad, 2))) return bail(rc);
+    if (hipMemset(h->d_grad, 0, 2 * sizeof(double)) != hipSuccess) return bail(fail(CF_ERR_DEVICE, "hipMemset"));
+    *out = h;
+    return CF_OK;
+}
+
+extern "C" int cf_problem_destroy(cf_problem* h) {
+    if (!h) return CF_OK;
+    DeviceGuard dg(h->device);
+    hipDeviceSynchronize();
+    free_all(h);
+    delete h;
+    return CF_OK;
+}
+
+extern "C" int cf_problem_sizes(const cf_problem* h, long long out[8]) {
+    if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    out[0] = h->gd.Nxh;
+    out[1] = h->gd.Nz;
+    out[2] = h->gd.Ny;
+    out[3] = h->gd.Nxp;
+    out[4] = h->gd.Nzp;
+    out[5] = h->gd.Ny;
+    out[6] = h->gd.S;
+    out[7] = (long long)h->gd.Nxp * h->gd.Nzp * h->gd.Ny;
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static YArgs base_yargs(cf_problem* h) {
+    YArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.gd = h->gd;
+    a.dct = h->py;
+    a.ld = h->yld;
+    a.norm = 1.0 / ((double)h->gd.Nxp * h->gd.Nzp * h->gd.P);
+    a.nu = h->nu;
+    a.base = h->d_base;
+    a.curv = h->d_curv;
+    a.curv_mean = h->curv_mean;
+    a.facP = h->d_facP;
+    a.facPN = h->d_facP;
+    return a;
+}
+
+static double2* fld(double2* base, long long n, int i) { return base + (size_t)n * i; }
+
+static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* const* out, hipStream_t st) {
+    ZArgs z;
+    std::memset(&z, 0, sizeof(z));
+    z.gd = h->gd;
+    z.plan = h->pz;
+    z.KT = h->KT;
+    z.ld = h->zld;
+    z.nfields = nf;
+    for (int i = 0; i < nf; ++i) {
+        z.in[i] = in[i];
+        z.out[i] = out[i];
+    }
+    HIPTRY(launch_zpass(dir, z, st));
+    return 0;
+}
+
+static XArgs base_xargs(cf_problem* h) {
+    XArgs x;
+    std::memset(&x, 0, sizeof(x));
+    x.gd = h->gd;
+    x.plan = h->px;
+    x.twx = h->d_twx;
+    x.LPB = h->LPB;
+    x.ld = h->xld;
+    return x;
+}
+
+// y-physical (V, Omega) in six -> y-physical nonlinear term in six[0..2]
+static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st) {
+    const GridDims& gd = h->gd;
+    double2* six[6];
+    double2* zb[6];
+    for (int i = 0; i < 6; ++i) {
+        six[i] = fld(h->six, gd.S, i);
+        zb[i] = fld(h->zbuf, gd.Sz, i);
+    }
+    if (int rc = zpass(h, +1, 6, six, zb, st)) return rc;
+    XArgs x = base_xargs(h);
+    x.mode = X_ROTATIONAL;
+    for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
+    for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a block reads all six lines first
+    HIPTRY(launch_xpass(x, st));
+    if (int rc = zpass(h, -1, 3, zb, six, st)) return rc;
+    return 0;
+}
+
+static int enqueue_prologue(cf_problem* h, double2* const U[3], hipStream_t st) {
+    YArgs a = base_yargs(h);
+    for (int i = 0; i < 3; ++i) a.U[i] = U[i];
+    for (int i = 0; i < 6; ++i) a.out6[i] = fld(h->six, h->gd.S, i);
+    HIPTRY(launch_ykernel(Y_PROLOGUE, a, st));
+    return 0;
+}
+
+struct Constraint {
+    int bulk;
+    double c0, c1;
+};
+
+static int resolve_constraint(cf_problem* h, int kind, const double* values, Constraint& c) {
+    if (kind == -1) {
+        kind = h->constraint;
+        values = h->cvals;
+    }
+    if (kind != CF_CONSTRAINT_PRESSURE_GRADIENT && kind != CF_CONSTRAINT_BULK_VELOCITY)
+        return fail(CF_ERR_ARGUMENT, "specify either pressuregradient or bulkvelocity");
+    double v0 = values ? values[0] : 0.0, v1 = values ? values[1] : 0.0;
+    if (!(std::isfinite(v0) && std::isfinite(v1))) return fail(CF_ERR_ARGUMENT, "mean-flow constraint values must be finite");
+    c.bulk = (kind == CF_CONSTRAINT_BULK_VELOCITY);
+    // StokesSolver: bulk target is the perturbation mean (stokes.jl:96-99)
+    c.c0 = c.bulk ? v0 - h->base_mean : v0;
+    c.c1 = v1;
+    return 0;
+}
+
+static int check_fields(const void* const* f, int n) {
+    for (int i = 0; i < n; ++i) {
+        if (!f[i]) return fail(CF_ERR_ARGUMENT, "null field pointer");
+        for (int j = 0; j < i; ++j)
+            if (f[i] == f[j]) return fail(CF_ERR_ARGUMENT, "fields must have distinct storage");
+    }
+    return 0;
+}
+
+static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st) {
+    const GridDims& gd = h->gd;
+    for (int step = 0; step < nsteps; ++step) {
+        if (int rc = enqueue_prologue(h, U, st)) return rc;
+        for (int j = 0; j < 3; ++j) {
+            if (int rc = enqueue_nonlinear_phys(h, st)) return rc;
+            YArgs a = base_yargs(h);
+            a.stage = j;
+            a.store_Q = (j < 2);
+            a.lambda = 1.0 / (CNRK2_C[j] * h->dt);
+            a.Aj = CNRK2_A[j];
+            a.weight = CNRK2_B[j] / CNRK2_C[j];
+            a.bulk = c.bulk;
+            a.c0 = c.c0;
+            a.c1 = c.c1;
+            a.bank = h->stage[j].view();
+            for (int i = 0; i < 3; ++i) {
+                a.Nphys[i] = fld(h->six, gd.S, i);
+                a.U[i] = U[i];
+                a.Q[i] = fld(h->Q, gd.S, i);
+            }
+            a.P = P;
+            a.grad_out = h->d_grad;
+            if (j < 2)
+                for (int i = 0; i < 6; ++i) a.out6[i] = fld(h->six, gd.S, i);
+            HIPTRY(launch_ykernel(Y_STAGE, a, st));
+        }
+    }
+    return 0;
+}
+
+extern "C" int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int nsteps,
+                          int constraint, const double* values, double* t_out, void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[4] = {U0, U1, U2, P};
+    if (int rc = check_fields(f, 4)) return rc;
+    if (nsteps < 0) return fail(CF_ERR_ARGUMENT, "nsteps must be nonnegative");
+    if (!std::isfinite(t)) return fail(CF_ERR_ARGUMENT, "t must be finite");
+    Constraint c;
+    if (int rc = resolve_constraint(h, constraint, values, c)) return rc;
+    DeviceGuard dg(h->device);
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    if (int rc = enqueue_steps(h, U, (double2*)P, nsteps, c, (hipStream_t)stream)) return rc;
+    if (t_out) *t_out = t + nsteps * h->dt;
+    return CF_OK;
+}
+
+extern "C" int cf_step(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int constraint,
+                       const double* values, double* t_out, void* stream) {
+    return cf_advance(h, U0, U1, U2, P, t, 1, constraint, values, t_out, stream);
+}
+
+extern "C" int cf_last_gradient(cf_problem* h, double out[2], void* stream) {
+    if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    DeviceGuard dg(h->device);
+    HIPTRY(hipMemcpyAsync(out, h->d_grad, 2 * sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIPTRY(hipStreamSynchronize((hipStream_t)stream));
+    return CF_OK;
+}
+
+extern "C" int cf_forward(cf_problem* h, const double* u, double* U, void* stream) {
+    if (!h || !u || !U) return fail(CF_ERR_ARGUMENT, "null argument");
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    const GridDims& gd = h->gd;
+    XArgs x = base_xargs(h);
+    x.mode = X_FORWARD;
+    x.nfields = 1;
+    x.rin[0] = u;
+    x.out[0] = h->zbuf;
+    HIPTRY(launch_xpass(x, st));
+    double2* zi[1] = {h->zbuf};
+    double2* zo[1] = {h->six};
+    if (int rc = zpass(h, -1, 1, zi, zo, st)) return rc;
+    YArgs a = base_yargs(h);
+    a.nfields = 1;
+    a.Nphys[0] = h->six;
+    a.Rout[0] = (double2*)U;
+    HIPTRY(launch_ykernel(Y_DCT_FWD, a, st));
+    (void)gd;
+    return CF_OK;
+}
+
+extern "C" int cf_inverse(cf_problem* h, const double* U, double* u, void* stream) {
+    if (!h || !u || !U) return fail(CF_ERR_ARGUMENT, "null argument");
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    YArgs a = base_yargs(h);
+    a.nfields = 1;
+    a.Rin[0] = (const double2*)U;
+    a.out6[0] = h->six;
+    HIPTRY(launch_ykernel(Y_DCT_INV, a, st));
+    double2* zi[1] = {h->six};
+    double2* zo[1] = {h->zbuf};
+    if (int rc = zpass(h, +1, 1, zi, zo, st)) return rc;
+    XArgs x = base_xargs(h);
+    x.mode = X_INVERSE;
+    x.nfields = 1;
+    x.in[0] = h->zbuf;
+    x.rout[0] = u;
+    HIPTRY(launch_xpass(x, st));
+    return CF_OK;
+}
+
+// nonlinear term of U into spectral N (3 fields)
+static int enqueue_nonlinear_spec(cf_problem* h, double2* const U[3], double2* const N[3], hipStream_t st) {
+    if (int rc = enqueue_prologue(h, U, st)) return rc;
+    if (int rc = enqueue_nonlinear_phys(h, st)) return rc;
+    YArgs a = base_yargs(h);
+    a.nfields = 3;
+    for (int i = 0; i < 3; ++i) {
+        a.Nphys[i] = fld(h->six, h->gd.S, i);
+        a.Rout[i] = N[i];
+    }
+    HIPTRY(launch_ykernel(Y_DCT_FWD, a, st));
+    return 0;
+}
+
+extern "C" int cf_nonlinear(cf_problem* h, const double* U0, const double* U1, const double* U2, double* N0, double* N1,
+                            double* N2, void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[6] = {U0, U1, U2, N0, N1, N2};
+    if (int rc = check_fields(f, 6)) return rc;
+    DeviceGuard dg(h->device);
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    double2* N[3] = {(double2*)N0, (double2*)N1, (double2*)N2};
+    return enqueue_nonlinear_spec(h, U, N, (hipStream_t)stream);
+}
+
+static int enqueue_solve(cf_problem* h, DevBank* bk, const double2* const R[3], double2* const U[3], double2* P,
+                         const Constraint& c, hipStream_t st) {
+    YArgs a = base_yargs(h);
+    a.bank = bk->view();
+    a.nu = bk->nu;
+    a.bulk = c.bulk;
+    a.c0 = c.c0;
+    a.c1 = c.c1;
+    for (int i = 0; i < 3; ++i) {
+        a.Rin[i] = R[i];
+        a.U[i] = U[i];
+    }
+    a.P = P;
+    a.grad_out = h->d_grad;
+    HIPTRY(launch_ykernel(Y_SOLVE, a, st));
+    return 0;
+}
+
+extern "C" int cf_stokes_solve(cf_problem* h, double nu, double lambda, const double* R0, const double* R1,
+                               const double* R2, double* U0, double* U1, double* U2, double* P, int constraint,
+                               const double* values, double* grads, void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[7] = {U0, U1, U2, P, R0, R1, R2};
+    if (int rc = check_fields(f, 7)) return rc;
+    if (!(std::isfinite(nu) && nu > 0)) return fail(CF_ERR_ARGUMENT, "nu must be finite and positive");
+    if (!(std::isfinite(lambda) && lambda >= 0)) return fail(CF_ERR_ARGUMENT, "lambda must be finite and nonnegative");
+    Constraint c;
+    if (constraint == -1) {
+        c.bulk = 0;
+        c.c0 = c.c1 = 0.0;
+    } else if (int rc = resolve_constraint(h, constraint, values, c)) {
+        return rc;
+    }
+    DeviceGuard dg(h->device);
+    DevBank* bk = nullptr;
+    if (int rc = get_bank(h, nu, lambda, &bk)) return rc;
+    const double2* R[3] = {(const double2*)R0, (const double2*)R1, (const double2*)R2};
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = enqueue_solve(h, bk, R, U, (double2*)P, c, st)) return rc;
+    if (grads) {
+        HIPTRY(hipMemcpyAsync(grads, h->d_grad, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPTRY(hipStreamSynchronize(st));
+    }
+    return CF_OK;
+}
+
+extern "C" int cf_project(cf_problem* h, double* U0, double* U1, double* U2, void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[3] = {U0, U1, U2};
+    if (int rc = check_fields(f, 3)) return rc;
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    DevBank* bk = nullptr;
+    if (int rc = get_bank(h, 1.0, 0.0, &bk)) return rc;
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    double2* R[3];
+    for (int i = 0; i < 3; ++i) R[i] = fld(h->R, h->gd.S, i);
+    YArgs a = base_yargs(h);
+    for (int i = 0; i < 3; ++i) {
+        a.U[i] = U[i];
+        a.Rout[i] = R[i];
+    }
+    HIPTRY(launch_ykernel(Y_NEGLAP, a, st));
+    Constraint c;
+    if (h->constraint == CF_CONSTRAINT_BULK_VELOCITY) {
+        if (int rc = resolve_constraint(h, CF_CONSTRAINT_BULK_VELOCITY, h->cvals, c)) return rc;
+    } else {
+        c.bulk = 0;
+        c.c0 = c.c1 = 0.0;
+    }
+    const double2* Rc[3] = {R[0], R[1], R[2]};
+    return enqueue_solve(h, bk, Rc, U, h->six, c, st);
+}
+
+extern "C" int cf_pressure(cf_problem* h, const double* U0, const double* U1, const double* U2, double* P,
+                           void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[4] = {U0, U1, U2, P};
+    if (int rc = check_fields(f, 4)) return rc;
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    double2* A[3];
+    for (int i = 0; i < 3; ++i) A[i] = fld(h->R, h->gd.S, i);
+    if (int rc = enqueue_nonlinear_spec(h, U, A, st)) return rc;
+    YArgs a = base_yargs(h);
+    for (int i = 0; i < 3; ++i) a.Rin[i] = A[i];
+    a.U[1] = U[1];
+    a.P = (double2*)P;
+    HIPTRY(launch_ykernel(Y_PRESSURE, a, st));
+    return CF_OK;
+}

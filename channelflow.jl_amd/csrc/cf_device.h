@@ -1,0 +1,458 @@
+// Device-side building blocks for the CNRK2 step on gfx950 (CDNA4, wave64).
+//
+//  * complex FP64 arithmetic on double2
+//  * lane-group cross-lane primitives (ds_bpermute) and scans used by the
+//    wall-normal (Chebyshev) recurrences: one mode is held by a group of L
+//    lanes, lane t owning coefficients n = 4t .. 4t+3
+//  * a batched Stockham autosort FFT staged in LDS (mixed radix 2/3/4/5/8 +
+//    generic odd radix) used by the x, z and Chebyshev transforms
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cf_common.h"
+
+namespace cf {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmk(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ double2 cconj(double2 a) { return cmk(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return cmk(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cmuli(double2 a) { return cmk(-a.y, a.x); }    // i*a
+__device__ __forceinline__ double2 cmulni(double2 a) { return cmk(a.y, -a.x); }   // -i*a
+__device__ __forceinline__ double2 cfma(double s, double2 a, double2 b) { return cmk(fma(s, a.x, b.x), fma(s, a.y, b.y)); }
+__device__ __forceinline__ double2 czero() { return cmk(0.0, 0.0); }
+
+// ---------------------------------------------------------------------------
+// cross-lane moves (ds_bpermute): value from absolute lane `src`
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bperm(double v, int src) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_ds_bpermute(src << 2, lo);
+    hi = __builtin_amdgcn_ds_bpermute(src << 2, hi);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double2 bperm(double2 v, int src) { return cmk(bperm(v.x, src), bperm(v.y, src)); }
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Group of L lanes (power of two, <= 64); t = index within group.
+template <int L>
+struct Group {
+    int lane, t, base;
+    __device__ Group() {
+        lane = lane_id();
+        t = lane & (L - 1);
+        base = lane - t;
+    }
+    // value from group lane t+d (or 0/identity if out of range: caller masks)
+    __device__ __forceinline__ int down(int d) const { return (t + d < L) ? lane + d : lane; }
+    __device__ __forceinline__ int up(int d) const { return (t >= d) ? lane - d : lane; }
+    __device__ __forceinline__ int at(int tt) const { return base + tt; }
+
+    __device__ __forceinline__ double sum(double v) const {
+#pragma unroll
+        for (int d = 1; d < L; d <<= 1) v += bperm(v, lane ^ d);
+        return v;
+    }
+    __device__ __forceinline__ double2 sum(double2 v) const {
+#pragma unroll
+        for (int d = 1; d < L; d <<= 1) {
+            int s = lane ^ d;
+            v.x += bperm(v.x, s);
+            v.y += bperm(v.y, s);
+        }
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Chebyshev coefficient vectors in the blocked group layout: lane t holds
+// v[e] = a_{4t+e}.  Entries above degree P must be zero.
+// ---------------------------------------------------------------------------
+struct CVec {
+    double2 v[4];
+};
+
+// d/dy (backward recurrence b_n = b_{n+2} + 2(n+1) a_{n+1}, b_0 halved;
+// _batchdiff!, batchedinfluence.jl:32-43) as two parity suffix sums of 2m a_m.
+template <int L>
+__device__ __forceinline__ CVec cheb_deriv(const Group<L>& g, const CVec& a) {
+    const int n0 = 4 * g.t;
+    double2 c[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) c[e] = cscale(a.v[e], 2.0 * (n0 + e));
+    double2 E = cadd(c[0], c[2]), O = cadd(c[1], c[3]);
+    // inclusive suffix sums over lanes of E and O
+    double2 sE = E, sO = O;
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+        int src = g.down(d);
+        bool ok = (g.t + d < L);
+        double2 tE = bperm(sE, src), tO = bperm(sO, src);
+        if (ok) { sE = cadd(sE, tE); sO = cadd(sO, tO); }
+    }
+    // exclusive
+    sE = csub(sE, E);
+    sO = csub(sO, O);
+    CVec b;
+    b.v[3] = sE;
+    b.v[2] = cadd(c[3], sO);
+    b.v[1] = cadd(c[2], sE);
+    b.v[0] = cadd(cadd(c[1], c[3]), sO);
+    if (g.t == 0) b.v[0] = cscale(b.v[0], 0.5);
+    return b;
+}
+
+// Wall values / derivatives / bulk mean as group reductions.
+template <int L>
+__device__ __forceinline__ double2 wall_deriv_plus(const Group<L>& g, const CVec& a) {  // sum n^2 a_n
+    double2 s = czero();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        double n = 4 * g.t + e;
+        s = cfma(n * n, a.v[e], s);
+    }
+    return g.sum(s);
+}
+template <int L>
+__device__ __forceinline__ double2 wall_deriv_minus(const Group<L>& g, const CVec& a) {  // sum (-1)^(n+1) n^2 a_n
+    double2 s = czero();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        double n = 4 * g.t + e;
+        s = cfma(((e & 1) ? 1.0 : -1.0) * n * n, a.v[e], s);
+    }
+    return g.sum(s);
+}
+template <int L>
+__device__ __forceinline__ double2 bulk_mean(const Group<L>& g, const CVec& a) {  // a_0 + sum_{even m>=2} a_m/(1-m^2)
+    double2 s = czero();
+#pragma unroll
+    for (int e = 0; e < 4; e += 2) {
+        double n = 4 * g.t + e;
+        s = cfma(1.0 / (1.0 - n * n), a.v[e], s);
+    }
+    return g.sum(s);
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev-tau Helmholtz solve (t0 D^2 - t1) a = f on rows 0..P-2 plus wall
+// rows, by the quasi-tridiagonal even/odd elimination.  `fac` are this mode's
+// elimination factors s_j (host-computed), indexed by degree n = 2j + parity.
+// Data are the parity wall data: Dirichlet even = (g+ + g-)/2, odd =
+// (g+ - g-)/2; Neumann even = (g+ - g-)/2, odd = (g+ + g-)/2.
+// Both parities are processed together: parity p lives in elements p, p+2.
+// ---------------------------------------------------------------------------
+struct HelmCoef {
+    double lo, up;  // row coefficients for unknown j (analytic in n, t1)
+};
+__device__ __forceinline__ double cfac_(int m) { return m == 0 ? 2.0 : 1.0; }
+
+template <int L>
+__device__ __forceinline__ CVec helm_solve(const Group<L>& g, int P, double t1, const double fac[4], bool neumann,
+                                           const CVec& f, double2 data_even, double2 data_odd) {
+    const int n0 = 4 * g.t;
+    // neighbouring coefficients of f for the 3-term tau stencil
+    double2 fm2[4], fp2[4];
+    {
+        int up1 = g.up(1), dn1 = g.down(1);
+        double2 a2 = bperm(f.v[2], up1), a3 = bperm(f.v[3], up1);
+        double2 b0 = bperm(f.v[0], dn1), b1 = bperm(f.v[1], dn1);
+        bool hasprev = g.t > 0, hasnext = g.t + 1 < L;
+        fm2[0] = hasprev ? a2 : czero();
+        fm2[1] = hasprev ? a3 : czero();
+        fm2[2] = f.v[0];
+        fm2[3] = f.v[1];
+        fp2[0] = f.v[2];
+        fp2[1] = f.v[3];
+        fp2[2] = hasnext ? b0 : czero();
+        fp2[3] = hasnext ? b1 : czero();
+    }
+    // per element: elimination coefficients; A = s*rhs, T = -up*s (backward), alpha = -lo*s
+    double T[4], alpha[4];
+    double2 A[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int n = n0 + e;
+        double nn = (double)n;
+        bool row = (n >= 2) && (n <= P);
+        double s = row ? fac[e] : 0.0;
+        double en2 = (n + 2 <= P - 2) ? 1.0 : 0.0;
+        double lo = -t1 * cfac_(n - 2) / (4.0 * nn * (nn - 1.0));
+        double up = -t1 * en2 / (4.0 * nn * (nn + 1.0));
+        double2 Fm = (n - 2 >= 0 && n - 2 <= P - 2) ? fm2[e] : czero();
+        double2 F0 = (n <= P - 2) ? f.v[e] : czero();
+        double2 Fp = (n + 2 <= P - 2) ? fp2[e] : czero();
+        double2 rhs = czero();
+        if (row) {
+            rhs = cscale(Fm, cfac_(n - 2) / (4.0 * nn * (nn - 1.0)));
+            rhs = cfma(-1.0 / (2.0 * (nn * nn - 1.0)), F0, rhs);
+            rhs = cfma(1.0 / (4.0 * nn * (nn + 1.0)), Fp, rhs);
+        }
+        A[e] = cscale(rhs, s);
+        T[e] = row ? -up * s : 0.0;
+        alpha[e] = row ? -lo * s : (n < 2 ? 1.0 : 0.0);  // j=0: identity (gamma_0 = 1)
+    }
+    // ---- backward sweep: beta_j = A_j + T_j beta_{j+1}, per parity ----
+    // lane composite over its two elements of each parity: j=2t (e=p), 2t+1 (e=p+2)
+    double cT[2];
+    double2 cA[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        cT[p] = T[p] * T[p + 2];
+        cA[p] = cfma(T[p], A[p + 2], A[p]);
+    }
+    // inclusive suffix scan of composites (f_t o f_{t+1} o ...)
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+        int src = g.down(d);
+        bool ok = g.t + d < L;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            double oT = bperm(cT[p], src);
+            double2 oA = bperm(cA[p], src);
+            if (ok) {
+                cA[p] = cfma(cT[p], oA, cA[p]);
+                cT[p] = cT[p] * oT;
+            }
+        }
+    }
+    // carry = beta at first j of next lane = A-part of inclusive scan at lane t+1
+    double2 beta[4];
+    {
+        int src = g.down(1);
+        bool ok = g.t + 1 < L;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            double2 carry = bperm(cA[p], src);
+            if (!ok) carry = czero();
+            beta[p + 2] = cfma(T[p + 2], carry, A[p + 2]);
+            beta[p] = cfma(T[p], beta[p + 2], A[p]);
+        }
+    }
+    // ---- forward sweep: delta_j = beta_j + alpha_j delta_{j-1}, gamma_j = prod alpha ----
+    double fT[2];
+    double2 fA[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        // composite g_{2t+1} o g_{2t}: x -> beta1 + alpha1 (beta0 + alpha0 x)
+        fT[p] = alpha[p + 2] * alpha[p];
+        fA[p] = cfma(alpha[p + 2], beta[p], beta[p + 2]);
+    }
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+        int src = g.up(d);
+        bool ok = g.t >= d;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            double oT = bperm(fT[p], src);
+            double2 oA = bperm(fA[p], src);
+            if (ok) {  // (later) o (earlier): (T, A) o (oT, oA) = (T oT, A + T oA)
+                fA[p] = cfma(fT[p], oA, fA[p]);
+                fT[p] = fT[p] * oT;
+            }
+        }
+    }
+    double2 dl[4];
+    double gm[4];
+    {
+        int src = g.up(1);
+        bool ok = g.t >= 1;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            double2 carryA = bperm(fA[p], src);
+            double carryT = bperm(fT[p], src);
+            if (!ok) { carryA = czero(); carryT = 1.0; }
+            dl[p] = cfma(alpha[p], carryA, beta[p]);
+            gm[p] = alpha[p] * carryT;
+            dl[p + 2] = cfma(alpha[p + 2], dl[p], beta[p + 2]);
+            gm[p + 2] = alpha[p + 2] * gm[p];
+        }
+    }
+    // ---- wall row: x0 = (data - sum w delta) / sum w gamma ----
+    double2 swd[2] = {czero(), czero()};
+    double swg[2] = {0.0, 0.0};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int n = n0 + e;
+        double w = (n <= P) ? (neumann ? (double)n * n : 1.0) : 0.0;
+        swd[e & 1] = cfma(w, dl[e], swd[e & 1]);
+        swg[e & 1] += w * gm[e];
+    }
+    CVec out;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        double2 s1 = g.sum(swd[p]);
+        double s2 = g.sum(swg[p]);
+        double2 dat = p == 0 ? data_even : data_odd;
+        double2 x0 = cscale(csub(dat, s1), 1.0 / s2);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            int e = p + 2 * q;
+            int n = n0 + e;
+            out.v[e] = (n <= P) ? cfma(gm[e], x0, dl[e]) : czero();
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Stockham FFT in LDS.
+// Lines of length n at stride `ld` (complex elements) in buffers a, b; the
+// result ends in *a or *b (returned).  sign = -1 forward, +1 backward.
+// tw[t] = exp(-2 pi i t / n), t < n (global memory, read-only).
+// ---------------------------------------------------------------------------
+template <int R>
+__device__ __forceinline__ void dft_small(double2* v, int sign) {
+    if constexpr (R == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const double c = -0.5, s = 0.86602540378443864676;  // cos(2pi/3), sin(2pi/3)
+        double2 a = v[0], b = v[1], d = v[2];
+        double2 t = cadd(b, d), u = csub(b, d);
+        double2 m = cfma(c, t, a);
+        double2 su = cscale(u, sign * s);  // forward: v1 = m - i s u
+        double2 isu = cmuli(su);
+        v[0] = cadd(a, t);
+        v[1] = cadd(m, isu);
+        v[2] = csub(m, isu);
+    } else if constexpr (R == 4) {
+        double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        double2 t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+        double2 jt3 = (sign < 0) ? cmulni(t3) : cmuli(t3);
+        v[0] = cadd(t0, t2);
+        v[1] = cadd(t1, jt3);
+        v[2] = csub(t0, t2);
+        v[3] = csub(t1, jt3);
+    } else if constexpr (R == 5) {
+        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+        double2 a = v[0];
+        double2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+        double2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+        double2 m1 = cadd(a, cadd(cscale(b1, c1), cscale(b2, c2)));
+        double2 m2 = cadd(a, cadd(cscale(b1, c2), cscale(b2, c1)));
+        // forward: -i*(s1 d1 + s2 d2), -i*(s2 d1 - s1 d2)
+        double2 n1 = cadd(cscale(d1, s1), cscale(d2, s2));
+        double2 n2 = csub(cscale(d1, s2), cscale(d2, s1));
+        double2 j1 = (sign < 0) ? cmulni(n1) : cmuli(n1);
+        double2 j2 = (sign < 0) ? cmulni(n2) : cmuli(n2);
+        v[0] = cadd(a, cadd(b1, b2));
+        v[1] = cadd(m1, j1);
+        v[4] = csub(m1, j1);
+        v[2] = cadd(m2, j2);
+        v[3] = csub(m2, j2);
+    } else if constexpr (R == 8) {
+        const double h = 0.70710678118654752440;
+        // radix-8 = 2 x radix-4 with twiddles
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        dft_small<4>(e, sign);
+        dft_small<4>(o, sign);
+        // twiddles W8^k, k=0..3 (forward W8 = exp(-i pi/4))
+        double2 w1 = cmk(h, sign < 0 ? -h : h);
+        double2 w2 = cmk(0.0, sign < 0 ? -1.0 : 1.0);
+        double2 w3 = cmk(-h, sign < 0 ? -h : h);
+        o[1] = cmul(o[1], w1);
+        o[2] = cmul(o[2], w2);
+        o[3] = cmul(o[3], w3);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void stockham_pass_fixed(const double2* src, double2* dst, int n, int nlines, int ld, int Ns,
+                                                    const double2* tw, int sign, int tid, int nthr) {
+    const int m = n / R;
+    const int twstep = n / (Ns * R);
+    const int total = nlines * m;
+    for (int idx = tid; idx < total; idx += nthr) {
+        int line = idx / m;
+        int j = idx - line * m;
+        const double2* s = src + (size_t)line * ld;
+        double2* d = dst + (size_t)line * ld;
+        int jm = j % Ns;
+        double2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double2 x = s[j + r * m];
+            if (r > 0) {
+                double2 w = tw[(r * jm * twstep) % n];
+                if (sign > 0) w = cconj(w);
+                x = cmul(x, w);
+            }
+            v[r] = x;
+        }
+        dft_small<R>(v, sign);
+        int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[idxD + r * Ns] = v[r];
+    }
+}
+
+// generic odd radix (rare sizes): O(R^2) per butterfly, reads LDS directly
+__device__ __forceinline__ void stockham_pass_generic(const double2* src, double2* dst, int n, int R, int nlines, int ld,
+                                                      int Ns, const double2* tw, int sign, int tid, int nthr) {
+    const int m = n / R;
+    const int twstep = n / (Ns * R);
+    const int total = nlines * m;
+    for (int idx = tid; idx < total; idx += nthr) {
+        int line = idx / m;
+        int j = idx - line * m;
+        const double2* s = src + (size_t)line * ld;
+        double2* d = dst + (size_t)line * ld;
+        int jm = j % Ns;
+        int idxD = (j / Ns) * Ns * R + jm;
+        for (int q = 0; q < R; ++q) {
+            double2 acc = czero();
+            for (int r = 0; r < R; ++r) {
+                double2 x = s[j + r * m];
+                // twiddle (r * jm / (Ns R)) and DFT_R phase (r q / R), combined:
+                // exp(-2 pi i r (jm + q Ns) / (Ns R))
+                int t = (int)(((long long)r * (jm + (long long)q * Ns) * twstep) % n);
+                double2 w = tw[t];
+                if (sign > 0) w = cconj(w);
+                acc = cadd(acc, cmul(x, w));
+            }
+            d[idxD + q * Ns] = acc;
+        }
+    }
+}
+
+// Run all passes; returns the buffer holding the result.  Must be called by
+// all threads of the block (contains __syncthreads).
+__device__ inline double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
+                                   int nthr) {
+    int Ns = 1;
+    double2* src = a;
+    double2* dst = b;
+    for (int f = 0; f < plan.nf; ++f) {
+        int R = plan.radix[f];
+        switch (R) {
+            case 2: stockham_pass_fixed<2>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+            case 3: stockham_pass_fixed<3>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+            case 4: stockham_pass_fixed<4>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+            case 5: stockham_pass_fixed<5>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+            case 8: stockham_pass_fixed<8>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+            default: stockham_pass_generic(src, dst, plan.n, R, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+        }
+        __syncthreads();
+        Ns *= R;
+        double2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+}  // namespace cf

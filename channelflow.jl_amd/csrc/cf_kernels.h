@@ -1,0 +1,78 @@
+// Kernel argument blocks and launchers (host-visible, no torch types).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cf_common.h"
+
+namespace cf {
+
+// Per-bank (one implicit shift lambda, one viscosity) solver constants.
+struct Bank {
+    const double* facV;    // (B, Ny) elimination factors of nu D^2 - (lambda + nu kappa^2)
+    const double* mconst;  // (B, MC_COUNT) influence/tau constants
+    const double* resp;    // (Ny) mean-mode unit-gradient response c
+    double resp_mean;      // bulk mean of c
+    double lambda, nu;
+};
+
+enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_INV };
+
+struct YArgs {
+    GridDims gd;
+    FFTPlan dct;  // length 2P
+    int ld;       // LDS line stride (complex elements)
+    double norm;  // forward normalisation 1/(Nxp Nzp P)
+    // CNRK2 stage
+    int stage, store_Q;
+    double lambda, nu, Aj, weight;
+    // mean-flow constraint: bulk=1 -> (c0, c1) perturbation bulk targets;
+    // bulk=0 -> prescribed uniform pressure gradient
+    int bulk;
+    double c0, c1;
+    const double* base;  // Chebyshev coefficients of Ub (Ny)
+    const double* curv;  // Ub'' coefficients (Ny)
+    double curv_mean;    // bulk mean of Ub''
+    const double* facP;  // (B, Ny) factors of D^2 - kappa^2 (Dirichlet; kappa^2=1 placeholder at (0,0))
+    const double* facPN; // (B, Ny) factors of D^2 - kappa^2 with Neumann rows (pressure init)
+    Bank bank;
+    const double2* Nphys[3];  // y-physical nonlinear term
+    double2* U[3];
+    double2* P;
+    double2* Q[3];
+    double2* out6[6];  // y-physical outputs (V, Omega) or inverse transforms
+    const double2* Rin[3];
+    double2* Rout[3];
+    double* grad_out;  // 2 doubles (device) or null
+    int nfields;
+};
+
+struct ZArgs {
+    GridDims gd;
+    FFTPlan plan;  // length Nzp
+    int KT, ld, nfields;
+    const double2* in[6];
+    double2* out[6];
+};
+
+enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD };
+
+struct XArgs {
+    GridDims gd;
+    FFTPlan plan;         // length Nxp/2 (even Nxp) or Nxp (odd)
+    const double2* twx;   // exp(-2 pi i t / Nxp), t < Nxp
+    int LPB, ld, mode, nfields;
+    const double2* in[6];  // (k, zp, y) complex, Nxh per line
+    double2* out[3];
+    const double* rin[3];  // (x, zp, y) real
+    double* rout[6];
+};
+
+hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st);
+int y_group_lanes(int Ny);
+size_t y_lds_bytes(int Ny, int ld);
+hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st);
+size_t z_lds_bytes(const ZArgs& a);
+hipError_t launch_xpass(const XArgs& a, hipStream_t st);
+size_t x_lds_bytes(const XArgs& a);
+
+}  // namespace cf

@@ -1,0 +1,594 @@
+// Wall-normal (Chebyshev) pencil kernels.
+//
+// A block owns M = NT/L consecutive Fourier modes s = ix + Nxh*iz of the
+// reference (k,l,n) layout; a group of L lanes owns one mode, lane t holding
+// Chebyshev coefficients n = 4t..4t+3 in registers.  HBM rows (fixed n, M
+// consecutive modes) are moved through LDS, which also stages the DCT-I
+// (REDFT00) as a length-2P Stockham FFT of the even extension.
+//
+// Reference: CNRK2 stage (src/timesteppers/cnrk2.jl:170-205), Stokes solve
+// (src/solvers/stokes.jl:81-108, batchedinfluence.jl:129-159, meanmode.jl:103-150),
+// Chebyshev transforms (src/ffts.jl:65-82,151-166, transforms/chebyshev.jl:45-66),
+// curl (src/fields/operators.jl:172-190), pressure (src/pressure.jl:196-285).
+#include "cf_device.h"
+#include "cf_kernels.h"
+
+namespace cf {
+
+constexpr int YNT = 256;  // threads per block of every y kernel
+
+struct ModeInfo {
+    long long s;
+    int ix, iz, l;
+    double kx, kz, kappa2;
+    bool valid, nyq, mean;
+};
+
+__device__ __forceinline__ ModeInfo mode_info(const GridDims& gd, long long s) {
+    ModeInfo mi;
+    mi.s = s;
+    mi.valid = s < gd.B;
+    long long ss = mi.valid ? s : 0;
+    mi.ix = (int)(ss % gd.Nxh);
+    mi.iz = (int)(ss / gd.Nxh);
+    mi.l = mi.iz <= (gd.Nz >> 1) ? mi.iz : mi.iz - gd.Nz;
+    mi.kx = gd.alpha * mi.ix;
+    mi.kz = gd.beta * mi.l;
+    mi.kappa2 = mi.kx * mi.kx + mi.kz * mi.kz;
+    mi.nyq = ((gd.Nx % 2 == 0) && mi.ix == gd.Nx / 2) || ((gd.Nz % 2 == 0) && mi.iz == gd.Nz / 2);
+    mi.mean = (ss == 0);
+    return mi;
+}
+
+// ---------------------------------------------------------------------------
+// Tile movement between HBM (k,l,n) rows and group registers via LDS.
+// ---------------------------------------------------------------------------
+template <int L>
+struct Tile {
+    static constexpr int M = YNT / L;
+    const GridDims& gd;
+    long long s0;
+    int tid, mloc, ld;
+    double2* buf;
+    __device__ Tile(const GridDims& g_, double2* b, int ld_) : gd(g_), ld(ld_), buf(b) {
+        tid = threadIdx.x;
+        mloc = tid / L;
+        s0 = (long long)blockIdx.x * M;
+    }
+    // spectral field rows n = 0..P -> registers (zero beyond P and for absent modes)
+    __device__ CVec load(const double2* __restrict__ src, const Group<L>& g) {
+        const int Ny = gd.Ny;
+        for (int idx = tid; idx < Ny * M; idx += YNT) {
+            int n = idx / M, m = idx - n * M;
+            long long s = s0 + m;
+            buf[m * ld + n] = (s < gd.B) ? src[s + gd.B * n] : czero();
+        }
+        __syncthreads();
+        CVec v;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            v.v[e] = (n < Ny) ? buf[mloc * ld + n] : czero();
+        }
+        __syncthreads();
+        return v;
+    }
+    __device__ void store(double2* __restrict__ dst, const Group<L>& g, const CVec& v) {
+        const int Ny = gd.Ny;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            if (n < Ny) buf[mloc * ld + n] = v.v[e];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < Ny * M; idx += YNT) {
+            int n = idx / M, m = idx - n * M;
+            long long s = s0 + m;
+            if (s < gd.B) dst[s + gd.B * n] = buf[m * ld + n];
+        }
+        __syncthreads();
+    }
+    // forward Chebyshev transform of y-physical rows (values at y_j, j=0..P):
+    // REDFT00, times norm, endpoint 1/2, Nyquist modes zeroed (normalize_forward!).
+    __device__ CVec dct_forward(const double2* __restrict__ src, double2* bufB, const FFTPlan& plan, double norm,
+                                const Group<L>& g, bool zero_mode) {
+        const int P = gd.P;
+        for (int idx = tid; idx < (P + 1) * M; idx += YNT) {
+            int n = idx / M, m = idx - n * M;
+            long long s = s0 + m;
+            double2 v = (s < gd.B) ? src[s + gd.B * n] : czero();
+            buf[m * ld + n] = v;
+            if (n >= 1 && n <= P - 1) buf[m * ld + 2 * P - n] = v;
+        }
+        __syncthreads();
+        double2* res = lds_fft(buf, bufB, plan, M, ld, -1, tid, YNT);
+        CVec v;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            double w = (n == 0 || n == P) ? 0.5 * norm : norm;
+            v.v[e] = (n <= P && !zero_mode) ? cscale(res[mloc * ld + n], w) : czero();
+        }
+        __syncthreads();
+        return v;
+    }
+    // inverse: coefficient registers -> values at y_j (endpoints doubled,
+    // REDFT00, x1/2: InverseFFT!'s chebyshev + copy_to_padded! scale).
+    __device__ void dct_inverse(double2* __restrict__ dst, double2* bufB, const FFTPlan& plan, const Group<L>& g,
+                                const CVec& a) {
+        const int P = gd.P;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            if (n <= P) {
+                double2 x = a.v[e];
+                buf[mloc * ld + n] = (n == 0 || n == P) ? cscale(x, 2.0) : x;
+                if (n >= 1 && n <= P - 1) buf[mloc * ld + 2 * P - n] = x;
+            }
+        }
+        __syncthreads();
+        double2* res = lds_fft(buf, bufB, plan, M, ld, -1, tid, YNT);
+        for (int idx = tid; idx < (P + 1) * M; idx += YNT) {
+            int n = idx / M, m = idx - n * M;
+            long long s = s0 + m;
+            if (s < gd.B) dst[s + gd.B * n] = cscale(res[m * ld + n], 0.5);
+        }
+        __syncthreads();
+    }
+};
+
+// coefficient of degree n (broadcast to the group)
+template <int L>
+__device__ __forceinline__ double2 coef_at(const Group<L>& g, const CVec& a, int n) {
+    double2 v = czero();
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        if (4 * g.t + e == n) v = a.v[e];
+    return g.sum(v);
+}
+
+template <int L>
+__device__ __forceinline__ void load_fac(const double* __restrict__ f, long long s, int Ny, const Group<L>& g,
+                                         double out[4]) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int n = 4 * g.t + e;
+        out[e] = (n < Ny) ? f[s * Ny + n] : 0.0;
+    }
+}
+
+// V = U + Ub e_x at (0,0); Omega = curl V (operators.jl:172-190); values
+// of the six fields at the Lobatto points into out6.
+template <int L>
+__device__ void inverse_six(Tile<L>& tile, double2* bufB, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
+                            const CVec U[3]) {
+    CVec V[3] = {U[0], U[1], U[2]};
+    if (mi.mean && mi.valid) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            if (n < a.gd.Ny) V[0].v[e] = cadd(V[0].v[e], cmk(a.base[n], 0.0));
+        }
+    }
+    tile.dct_inverse(a.out6[0], bufB, a.dct, g, V[0]);
+    tile.dct_inverse(a.out6[1], bufB, a.dct, g, V[1]);
+    tile.dct_inverse(a.out6[2], bufB, a.dct, g, V[2]);
+    CVec d3 = cheb_deriv(g, V[2]);
+    CVec W;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V[1].v[e], mi.kz)));  // dy V3 - i l V2
+    tile.dct_inverse(a.out6[3], bufB, a.dct, g, W);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)  // i (l V1 - k V3)
+        W.v[e] = cmuli(csub(cscale(V[0].v[e], mi.kz), cscale(V[2].v[e], mi.kx)));
+    tile.dct_inverse(a.out6[4], bufB, a.dct, g, W);
+    CVec d1 = cheb_deriv(g, V[0]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) W.v[e] = csub(cmuli(cscale(V[1].v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
+    tile.dct_inverse(a.out6[5], bufB, a.dct, g, W);
+}
+
+// ---------------------------------------------------------------------------
+// Stokes solve of one mode given R (re-solve form of the influence + tau
+// correction; see the derivation in the library documentation).  Returns
+// U[3], P.  mean/nyq modes handled separately by the caller.
+// ---------------------------------------------------------------------------
+template <int L>
+__device__ void solve_nonzero(const Group<L>& g, const YArgs& a, const Bank& bk, const ModeInfo& mi, const CVec R[3],
+                              CVec Uout[3], CVec& Pout) {
+    const int P = a.gd.P, Ny = a.gd.Ny;
+    double fp[4], fv[4];
+    load_fac(a.facP, mi.s, Ny, g, fp);
+    load_fac(bk.facV, mi.s, Ny, g, fv);
+    const double* mc = bk.mconst + mi.s * MC_COUNT;
+    const double shift = bk.lambda + bk.nu * mi.kappa2;
+    // r = D Ry + i k Rx + i l Rz
+    CVec r = cheb_deriv(g, R[1]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        r.v[e] = cadd(r.v[e], cadd(cmuli(cscale(R[0].v[e], mi.kx)), cmuli(cscale(R[2].v[e], mi.kz))));
+    CVec p1 = helm_solve(g, P, mi.kappa2, fp, false, r, czero(), czero());
+    CVec q = cheb_deriv(g, p1);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R[1].v[e]);
+    CVec v1 = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+    double2 bp = cscale(wall_deriv_plus(g, v1), -1.0);
+    double2 bm = cscale(wall_deriv_minus(g, v1), -1.0);
+    double2 dp = cadd(cscale(bp, mc[MC_A11]), cscale(bm, mc[MC_A12]));
+    double2 dm = cadd(cscale(bp, mc[MC_A21]), cscale(bm, mc[MC_A22]));
+    double2 p1P = coef_at(g, p1, P);
+    double2 v1a = coef_at(g, v1, P - 1), v1b = coef_at(g, v1, P);
+    double2 ra = coef_at(g, R[1], P - 1), rb = coef_at(g, R[1], P);
+    double2 p2P = cadd(p1P, cadd(cscale(dp, mc[MC_PPT]), cscale(dm, mc[MC_PMT])));
+    double2 v2a = cadd(v1a, cadd(cscale(dp, mc[MC_VP1]), cscale(dm, mc[MC_VM1])));
+    double2 v2b = cadd(v1b, cadd(cscale(dp, mc[MC_VP0]), cscale(dm, mc[MC_VM0])));
+    double2 s1 = cscale(csub(cadd(cscale(v2a, shift), cscale(p2P, 2.0 * P)), ra), 1.0 / (1.0 - mc[MC_SIG1]));
+    double2 s2 = cscale(csub(cscale(v2b, shift), rb), 1.0 / (1.0 - mc[MC_SIG2]));
+    // final pressure: rhs r + s1 D T_{P-1} + s2 D T_P, wall data per parity
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        int n = 4 * g.t + e;
+        if (n % 2 == 0) {
+            if (n < P - 1) r.v[e] = cfma(n == 0 ? (double)(P - 1) : 2.0 * (P - 1), s1, r.v[e]);
+        } else {
+            if (n < P) r.v[e] = cfma(2.0 * P, s2, r.v[e]);
+        }
+    }
+    double2 de = cadd(cscale(cadd(dp, dm), 0.5), cscale(s1, 0.5 * (mc[MC_D0P] + mc[MC_D0M])));
+    double2 dd = cadd(cscale(csub(dp, dm), 0.5), cscale(s2, 0.5 * (mc[MC_D0P] - mc[MC_D0M])));
+    Pout = helm_solve(g, P, mi.kappa2, fp, false, r, de, dd);
+    q = cheb_deriv(g, Pout);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R[1].v[e]);
+    Uout[1] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kx)), R[0].v[e]);
+    Uout[0] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kz)), R[2].v[e]);
+    Uout[2] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+}
+
+// mean mode (meanmode.jl:103-150); returns the new gradients
+template <int L>
+__device__ void solve_mean(const Group<L>& g, const YArgs& a, const Bank& bk, const CVec R[3], CVec Uout[3], CVec& Pout,
+                           double gout[2]) {
+    const int P = a.gd.P, Ny = a.gd.Ny;
+    // pressure: integrate Ry through degree P-1, zero bulk mean
+    {
+        int up1 = g.up(1), dn1 = g.down(1);
+        double2 prev3 = bperm(R[1].v[3], up1);
+        double2 next0 = bperm(R[1].v[0], dn1);
+        CVec r = R[1];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            double2 rm = (e == 0) ? prev3 : r.v[e - 1];  // r_{n-1}
+            double2 rp = (e == 3) ? next0 : r.v[e + 1];  // r_{n+1}
+            double2 lower = (n == 1) ? rm : cscale(rm, 0.5);
+            double2 upper = (n + 1 < P) ? cscale(rp, 0.5) : czero();
+            Pout.v[e] = (n >= 1 && n <= P) ? cscale(csub(lower, upper), 1.0 / n) : czero();
+        }
+        double2 m = bulk_mean(g, Pout);  // p_0 = 0 at this point
+        if (g.t == 0) Pout.v[0] = cscale(m, -1.0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Uout[1].v[e] = czero();
+    double fv[4];
+    load_fac(bk.facV, 0, Ny, g, fv);
+    for (int c = 0; c < 2; ++c) {
+        const CVec& src = R[c == 0 ? 0 : 2];
+        CVec f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) f.v[e] = cscale(src.v[e], -1.0);
+        CVec u = helm_solve(g, P, bk.lambda, fv, false, f, czero(), czero());
+        double grad;
+        if (a.bulk) {
+            double target = c == 0 ? a.c0 : a.c1;
+            grad = (target - bulk_mean(g, u).x) / bk.resp_mean;
+        } else {
+            grad = c == 0 ? a.c0 : a.c1;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            if (n < Ny) u.v[e] = cadd(u.v[e], cmk(grad * bk.resp[n], 0.0));
+        }
+        Uout[c == 0 ? 0 : 2] = u;
+        gout[c] = grad;
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void zero_vec(CVec& v) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v.v[e] = czero();
+}
+
+// Laplacian D^2 a - kappa^2 a
+template <int L>
+__device__ __forceinline__ CVec laplacian(const Group<L>& g, const CVec& a, double kappa2) {
+    CVec d = cheb_deriv(g, cheb_deriv(g, a));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) d.v[e] = cfma(-kappa2, a.v[e], d.v[e]);
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Full CNRK2 stage: forward DCT of the nonlinear term, Q update, RHS,
+// Stokes solve, state write, inverse transforms for the next stage.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    const int ld = a.ld;
+    double2* bufA = smem;
+    double2* bufB = smem + (Tile<L>::M) * ld;
+    Tile<L> tile(a.gd, bufA, ld);
+    Group<L> g;
+    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    const Bank& bk = a.bank;
+    const int Ny = a.gd.Ny;
+
+    CVec N[3];
+    for (int c = 0; c < 3; ++c) N[c] = tile.dct_forward(a.Nphys[c], bufB, a.dct, a.norm, g, mi.nyq);
+
+    // explicit-history update Q = N (stage 1) or A_j Q + N
+    CVec Q[3];
+    for (int c = 0; c < 3; ++c) {
+        if (a.stage == 0) {
+            Q[c] = N[c];
+        } else {
+            Q[c] = tile.load(a.Q[c], g);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Q[c].v[e] = cfma(a.Aj, Q[c].v[e], N[c].v[e]);
+        }
+        if (a.store_Q) tile.store(a.Q[c], g, Q[c]);
+    }
+    // old uniform gradient (bulk: _bulkpressuregradient, cnrk2.jl:98-107)
+    double gold[2] = {a.c0, a.c1};
+    CVec Pold = tile.load(a.P, g);
+    CVec R[3];
+    for (int c = 0; c < 3; ++c) {
+        CVec U = tile.load(a.U[c], g);
+        if (a.bulk && mi.mean && mi.valid && c != 1) {
+            double2 sh = csub(wall_deriv_plus(g, U), wall_deriv_minus(g, U));
+            double base = (c == 0) ? a.nu * a.curv_mean : 0.0;
+            gold[c == 0 ? 0 : 1] = a.nu * sh.x / 2.0 + base + bulk_mean(g, N[c]).x;
+        }
+        CVec lap = laplacian(g, U, mi.kappa2);
+        CVec dP;
+        if (c == 1) {
+            dP = cheb_deriv(g, Pold);
+        } else {
+            double kk = (c == 0) ? mi.kx : mi.kz;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dP.v[e] = cmuli(cscale(Pold.v[e], kk));
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            double2 x = cscale(U.v[e], a.lambda);
+            x = cfma(a.nu, lap.v[e], x);
+            x = csub(x, dP.v[e]);
+            R[c].v[e] = cfma(a.weight, Q[c].v[e], x);
+        }
+    }
+    if (mi.mean && mi.valid) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            if (n < Ny) R[0].v[e] = cadd(R[0].v[e], cmk(2.0 * a.nu * a.curv[n], 0.0));
+        }
+        if (g.t == 0) {
+            R[0].v[0] = csub(R[0].v[0], cmk(gold[0], 0.0));
+            R[2].v[0] = csub(R[2].v[0], cmk(gold[1], 0.0));
+        }
+    }
+    CVec Un[3], Pn;
+    double gnew[2] = {0.0, 0.0};
+    if (mi.mean && mi.valid) {
+        solve_mean(g, a, bk, R, Un, Pn, gnew);
+        if (g.t == 0 && a.grad_out) {
+            a.grad_out[0] = gnew[0];
+            a.grad_out[1] = gnew[1];
+        }
+    } else if (mi.nyq || !mi.valid) {
+        for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
+        zero_vec<L>(Pn);
+    } else {
+        solve_nonzero(g, a, bk, mi, R, Un, Pn);
+    }
+    for (int c = 0; c < 3; ++c) tile.store(a.U[c], g, Un[c]);
+    tile.store(a.P, g, Pn);
+    if (a.out6[0]) inverse_six(tile, bufB, a, g, mi, Un);
+}
+
+// prologue: state U -> six y-physical fields (velocity and vorticity)
+template <int L>
+__global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    const int ld = a.ld;
+    double2* bufA = smem;
+    double2* bufB = smem + (Tile<L>::M) * ld;
+    Tile<L> tile(a.gd, bufA, ld);
+    Group<L> g;
+    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    CVec U[3];
+    for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
+    inverse_six(tile, bufB, a, g, mi, U);
+}
+
+// Stokes solve! of a bank: R (3 spectral) -> U (3), P (stokes.jl:81-108)
+template <int L>
+__global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    const int ld = a.ld;
+    double2* bufA = smem;
+    Tile<L> tile(a.gd, bufA, ld);
+    Group<L> g;
+    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    CVec R[3];
+    for (int c = 0; c < 3; ++c) R[c] = tile.load(a.Rin[c], g);
+    CVec Un[3], Pn;
+    double gnew[2] = {0.0, 0.0};
+    if (mi.mean && mi.valid) {
+        solve_mean(g, a, a.bank, R, Un, Pn, gnew);
+        if (g.t == 0 && a.grad_out) {
+            a.grad_out[0] = gnew[0];
+            a.grad_out[1] = gnew[1];
+        }
+    } else if (mi.nyq || !mi.valid) {
+        for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
+        zero_vec<L>(Pn);
+    } else {
+        solve_nonzero(g, a, a.bank, mi, R, Un, Pn);
+    }
+    for (int c = 0; c < 3; ++c) tile.store(a.U[c], g, Un[c]);
+    tile.store(a.P, g, Pn);
+}
+
+// R = -Laplacian(U) (projection source, pressure.jl:125)
+template <int L>
+__global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    Tile<L> tile(a.gd, smem, a.ld);
+    Group<L> g;
+    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    for (int c = 0; c < 3; ++c) {
+        CVec U = tile.load(a.U[c], g);
+        CVec lap = laplacian(g, U, mi.kappa2);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) lap.v[e] = cscale(lap.v[e], -1.0);
+        tile.store(a.Rout[c], g, lap);
+    }
+}
+
+// pressure reconstruction (_pressure_poisson, pressure.jl:196-285):
+// A (3 spectral accelerations), v = U[1] -> P
+template <int L>
+__global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    Tile<L> tile(a.gd, smem, a.ld);
+    Group<L> g;
+    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    const int P = a.gd.P, Ny = a.gd.Ny;
+    CVec A[3];
+    for (int c = 0; c < 3; ++c) A[c] = tile.load(a.Rin[c], g);
+    CVec v = tile.load(a.U[1], g);
+    CVec Pn;
+    if (mi.mean && mi.valid) {
+        int up1 = g.up(1), dn1 = g.down(1);
+        double2 prev3 = bperm(A[1].v[3], up1);
+        double2 next0 = bperm(A[1].v[0], dn1);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            double2 rm = (e == 0) ? prev3 : A[1].v[e - 1];
+            double2 rp = (e == 3) ? next0 : A[1].v[e + 1];
+            double2 lower = (n == 1) ? rm : cscale(rm, 0.5);
+            double2 upper = (n + 1 < Ny - 1) ? cscale(rp, 0.5) : czero();
+            Pn.v[e] = (n >= 1 && n <= P) ? cscale(csub(lower, upper), 1.0 / n) : czero();
+        }
+        double2 m = bulk_mean(g, Pn);
+        if (g.t == 0) Pn.v[0] = cscale(m, -1.0);
+    } else if (mi.nyq || !mi.valid) {
+        zero_vec<L>(Pn);
+    } else {
+        // div A
+        CVec r = cheb_deriv(g, A[1]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            r.v[e] = cadd(r.v[e], cadd(cmuli(cscale(A[0].v[e], mi.kx)), cmuli(cscale(A[2].v[e], mi.kz))));
+        // wall data P_y(+-1) = A_y(+-1) + nu v_yy(+-1)
+        double2 up = czero(), lo = czero();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            if (n < Ny) {
+                double nn = n;
+                double2 term = cfma(a.nu * (nn * nn * (nn * nn - 1.0) / 3.0), v.v[e], A[1].v[e]);
+                up = cadd(up, term);
+                lo = (n % 2 == 0) ? cadd(lo, term) : csub(lo, term);
+            }
+        }
+        up = g.sum(up);
+        lo = g.sum(lo);
+        double fp[4];
+        load_fac(a.facPN, mi.s, Ny, g, fp);
+        // Neumann parity data: even (g+ - g-)/2, odd (g+ + g-)/2
+        Pn = helm_solve(g, P, mi.kappa2, fp, true, r, cscale(csub(up, lo), 0.5), cscale(cadd(up, lo), 0.5));
+    }
+    tile.store(a.P, g, Pn);
+}
+
+// Batched Chebyshev transforms of spectral fields (no group logic).
+// dir < 0: y-physical values -> normalised coefficients (normalize_forward!)
+// dir > 0: coefficients -> values at the Lobatto points
+template <int L>
+__global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
+    extern __shared__ double2 smem[];
+    const int ld = a.ld;
+    double2* bufA = smem;
+    double2* bufB = smem + (Tile<L>::M) * ld;
+    Tile<L> tile(a.gd, bufA, ld);
+    Group<L> g;
+    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    for (int c = 0; c < a.nfields; ++c) {
+        if (dir < 0) {
+            CVec v = tile.dct_forward(a.Nphys[c], bufB, a.dct, a.norm, g, mi.nyq);
+            tile.store(a.Rout[c], g, v);
+        } else {
+            CVec v = tile.load(a.Rin[c], g);
+            tile.dct_inverse(a.out6[c], bufB, a.dct, g, v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <int L>
+static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
+    constexpr int M = YNT / L;
+    dim3 grid((unsigned)((a.gd.B + M - 1) / M));
+    size_t lds = 2 * (size_t)M * a.ld * sizeof(double2);
+    switch (which) {
+        case Y_STAGE: hipLaunchKernelGGL(ystage_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_PROLOGUE: hipLaunchKernelGGL(yprologue_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_SOLVE: hipLaunchKernelGGL(ysolve_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_NEGLAP: hipLaunchKernelGGL(yneglap_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_PRESSURE: hipLaunchKernelGGL(ypressure_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_DCT_FWD: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, -1); break;
+        case Y_DCT_INV: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, +1); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int y_group_lanes(int Ny) {
+    int need = (Ny + 3) / 4;
+    if (need <= 4) return 4;
+    if (need <= 8) return 8;
+    if (need <= 16) return 16;
+    if (need <= 32) return 32;
+    if (need <= 64) return 64;
+    return 0;
+}
+
+size_t y_lds_bytes(int Ny, int ld) {
+    int L = y_group_lanes(Ny);
+    return L ? 2 * (size_t)(YNT / L) * ld * sizeof(double2) : 0;
+}
+
+hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
+    switch (y_group_lanes(a.gd.Ny)) {
+        case 4: return launch_y<4>(which, a, st);
+        case 8: return launch_y<8>(which, a, st);
+        case 16: return launch_y<16>(which, a, st);
+        case 32: return launch_y<32>(which, a, st);
+        case 64: return launch_y<64>(which, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cf

@@ -1,0 +1,115 @@
+/*
+ * channelflow_hip.h — C ABI of the MI355X (gfx950) CNRK2 step library
+ * (libcfhip.so).  Drop-in device backend for ChannelFlow.jl's hot path:
+ * the three-stage CNRK2 `step!` of plane Couette/Poiseuille flow.
+ *
+ * Conventions (identical to the reference's storage):
+ *   - spectral field: (k, l, n) column-major, k fastest, interleaved
+ *     complex128, size Nxh*Nz*Ny with Nxh = Nx/2+1
+ *     (src/fields/spectralfield.jl:16-29, src/grids.jl:71-74)
+ *   - padded physical field: (x, z, y) column-major float64, size
+ *     Nxp*Nzp*Ny, Nxp = ceil(3Nx/2), Nzp = ceil(3Nz/2)
+ *     (src/fields/physicalfield.jl:13-57, src/grids.jl:49-63)
+ *   - all field pointers are DEVICE pointers; `stream` is a hipStream_t
+ *     (NULL = default stream).  Calls are asynchronous on that stream
+ *     unless stated otherwise.
+ *   - status: 0 = success; CF_ERR_ARGUMENT mirrors Julia's ArgumentError,
+ *     CF_ERR_DIMENSION DimensionMismatch; all validation happens before any
+ *     work is enqueued; cf_last_error() describes the last failure.
+ *   - a problem handle is not thread-safe (one trajectory per handle;
+ *     src/timesteppers/cnrk2.jl:147); distinct handles may run concurrently
+ *     on distinct streams.
+ */
+#ifndef CHANNELFLOW_HIP_H
+#define CHANNELFLOW_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF_OK 0
+#define CF_ERR_ARGUMENT 1
+#define CF_ERR_DIMENSION 2
+#define CF_ERR_DEVICE 3
+#define CF_ERR_UNSUPPORTED 4
+
+/* nonlinear forms (src/nonlinear.jl:214-224) */
+#define CF_FORM_ROTATING 0
+#define CF_FORM_CONVECTIVE 1
+#define CF_FORM_DIVERGENCE 2
+
+/* mean-flow constraint (src/problem.jl:81-87) */
+#define CF_CONSTRAINT_PRESSURE_GRADIENT 0
+#define CF_CONSTRAINT_BULK_VELOCITY 1
+
+typedef struct cf_problem cf_problem;
+
+typedef struct cf_config {
+    int Nx, Ny, Nz;          /* Grid(Nx, Ny, Nz, Lx, Lz), src/grids.jl:27-31 */
+    double Lx, Lz;
+    double nu, dt;           /* CNRK2(grid, baseflow, nu, dt), cnrk2.jl:58-83 */
+    const double* baseflow;  /* HOST: Ny base-velocity values at the Lobatto
+                                points y_j = cos(pi j/(Ny-1)) (problem.jl:105) */
+    int form;                /* CF_FORM_* (only CF_FORM_ROTATING on device) */
+    int constraint;          /* CF_CONSTRAINT_*: the problem default */
+    double constraint_values[2];
+    int device;              /* HIP device ordinal */
+} cf_config;
+
+/* ChannelFlowProblem(grid, profile, nu, dt; form, pressuregradient|bulkvelocity)
+ * (src/problem.jl:69-107; CouetteFlow/PoiseuilleFlow 122-144) followed by
+ * adapt(CuArray, problem) (ext/ChannelFlowCUDAExt.jl:18-32): factors all three
+ * stage Stokes solvers on the host in FP64 and uploads them. */
+int cf_problem_create(const cf_config* cfg, cf_problem** out);
+int cf_problem_destroy(cf_problem* h);
+const char* cf_last_error(void);
+
+/* sizes: out[0..7] = Nxh, Nz, Ny (spectral), Nxp, Nzp, Ny (padded physical),
+ * spectral complex count, physical real count */
+int cf_problem_sizes(const cf_problem* h, long long out[8]);
+
+/* step!(scheme, nlterm, U, P, t; forcing=NoForcing(), pressuregradient | bulkvelocity)
+ * (src/timesteppers/cnrk2.jl:149-207).  Advances U = (U0, U1, U2) and stage
+ * pressure P in place by dt; *t_out = t + dt.  constraint = -1 uses the
+ * problem's own constraint, otherwise CF_CONSTRAINT_* with values[2]
+ * (pressure gradient (dPdx, dPdz) or TOTAL bulk velocity (Ubulk, Wbulk)). */
+int cf_step(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int constraint,
+            const double* values, double* t_out, void* stream);
+
+/* nsteps consecutive step! calls (same semantics, one enqueue) */
+int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int nsteps,
+               int constraint, const double* values, double* t_out, void* stream);
+
+/* Uniform pressure gradient (dPdx, dPdz) returned by the last Stokes solve of
+ * the last step (stokes.jl:101-107).  Synchronises the stream; HOST output. */
+int cf_last_gradient(cf_problem* h, double out[2], void* stream);
+
+/* ForwardFFT! (src/ffts.jl:65-82): padded physical u -> resolved spectrum U */
+int cf_forward(cf_problem* h, const double* u, double* U, void* stream);
+/* InverseFFT! (src/ffts.jl:151-166): resolved spectrum U -> padded physical u */
+int cf_inverse(cf_problem* h, const double* U, double* u, void* stream);
+
+/* NonLinearTerm (rotational form) N = FFT(V x curl V), V = U + Ub e_x
+ * (src/nonlinear.jl:173-193); overwrites N0..N2 */
+int cf_nonlinear(cf_problem* h, const double* U0, const double* U1, const double* U2, double* N0, double* N1,
+                 double* N2, void* stream);
+
+/* solve!(StokesSolver(grid, nu, lambda), U, P, R; pressuregradient|bulkvelocity,
+ * baseflow=problem base) (src/solvers/stokes.jl:81-108).  constraint/values as
+ * in cf_step (-1 = zero pressure gradient).  grads (HOST, may be NULL)
+ * receives (dPdx, dPdz) and synchronises the stream. */
+int cf_stokes_solve(cf_problem* h, double nu, double lambda, const double* R0, const double* R1, const double* R2,
+                    double* U0, double* U1, double* U2, double* P, int constraint, const double* values,
+                    double* grads, void* stream);
+
+/* project!(U, problem) (src/pressure.jl:117-135), in place */
+int cf_project(cf_problem* h, double* U0, double* U1, double* U2, void* stream);
+
+/* pressure(U, problem) (src/pressure.jl:176-185): stage pressure of a
+ * divergence-free no-slip velocity (modified pressure for the rotational form) */
+int cf_pressure(cf_problem* h, const double* U0, const double* U1, const double* U2, double* P, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHANNELFLOW_HIP_H */

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+inputs.  FP64 tolerances are stated per test; the step bar is 1e-12 relative
+(max-abs difference / max-abs value of the field)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    den = np.abs(b).max()
+    return np.abs(a - b).max() / (den if den > 0 else 1.0)
+
+
+def say(*args):
+    print(*args, flush=True)
+
+
+def hermitian_random(g, rng, n=None, scale=1.0):
+    """Spectral field(s) of a real physical field: Gaussian physical noise
+    forward-transformed by the oracle."""
+    shape = (g.Ny, g.Nzp, g.Nxp)
+    if n is None:
+        return O.forward(g, scale * rng.standard_normal(shape))
+    return np.stack([O.forward(g, scale * rng.standard_normal(shape)) for _ in range(n)])
+
+
+GRIDS = [(8, 17, 8), (9, 17, 9), (6, 9, 8), (7, 9, 9), (16, 33, 12), (32, 33, 32), (12, 35, 10), (1, 9, 5), (5, 9, 1)]
+
+
+def make(cf, dims, L=(5.3, 7.1), profile=lambda y: y, nu=1 / 400, dt=0.01, **kw):
+    Nx, Ny, Nz = dims
+    gc = cf.Grid(Nx, Ny, Nz, *L)
+    go = O.Grid(Nx, Ny, Nz, *L)
+    return gc, go, cf.ChannelFlowProblem(gc, profile, nu, dt, **kw)
+
+
+@pytest.mark.parametrize("dims", GRIDS)
+def test_transforms(gpu, dims):
+    import torch
+    cf = gpu
+    say("transforms", dims)
+    gc, go, prob = make(cf, dims)
+    rng = np.random.default_rng(sum(dims))
+    u = rng.standard_normal(go.phys_shape)
+    U = cf.FFT(prob, torch.as_tensor(u, device="cuda")).cpu().numpy()
+    Uref = O.forward(go, u)
+    assert rel(U, Uref) < 1e-13
+    V = Uref + 0.3 * hermitian_random(go, rng)
+    v = cf.IFFT(prob, torch.as_tensor(V, device="cuda")).cpu().numpy()
+    assert rel(v, O.inverse(go, V)) < 1e-13
+
+
+@pytest.mark.parametrize("dims", GRIDS)
+def test_nonlinear(gpu, dims):
+    import torch
+    cf = gpu
+    say("nonlinear", dims)
+    gc, go, prob = make(cf, dims)
+    rng = np.random.default_rng(3 + sum(dims))
+    U = hermitian_random(go, rng, 3)
+    N = prob.nlterm(0.0, torch.as_tensor(U, device="cuda")).cpu().numpy()
+    ref = O.Problem(go, lambda y: y, 1 / 400, 0.01).nonlinear(U)
+    assert rel(N, ref) < 1e-12
+
+
+@pytest.mark.parametrize("dims", GRIDS)
+@pytest.mark.parametrize("lam", [0.0, 600.0])
+@pytest.mark.parametrize("constraint", ["gradient", "bulk"])
+def test_stokes_solve(gpu, dims, lam, constraint):
+    import torch
+    cf = gpu
+    say("stokes", dims, lam, constraint)
+    gc, go, prob = make(cf, dims, profile=lambda y: 1 - y ** 2)
+    rng = np.random.default_rng(7 + sum(dims))
+    R = hermitian_random(go, rng, 3)
+    nu = 1 / 400
+    kw = {"bulkvelocity": (0.7, -0.1)} if constraint == "bulk" else {"pressuregradient": (-0.3, 0.2)}
+    U, P, grads = cf.stokes_solve(prob, nu, lam, torch.as_tensor(R, device="cuda"), **kw)
+    basec = O.chebcoeffs(1 - go.y ** 2)
+    Ur, Pr, gr = O.stokes(go, nu, lam, R, baseflow=basec, **kw)
+    assert rel(U.cpu().numpy(), Ur) < 1e-12
+    assert rel(P.cpu().numpy(), Pr) < 1e-12
+    assert np.allclose(grads, gr, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 9, 7), (32, 33, 32)])
+@pytest.mark.parametrize("bulk", [False, True])
+def test_project_and_pressure(gpu, dims, bulk):
+    import torch
+    cf = gpu
+    say("project/pressure", dims, bulk)
+    kw = {"bulkvelocity": (2 / 3 + 0.05, 0.0)} if bulk else {"pressuregradient": (-2 / 400, 0.0)}
+    gc, go, prob = make(cf, dims, profile=lambda y: 1 - y ** 2, **kw)
+    ref = O.Problem(go, lambda y: 1 - y ** 2, 1 / 400, 0.01, **kw)
+    rng = np.random.default_rng(11)
+    U = hermitian_random(go, rng, 3, 0.1)
+    Ud = torch.as_tensor(U, device="cuda")
+    cf.project_(Ud, prob)
+    Uref = ref.project(U)
+    assert rel(Ud.cpu().numpy(), Uref) < 1e-12
+    P = cf.pressure(torch.as_tensor(Uref, device="cuda"), prob).cpu().numpy()
+    assert rel(P, ref.pressure(Uref)) < 1e-12
+
+
+def run_pair(cf, dims, nsteps, L=(2 * np.pi / 1.14, 2 * np.pi / 2.5), profile=lambda y: y, nu=1 / 400, dt=0.01,
+             eps=0.1, seed=42, **kw):
+    import torch
+    gc, go, prob = make(cf, dims, L=L, profile=profile, nu=nu, dt=dt, **kw)
+    ref = O.Problem(go, profile, nu, dt, **kw)
+    U0, P0 = ref.random_state(cf.gaussian_noise(gc, eps, seed))
+    U = torch.as_tensor(U0, device="cuda")
+    P = torch.as_tensor(P0, device="cuda")
+    t = cf.step_(prob.scheme, prob.nlterm, U, P, 0.0, nsteps=nsteps, **prob.constraint)
+    assert abs(t - nsteps * dt) < 1e-12
+    U1, P1 = ref.step(U0, P0, nsteps)
+    return U.cpu().numpy(), P.cpu().numpy(), U1, P1, prob, ref
+
+
+@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 17, 9), (16, 17, 16), (6, 9, 5)])
+def test_step_small(gpu, dims):
+    say("step small", dims)
+    U, P, Ur, Pr, _, _ = run_pair(gpu, dims, 5)
+    for c in range(3):
+        assert rel(U[c], Ur[c]) < TOL, c
+    assert rel(P, Pr) < TOL
+
+
+def test_step_couette_32_100steps(gpu):
+    # configuration 1: Couette 32x33x32, Lx=2pi/1.14, Lz=2pi/2.5, Re=400, dt=0.01,
+    # random_state amplitude 0.1 (seed 42), 100 CNRK2 steps
+    say("step couette 32x33x32 100 steps")
+    U, P, Ur, Pr, _, _ = run_pair(gpu, (32, 33, 32), 100)
+    errs = [rel(U[c], Ur[c]) for c in range(3)] + [rel(P, Pr)]
+    say("rel errors", errs)
+    assert max(errs) < TOL
+
+
+@pytest.mark.parametrize("bulk", [False, True])
+def test_step_poiseuille(gpu, bulk):
+    say("step poiseuille bulk", bulk)
+    nu = 1 / 2800
+    kw = {"bulkvelocity": (2 / 3, 0.0)} if bulk else {"pressuregradient": (-2 * nu, 0.0)}
+    U, P, Ur, Pr, prob, ref = run_pair(gpu, (16, 33, 16), 10, L=(4 * np.pi, 4 * np.pi / 3),
+                                       profile=lambda y: 1 - y ** 2, nu=nu, dt=0.005, eps=0.05, **kw)
+    errs = [rel(U[c], Ur[c]) for c in range(3)] + [rel(P, Pr)]
+    say("rel errors", errs)
+    assert max(errs) < TOL
+    import channelflow as cf
+    g = cf.last_gradient(prob.scheme)
+    gr = ref.last_gradient()
+    assert np.allclose(g, gr, rtol=1e-10, atol=1e-15)
