@@ -1,0 +1,174 @@
+"""Benchmark of the MI355X CNRK2 step (one JSON line on stdout).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps C] [--workload NAME]
+
+Workload (default): plain plane Poiseuille DNS, FP64, constant bulk flux,
+grid 128x97x128, Lx=4pi, Lz=4pi/3, nu=1/2800 (bulk Re 5600), dt=0.005,
+base flow 1.5(1-y^2), bulk velocity (1, 0), seeded divergence-free
+perturbation of amplitude 0.05 with kappa^-4 energy spectrum.
+A step is one CNRK2 step! call (three stages) on device-resident fields.
+
+value = steps/s on one GPU.  roofline = the dominant kernel class (by device
+time) with its algorithmic bytes per launch over its event-timed mean launch
+duration.  cpu_baseline = the oracle (C++ restatement of the reference CPU
+step, all host cores) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "channelflow.jl_amd")]
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "poiseuille128": dict(N=(128, 97, 128), L=(4 * math.pi, 4 * math.pi / 3), nu=1 / 2800, dt=0.005,
+                          profile="poiseuille15", bulk=(1.0, 0.0), amp=0.05,
+                          desc="plane Poiseuille 128x97x128 FP64 bulk-flux, Lx=4pi Lz=4pi/3 nu=1/2800 dt=0.005"),
+    "couette32": dict(N=(32, 33, 32), L=(2 * math.pi / 1.14, 2 * math.pi / 2.5), nu=1 / 400, dt=0.01,
+                      profile="couette", bulk=None, amp=0.1,
+                      desc="plane Couette 32x33x32 FP64, Lx=2pi/1.14 Lz=2pi/2.5 Re=400 dt=0.01"),
+}
+
+CLASSES = ["y_prologue", "z_inverse", "x_pass", "z_forward", "y_stage"]
+LAUNCHES_PER_STEP = [1, 3, 3, 3, 3]
+
+
+def algorithmic_bytes(Nx, Ny, Nz):
+    """Per-launch algorithmic HBM bytes (one read of every input, one write of
+    every output, complex128) of each kernel class, and the canonical
+    per-step figure 16(90 S + 54 Sz) used for the step-level roofline."""
+    Nxh, Nzp = Nx // 2 + 1, (3 * Nz + 1) // 2
+    S = Nxh * Nz * Ny
+    Sz = Nxh * Nzp * Ny
+    per_launch = {
+        "y_prologue": 16 * 9 * S,             # U (3) -> V, Omega (6)
+        "z_inverse": 16 * (6 * S + 6 * Sz),   # 6 fields (k,l,y) -> (k,zp,y)
+        "x_pass": 16 * 9 * Sz,                # 6 in, 3 out
+        "z_forward": 16 * (3 * Sz + 3 * S),   # 3 fields (k,zp,y) -> (k,l,y)
+        "y_stage": 16 * 19 * S,               # stage mean of (7+13, 10+13, 10+4) S
+    }
+    step_canonical = 16 * (90 * S + 54 * Sz)
+    return per_launch, step_canonical, S, Sz
+
+
+def make_problem(cf, w):
+    Nx, Ny, Nz = w["N"]
+    g = cf.Grid(Nx, Ny, Nz, *w["L"])
+    prof = (lambda y: 1.5 * (1 - y ** 2)) if w["profile"] == "poiseuille15" else (lambda y: y)
+    kw = {"bulkvelocity": w["bulk"]} if w["bulk"] else {"pressuregradient": (0.0, 0.0)}
+    return g, prof, kw, cf.ChannelFlowProblem(g, prof, w["nu"], w["dt"], **kw)
+
+
+def cpu_baseline(w, prof, kw, U0, P0, nsteps):
+    """Oracle (CPU restatement) on `nsteps` steps of the same workload."""
+    import numpy as np
+    from oracle import oracle as O
+
+    Nx, Ny, Nz = w["N"]
+    go = O.Grid(Nx, Ny, Nz, *w["L"])
+    ref = O.Problem(go, prof, w["nu"], w["dt"], **kw)
+    U, P = np.ascontiguousarray(U0), np.ascontiguousarray(P0)
+    ref.step_inplace(U, P, 1)  # warm-up (page-in, thread pool)
+    t0 = time.perf_counter()
+    ref.step_inplace(U, P, nsteps)
+    dt = time.perf_counter() - t0
+    return nsteps / dt, O.num_threads(), dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--workload", default="poiseuille128", choices=sorted(WORKLOADS))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (requested --gpus {args.gpus}); "
+              "the multi-GPU pencil decomposition is a design note, not built.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import channelflow as cf
+
+    cf.require_device()
+    w = WORKLOADS[args.workload]
+    g, prof, kw, prob = make_problem(cf, w)
+    state = cf.perturbation_state(prob, w["amp"], seed=42)
+    U, P = state.velocity, state.stagepressure
+    U0, P0 = U.cpu().numpy(), P.cpu().numpy()
+    t = 0.0
+    for _ in range(args.warmup):
+        t = cf.step_(prob.scheme, prob.nlterm, U, P, t, **prob.constraint)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        t = cf.step_(prob.scheme, prob.nlterm, U, P, t, **prob.constraint)
+    stop.record()
+    torch.cuda.synchronize()
+    ms_total = start.elapsed_time(stop)
+    ms_per_step = ms_total / args.steps
+    steps_per_s = 1000.0 / ms_per_step
+    assert bool(torch.isfinite(torch.view_as_real(U)).all()), "non-finite velocity"
+
+    # per-kernel-class device times (HIP events on the launch stream)
+    acc = [0.0] * 5
+    for _ in range(args.profile_steps):
+        ms = cf.step_timed(prob.scheme, U, P, t, **prob.constraint)
+        acc = [a + m for a, m in zip(acc, ms)]
+    per_step_ms = [a / args.profile_steps for a in acc]
+    dom = max(range(5), key=lambda i: per_step_ms[i])
+    per_launch_bytes, step_bytes, S, Sz = algorithmic_bytes(*w["N"])
+    launch_ms = per_step_ms[dom] / LAUNCHES_PER_STEP[dom]
+    achieved = per_launch_bytes[CLASSES[dom]] / (launch_ms * 1e-3) / 1e9
+    breakdown = {CLASSES[i]: {"ms_per_step": round(per_step_ms[i], 5),
+                              "gbs": round(per_launch_bytes[CLASSES[i]] * LAUNCHES_PER_STEP[i] /
+                                           (per_step_ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
+    step_gbs = step_bytes * steps_per_s / 1e9
+
+    cpu = None
+    if args.cpu_steps > 0:
+        cps, cores, secs = cpu_baseline(w, prof, kw, U0, P0, args.cpu_steps)
+        cpu = {"value": round(cps, 5), "unit": "steps/s", "cores": cores, "kind": "port",
+               "sample": f"{args.cpu_steps} oracle steps of the same workload after 1 warm-up step "
+                         f"({secs:.1f} s, {cores} OpenMP threads)"}
+
+    out = {
+        "metric": "CNRK2 steps/s & %HBM roofline, 128×97×128 FP64; mixed/batched/closure vs plain",
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded divergence-free perturbation, kappa^-4 spectrum, amplitude 0.05)",
+        "config": {"workload": w["desc"], "grid": list(w["N"]), "form": "rotational",
+                   "constraint": "bulkvelocity (1.0, 0.0)" if w["bulk"] else "pressuregradient (0, 0)",
+                   "step_algorithmic_bytes": step_bytes,
+                   "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                   "kernel_breakdown": breakdown},
+        "roofline": {"bound": "hbm", "kernel": CLASSES[dom], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -348,6 +348,42 @@ def random_state(problem, epsilon, seed=42):
     return State(U, pressure(U, problem))
 
 
+def perturbation_state(problem, amplitude, seed=42, decay=4.0):
+    """Seeded divergence-free perturbation with spectral energy ~ kappa^-decay:
+    Gaussian coefficients of amplitude (1+kappa^2)^(-decay/4) (n+1)^-2, made
+    real by a physical round trip, projected (project!), scaled so that the
+    largest padded-grid velocity component equals `amplitude`, re-projected,
+    and completed by the consistent stage pressure (pressure)."""
+    torch = _torch()
+    g = problem.grid
+    rng = np.random.default_rng(seed)
+    k = 2 * np.pi / g.Lx * np.arange(g.Nxh)
+    l = 2 * np.pi / g.Lz * np.array([iz if iz <= g.Nz // 2 else iz - g.Nz for iz in range(g.Nz)])
+    kap2 = k[None, None, :] ** 2 + l[None, :, None] ** 2
+    n = np.arange(g.Ny)[:, None, None]
+    amp = (1.0 + kap2) ** (-decay / 4.0) * (n + 1.0) ** -2
+    coef = (rng.standard_normal((3,) + g.spec_shape) + 1j * rng.standard_normal((3,) + g.spec_shape)) * amp
+    U = torch.as_tensor(coef, device="cuda")
+    U = FFT(problem, IFFT(problem, U))  # keep the part representing a real field
+    project_(U, problem)
+    umax = float(IFFT(problem, U).abs().max())
+    U *= amplitude / umax
+    project_(U, problem)
+    return State(U, pressure(U, problem))
+
+
+def step_timed(scheme, U, P, t, pressuregradient=None, bulkvelocity=None, stream=None):
+    """One step with per-kernel-class device times (ms): prologue, z inverse,
+    x pass, z forward, y stage."""
+    kind, vals = _constraint(pressuregradient, bulkvelocity)
+    arr = (C.c_double * 2)(*vals)
+    ms = (C.c_double * 5)()
+    tout = C.c_double()
+    _lib.check(_lib.load().cf_step_timed(scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), float(t), kind, arr,
+                                         C.byref(tout), ms, _stream(stream)))
+    return list(ms)
+
+
 # ---------------------------------------------------------------------------
 # diagnostics (postprocessing.jl:21-95), torch reductions on the device
 # ---------------------------------------------------------------------------

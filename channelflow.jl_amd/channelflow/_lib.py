@@ -34,6 +34,7 @@ _lib = None
 
 # symbols declared in include/channelflow_hip.h
 SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_problem_sizes", "cf_step", "cf_advance",
+           "cf_step_timed",
            "cf_last_gradient", "cf_forward", "cf_inverse", "cf_nonlinear", "cf_stokes_solve", "cf_project",
            "cf_pressure"]
 
@@ -53,6 +54,7 @@ def load(path=LIB_PATH):
     L.cf_problem_sizes.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.cf_step.argtypes = [vp, vp, vp, vp, vp, d, i, dp, dp, vp]
     L.cf_advance.argtypes = [vp, vp, vp, vp, vp, d, i, i, dp, dp, vp]
+    L.cf_step_timed.argtypes = [vp, vp, vp, vp, vp, d, i, dp, dp, dp, vp]
     L.cf_last_gradient.argtypes = [vp, dp, vp]
     L.cf_forward.argtypes = [vp, vp, vp, vp]
     L.cf_inverse.argtypes = [vp, vp, vp, vp]

@@ -338,7 +338,9 @@ static XArgs base_xargs(cf_problem* h) {
 }
 
 // y-physical (V, Omega) in six -> y-physical nonlinear term in six[0..2]
-static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st) {
+struct Timer;
+static int tmark(Timer* tm, int c, hipStream_t st);
+static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st, Timer* tm = nullptr) {
     const GridDims& gd = h->gd;
     double2* six[6];
     double2* zb[6];
@@ -347,12 +349,15 @@ static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st) {
         zb[i] = fld(h->zbuf, gd.Sz, i);
     }
     if (int rc = zpass(h, +1, 6, six, zb, st)) return rc;
+    if (int rc = tmark(tm, 1, st)) return rc;
     XArgs x = base_xargs(h);
     x.mode = X_ROTATIONAL;
     for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
     for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a block reads all six lines first
     HIPTRY(launch_xpass(x, st));
+    if (int rc = tmark(tm, 2, st)) return rc;
     if (int rc = zpass(h, -1, 3, zb, six, st)) return rc;
+    if (int rc = tmark(tm, 3, st)) return rc;
     return 0;
 }
 
@@ -394,12 +399,35 @@ static int check_fields(const void* const* f, int n) {
     return 0;
 }
 
-static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st) {
+// Optional per-kernel-class timing with HIP events recorded on the launch
+// stream around every launch (measurement only).
+struct Timer {
+    static constexpr int NCLASS = 5;  // prologue, z inverse, x pass, z forward, y stage
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    int mark(int c, hipStream_t st) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return fail(CF_ERR_DEVICE, "hipEventCreate");
+        if (hipEventRecord(e, st) != hipSuccess) return fail(CF_ERR_DEVICE, "hipEventRecord");
+        ev.push_back(e);
+        cls.push_back(c);
+        return 0;
+    }
+    ~Timer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+static int tmark(Timer* tm, int c, hipStream_t st) { return tm ? tm->mark(c, st) : 0; }
+
+static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
+                         Timer* tm = nullptr) {
     const GridDims& gd = h->gd;
     for (int step = 0; step < nsteps; ++step) {
+        if (int rc = tmark(tm, -1, st)) return rc;
         if (int rc = enqueue_prologue(h, U, st)) return rc;
+        if (int rc = tmark(tm, 0, st)) return rc;
         for (int j = 0; j < 3; ++j) {
-            if (int rc = enqueue_nonlinear_phys(h, st)) return rc;
+            if (int rc = enqueue_nonlinear_phys(h, st, tm)) return rc;
             YArgs a = base_yargs(h);
             a.stage = j;
             a.store_Q = (j < 2);
@@ -420,6 +448,7 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
             if (j < 2)
                 for (int i = 0; i < 6; ++i) a.out6[i] = fld(h->six, gd.S, i);
             HIPTRY(launch_ykernel(Y_STAGE, a, st));
+            if (int rc = tmark(tm, 4, st)) return rc;
         }
     }
     return 0;
@@ -444,6 +473,29 @@ extern "C" int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, dou
 extern "C" int cf_step(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int constraint,
                        const double* values, double* t_out, void* stream) {
     return cf_advance(h, U0, U1, U2, P, t, 1, constraint, values, t_out, stream);
+}
+
+extern "C" int cf_step_timed(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int constraint,
+                             const double* values, double* t_out, double ms[5], void* stream) {
+    if (!h || !ms) return fail(CF_ERR_ARGUMENT, "null argument");
+    const void* f[4] = {U0, U1, U2, P};
+    if (int rc = check_fields(f, 4)) return rc;
+    Constraint c;
+    if (int rc = resolve_constraint(h, constraint, values, c)) return rc;
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    Timer tm;
+    if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, &tm)) return rc;
+    HIPTRY(hipEventSynchronize(tm.ev.back()));
+    for (int i = 0; i < Timer::NCLASS; ++i) ms[i] = 0.0;
+    for (size_t i = 1; i < tm.ev.size(); ++i) {
+        float e = 0.f;
+        HIPTRY(hipEventElapsedTime(&e, tm.ev[i - 1], tm.ev[i]));
+        if (tm.cls[i] >= 0) ms[tm.cls[i]] += e;
+    }
+    if (t_out) *t_out = t + h->dt;
+    return CF_OK;
 }
 
 extern "C" int cf_last_gradient(cf_problem* h, double out[2], void* stream) {

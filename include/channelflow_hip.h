@@ -80,6 +80,13 @@ int cf_step(cf_problem* h, double* U0, double* U1, double* U2, double* P, double
 int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int nsteps,
                int constraint, const double* values, double* t_out, void* stream);
 
+/* Measurement variant of cf_step (no counterpart in the reference): one step
+ * with HIP events recorded on `stream` around every launch; waits for the
+ * step, then ms[0..4] = device milliseconds spent in the prologue (y), the
+ * z-inverse passes, the x passes, the z-forward passes and the y stages. */
+int cf_step_timed(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int constraint,
+                  const double* values, double* t_out, double ms[5], void* stream);
+
 /* Uniform pressure gradient (dPdx, dPdz) returned by the last Stokes solve of
  * the last step (stokes.jl:101-107).  Synchronises the stream; HOST output. */
 int cf_last_gradient(cf_problem* h, double out[2], void* stream);
