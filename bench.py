@@ -88,7 +88,18 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--workload", default="poiseuille128", choices=sorted(WORKLOADS))
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run N device copies of 256 MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(args.copy_calibration):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each": src.numel() * 8, "copies": args.copy_calibration}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (requested --gpus {args.gpus}); "
               "the multi-GPU pencil decomposition is a design note, not built.", file=sys.stderr)
@@ -126,7 +137,9 @@ def main():
     for _ in range(args.profile_steps):
         ms = cf.step_timed(prob.scheme, U, P, t, **prob.constraint)
         acc = [a + m for a, m in zip(acc, ms)]
-    per_step_ms = [a / args.profile_steps for a in acc]
+    per_step_ms = [a / max(args.profile_steps, 1) for a in acc]
+    if args.profile_steps == 0:
+        per_step_ms = [1.0] * 5  # roofline not measured in this run
     dom = max(range(5), key=lambda i: per_step_ms[i])
     per_launch_bytes, step_bytes, S, Sz = algorithmic_bytes(*w["N"])
     launch_ms = per_step_ms[dom] / LAUNCHES_PER_STEP[dom]

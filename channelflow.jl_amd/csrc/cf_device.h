@@ -44,7 +44,7 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 template <int L>
 struct Group {
     int lane, t, base;
-    __device__ Group() {
+    __device__ __forceinline__ Group() {
         lane = lane_id();
         t = lane & (L - 1);
         base = lane - t;
@@ -401,7 +401,7 @@ __device__ __forceinline__ void stockham_pass_fixed(const double2* src, double2*
 }
 
 // generic odd radix (rare sizes): O(R^2) per butterfly, reads LDS directly
-__device__ __forceinline__ void stockham_pass_generic(const double2* src, double2* dst, int n, int R, int nlines, int ld,
+__device__ __noinline__ void stockham_pass_generic(const double2* src, double2* dst, int n, int R, int nlines, int ld,
                                                       int Ns, const double2* tw, int sign, int tid, int nthr) {
     const int m = n / R;
     const int twstep = n / (Ns * R);
@@ -431,7 +431,7 @@ __device__ __forceinline__ void stockham_pass_generic(const double2* src, double
 
 // Run all passes; returns the buffer holding the result.  Must be called by
 // all threads of the block (contains __syncthreads).
-__device__ inline double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
+__device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
                                    int nthr) {
     int Ns = 1;
     double2* src = a;

@@ -84,7 +84,7 @@ struct XCtx {
     bool even;
     long long gl0, nlines;
     double2 *bufA, *bufB;
-    __device__ XCtx(const XArgs& a_, double2* s) : a(a_) {
+    __device__ __forceinline__ XCtx(const XArgs& a_, double2* s) : a(a_) {
         Nxp = a.gd.Nxp;
         Nxh = a.gd.Nxh;
         even = (Nxp % 2 == 0);
@@ -99,7 +99,7 @@ struct XCtx {
     }
     // inverse real transform of LPB lines of `in` (Nxh complex per line)
     // into phys[line * Nxp + x]
-    __device__ void c2r(const double2* __restrict__ in, double* phys) {
+    __device__ __forceinline__ void c2r(const double2* __restrict__ in, double* phys) {
         const bool xeven = (a.gd.Nx % 2 == 0);
         // stage X_k (k = 0..M) in bufB (zero padding, Nyquist plane and Im(DC) removed)
         const int nX = even ? M + 1 : Nxp / 2 + 1;
@@ -145,7 +145,7 @@ struct XCtx {
         __syncthreads();
     }
     // forward real transform of phys lines -> out (Nxh complex per line)
-    __device__ void r2c(const double* phys, double2* __restrict__ out) {
+    __device__ __forceinline__ void r2c(const double* phys, double2* __restrict__ out) {
         for (int idx = tid; idx < LPB * M; idx += XNT) {
             int ln = idx / M, j = idx - ln * M;
             bufA[ln * ld + j] = even ? cmk(phys[ln * Nxp + 2 * j], phys[ln * Nxp + 2 * j + 1]) : cmk(phys[ln * Nxp + j], 0.0);

@@ -50,13 +50,13 @@ struct Tile {
     long long s0;
     int tid, mloc, ld;
     double2* buf;
-    __device__ Tile(const GridDims& g_, double2* b, int ld_) : gd(g_), ld(ld_), buf(b) {
+    __device__ __forceinline__ Tile(const GridDims& g_, double2* b, int ld_) : gd(g_), ld(ld_), buf(b) {
         tid = threadIdx.x;
         mloc = tid / L;
         s0 = (long long)blockIdx.x * M;
     }
     // spectral field rows n = 0..P -> registers (zero beyond P and for absent modes)
-    __device__ CVec load(const double2* __restrict__ src, const Group<L>& g) {
+    __device__ __forceinline__ CVec load(const double2* __restrict__ src, const Group<L>& g) {
         const int Ny = gd.Ny;
         for (int idx = tid; idx < Ny * M; idx += YNT) {
             int n = idx / M, m = idx - n * M;
@@ -73,7 +73,7 @@ struct Tile {
         __syncthreads();
         return v;
     }
-    __device__ void store(double2* __restrict__ dst, const Group<L>& g, const CVec& v) {
+    __device__ __forceinline__ void store(double2* __restrict__ dst, const Group<L>& g, const CVec& v) {
         const int Ny = gd.Ny;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -90,7 +90,7 @@ struct Tile {
     }
     // forward Chebyshev transform of y-physical rows (values at y_j, j=0..P):
     // REDFT00, times norm, endpoint 1/2, Nyquist modes zeroed (normalize_forward!).
-    __device__ CVec dct_forward(const double2* __restrict__ src, double2* bufB, const FFTPlan& plan, double norm,
+    __device__ __forceinline__ CVec dct_forward(const double2* __restrict__ src, double2* bufB, const FFTPlan& plan, double norm,
                                 const Group<L>& g, bool zero_mode) {
         const int P = gd.P;
         for (int idx = tid; idx < (P + 1) * M; idx += YNT) {
@@ -114,7 +114,7 @@ struct Tile {
     }
     // inverse: coefficient registers -> values at y_j (endpoints doubled,
     // REDFT00, x1/2: InverseFFT!'s chebyshev + copy_to_padded! scale).
-    __device__ void dct_inverse(double2* __restrict__ dst, double2* bufB, const FFTPlan& plan, const Group<L>& g,
+    __device__ __forceinline__ void dct_inverse(double2* __restrict__ dst, double2* bufB, const FFTPlan& plan, const Group<L>& g,
                                 const CVec& a) {
         const int P = gd.P;
 #pragma unroll
@@ -136,6 +136,21 @@ struct Tile {
         __syncthreads();
     }
 };
+
+// Lane-private LDS stash of a coefficient vector (each lane writes and reads
+// back only its own four slots, so no barrier is needed between the two).
+template <int L>
+__device__ __forceinline__ void stash(double2* slot, const Group<L>& g, const CVec& v) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) slot[4 * g.t + e] = v.v[e];
+}
+template <int L>
+__device__ __forceinline__ CVec unstash(const double2* slot, const Group<L>& g) {
+    CVec v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v.v[e] = slot[4 * g.t + e];
+    return v;
+}
 
 // coefficient of degree n (broadcast to the group)
 template <int L>
@@ -160,7 +175,7 @@ __device__ __forceinline__ void load_fac(const double* __restrict__ f, long long
 // V = U + Ub e_x at (0,0); Omega = curl V (operators.jl:172-190); values
 // of the six fields at the Lobatto points into out6.
 template <int L>
-__device__ void inverse_six(Tile<L>& tile, double2* bufB, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
+__device__ __forceinline__ void inverse_six(Tile<L>& tile, double2* bufB, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
                             const CVec U[3]) {
     CVec V[3] = {U[0], U[1], U[2]};
     if (mi.mean && mi.valid) {
@@ -194,31 +209,47 @@ __device__ void inverse_six(Tile<L>& tile, double2* bufB, const YArgs& a, const 
 // U[3], P.  mean/nyq modes handled separately by the caller.
 // ---------------------------------------------------------------------------
 template <int L>
-__device__ void solve_nonzero(const Group<L>& g, const YArgs& a, const Bank& bk, const ModeInfo& mi, const CVec R[3],
-                              CVec Uout[3], CVec& Pout) {
+__device__ __forceinline__ void solve_nonzero(const Group<L>& g, const YArgs& a, const Bank& bk, const ModeInfo& mi,
+                                              const double2* Rs0, const double2* Rs1, const double2* Rs2,
+                                              CVec Uout[3], CVec& Pout) {
     const int P = a.gd.P, Ny = a.gd.Ny;
-    double fp[4], fv[4];
-    load_fac(a.facP, mi.s, Ny, g, fp);
-    load_fac(bk.facV, mi.s, Ny, g, fv);
     const double* mc = bk.mconst + mi.s * MC_COUNT;
     const double shift = bk.lambda + bk.nu * mi.kappa2;
+    double fp[4], fv[4];
+    load_fac(a.facP, mi.s, Ny, g, fp);
     // r = D Ry + i k Rx + i l Rz
-    CVec r = cheb_deriv(g, R[1]);
+    CVec r;
+    {
+        CVec R1 = unstash(Rs1, g);
+        r = cheb_deriv(g, R1);
+        CVec R0 = unstash(Rs0, g), R2 = unstash(Rs2, g);
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-        r.v[e] = cadd(r.v[e], cadd(cmuli(cscale(R[0].v[e], mi.kx)), cmuli(cscale(R[2].v[e], mi.kz))));
-    CVec p1 = helm_solve(g, P, mi.kappa2, fp, false, r, czero(), czero());
-    CVec q = cheb_deriv(g, p1);
+        for (int e = 0; e < 4; ++e)
+            r.v[e] = cadd(r.v[e], cadd(cmuli(cscale(R0.v[e], mi.kx)), cmuli(cscale(R2.v[e], mi.kz))));
+    }
+    double2 p1P, v1a, v1b, bp, bm;
+    {
+        CVec p1 = helm_solve(g, P, mi.kappa2, fp, false, r, czero(), czero());
+        p1P = coef_at(g, p1, P);
+        CVec q = cheb_deriv(g, p1);
+        CVec R1 = unstash(Rs1, g);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R[1].v[e]);
-    CVec v1 = helm_solve(g, P, shift, fv, false, q, czero(), czero());
-    double2 bp = cscale(wall_deriv_plus(g, v1), -1.0);
-    double2 bm = cscale(wall_deriv_minus(g, v1), -1.0);
+        for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R1.v[e]);
+        load_fac(bk.facV, mi.s, Ny, g, fv);
+        CVec v1 = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+        bp = cscale(wall_deriv_plus(g, v1), -1.0);
+        bm = cscale(wall_deriv_minus(g, v1), -1.0);
+        v1a = coef_at(g, v1, P - 1);
+        v1b = coef_at(g, v1, P);
+    }
     double2 dp = cadd(cscale(bp, mc[MC_A11]), cscale(bm, mc[MC_A12]));
     double2 dm = cadd(cscale(bp, mc[MC_A21]), cscale(bm, mc[MC_A22]));
-    double2 p1P = coef_at(g, p1, P);
-    double2 v1a = coef_at(g, v1, P - 1), v1b = coef_at(g, v1, P);
-    double2 ra = coef_at(g, R[1], P - 1), rb = coef_at(g, R[1], P);
+    double2 ra, rb;
+    {
+        CVec R1 = unstash(Rs1, g);
+        ra = coef_at(g, R1, P - 1);
+        rb = coef_at(g, R1, P);
+    }
     double2 p2P = cadd(p1P, cadd(cscale(dp, mc[MC_PPT]), cscale(dm, mc[MC_PMT])));
     double2 v2a = cadd(v1a, cadd(cscale(dp, mc[MC_VP1]), cscale(dm, mc[MC_VM1])));
     double2 v2b = cadd(v1b, cadd(cscale(dp, mc[MC_VP0]), cscale(dm, mc[MC_VM0])));
@@ -237,21 +268,34 @@ __device__ void solve_nonzero(const Group<L>& g, const YArgs& a, const Bank& bk,
     double2 de = cadd(cscale(cadd(dp, dm), 0.5), cscale(s1, 0.5 * (mc[MC_D0P] + mc[MC_D0M])));
     double2 dd = cadd(cscale(csub(dp, dm), 0.5), cscale(s2, 0.5 * (mc[MC_D0P] - mc[MC_D0M])));
     Pout = helm_solve(g, P, mi.kappa2, fp, false, r, de, dd);
-    q = cheb_deriv(g, Pout);
+    __builtin_amdgcn_sched_barrier(0);
+    CVec q;
+    {
+        CVec R0 = unstash(Rs0, g);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R[1].v[e]);
-    Uout[1] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
-#pragma unroll
-    for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kx)), R[0].v[e]);
+        for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kx)), R0.v[e]);
+    }
     Uout[0] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        CVec R2 = unstash(Rs2, g);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kz)), R[2].v[e]);
+        for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kz)), R2.v[e]);
+    }
     Uout[2] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+    __builtin_amdgcn_sched_barrier(0);
+    q = cheb_deriv(g, Pout);
+    {
+        CVec R1 = unstash(Rs1, g);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R1.v[e]);
+    }
+    Uout[1] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
 }
 
 // mean mode (meanmode.jl:103-150); returns the new gradients
 template <int L>
-__device__ void solve_mean(const Group<L>& g, const YArgs& a, const Bank& bk, const CVec R[3], CVec Uout[3], CVec& Pout,
+__device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, const Bank& bk, const CVec R[3], CVec Uout[3], CVec& Pout,
                            double gout[2]) {
     const int P = a.gd.P, Ny = a.gd.Ny;
     // pressure: integrate Ry through degree P-1, zero bulk mean
@@ -327,34 +371,30 @@ __global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
     Tile<L> tile(a.gd, bufA, ld);
     Group<L> g;
     ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
-    const Bank& bk = a.bank;
     const int Ny = a.gd.Ny;
+    const bool meanmode = mi.mean && mi.valid;
 
-    CVec N[3];
-    for (int c = 0; c < 3; ++c) N[c] = tile.dct_forward(a.Nphys[c], bufB, a.dct, a.norm, g, mi.nyq);
-
-    // explicit-history update Q = N (stage 1) or A_j Q + N
-    CVec Q[3];
-    for (int c = 0; c < 3; ++c) {
-        if (a.stage == 0) {
-            Q[c] = N[c];
-        } else {
-            Q[c] = tile.load(a.Q[c], g);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) Q[c].v[e] = cfma(a.Aj, Q[c].v[e], N[c].v[e]);
-        }
-        if (a.store_Q) tile.store(a.Q[c], g, Q[c]);
-    }
-    // old uniform gradient (bulk: _bulkpressuregradient, cnrk2.jl:98-107)
-    double gold[2] = {a.c0, a.c1};
+    // Per component: forward DCT of the nonlinear term, Q = N (stage 1) or
+    // A_j Q + N, R_c = lambda U_c + nu Lap U_c - d_c P + (B_j/C_j) Q_c.
+    double gold0 = a.c0, gold1 = a.c1;
     CVec Pold = tile.load(a.P, g);
     CVec R[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
+        CVec Qc = tile.dct_forward(a.Nphys[c], bufB, a.dct, a.norm, g, mi.nyq);
+        double bN = 0.0;
+        if (a.bulk && meanmode && c != 1) bN = bulk_mean(g, Qc).x;
+        if (a.stage != 0) {
+            CVec Qo = tile.load(a.Q[c], g);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Qc.v[e] = cfma(a.Aj, Qo.v[e], Qc.v[e]);
+        }
+        if (a.store_Q) tile.store(a.Q[c], g, Qc);
         CVec U = tile.load(a.U[c], g);
-        if (a.bulk && mi.mean && mi.valid && c != 1) {
+        if (a.bulk && meanmode && c != 1) {  // _bulkpressuregradient, cnrk2.jl:98-107
             double2 sh = csub(wall_deriv_plus(g, U), wall_deriv_minus(g, U));
-            double base = (c == 0) ? a.nu * a.curv_mean : 0.0;
-            gold[c == 0 ? 0 : 1] = a.nu * sh.x / 2.0 + base + bulk_mean(g, N[c]).x;
+            double gv = a.nu * sh.x / 2.0 + (c == 0 ? a.nu * a.curv_mean : 0.0) + bN;
+            if (c == 0) gold0 = gv; else gold1 = gv;
         }
         CVec lap = laplacian(g, U, mi.kappa2);
         CVec dP;
@@ -370,35 +410,43 @@ __global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
             double2 x = cscale(U.v[e], a.lambda);
             x = cfma(a.nu, lap.v[e], x);
             x = csub(x, dP.v[e]);
-            R[c].v[e] = cfma(a.weight, Q[c].v[e], x);
+            R[c].v[e] = cfma(a.weight, Qc.v[e], x);
         }
     }
-    if (mi.mean && mi.valid) {
+    if (meanmode) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int n = 4 * g.t + e;
             if (n < Ny) R[0].v[e] = cadd(R[0].v[e], cmk(2.0 * a.nu * a.curv[n], 0.0));
         }
         if (g.t == 0) {
-            R[0].v[0] = csub(R[0].v[0], cmk(gold[0], 0.0));
-            R[2].v[0] = csub(R[2].v[0], cmk(gold[1], 0.0));
+            R[0].v[0] = csub(R[0].v[0], cmk(gold0, 0.0));
+            R[2].v[0] = csub(R[2].v[0], cmk(gold1, 0.0));
         }
     }
     CVec Un[3], Pn;
-    double gnew[2] = {0.0, 0.0};
-    if (mi.mean && mi.valid) {
-        solve_mean(g, a, bk, R, Un, Pn, gnew);
+    if (meanmode) {
+        double gnew[2] = {0.0, 0.0};
+        solve_mean(g, a, a.bank, R, Un, Pn, gnew);
         if (g.t == 0 && a.grad_out) {
             a.grad_out[0] = gnew[0];
             a.grad_out[1] = gnew[1];
         }
     } else if (mi.nyq || !mi.valid) {
+#pragma unroll
         for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
         zero_vec<L>(Pn);
     } else {
-        solve_nonzero(g, a, bk, mi, R, Un, Pn);
+        double2* st = smem + (size_t)tile.mloc * 3 * (4 * L);
+        stash(st, g, R[0]);
+        stash(st + 4 * L, g, R[1]);
+        stash(st + 8 * L, g, R[2]);
+        solve_nonzero(g, a, a.bank, mi, st, st + 4 * L, st + 8 * L, Un, Pn);
     }
-    for (int c = 0; c < 3; ++c) tile.store(a.U[c], g, Un[c]);
+    __syncthreads();  // LDS stash regions are reused by the stores below
+    tile.store(a.U[0], g, Un[0]);
+    tile.store(a.U[1], g, Un[1]);
+    tile.store(a.U[2], g, Un[2]);
     tile.store(a.P, g, Pn);
     if (a.out6[0]) inverse_six(tile, bufB, a, g, mi, Un);
 }
@@ -414,6 +462,7 @@ __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     Group<L> g;
     ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
     CVec U[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
     inverse_six(tile, bufB, a, g, mi, U);
 }
@@ -428,6 +477,7 @@ __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
     Group<L> g;
     ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
     CVec R[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) R[c] = tile.load(a.Rin[c], g);
     CVec Un[3], Pn;
     double gnew[2] = {0.0, 0.0};
@@ -441,8 +491,14 @@ __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
         for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
         zero_vec<L>(Pn);
     } else {
-        solve_nonzero(g, a, a.bank, mi, R, Un, Pn);
+        double2* st = smem + (size_t)tile.mloc * 3 * (4 * L);
+        stash(st, g, R[0]);
+        stash(st + 4 * L, g, R[1]);
+        stash(st + 8 * L, g, R[2]);
+        solve_nonzero(g, a, a.bank, mi, st, st + 4 * L, st + 8 * L, Un, Pn);
     }
+    __syncthreads();
+#pragma unroll
     for (int c = 0; c < 3; ++c) tile.store(a.U[c], g, Un[c]);
     tile.store(a.P, g, Pn);
 }
@@ -547,11 +603,14 @@ __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+int y_group_lanes(int Ny);
+size_t y_lds_bytes(int Ny, int ld);
+
 template <int L>
 static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     constexpr int M = YNT / L;
     dim3 grid((unsigned)((a.gd.B + M - 1) / M));
-    size_t lds = 2 * (size_t)M * a.ld * sizeof(double2);
+    size_t lds = y_lds_bytes(a.gd.Ny, a.ld);
     switch (which) {
         case Y_STAGE: hipLaunchKernelGGL(ystage_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_PROLOGUE: hipLaunchKernelGGL(yprologue_kernel<L>, grid, dim3(YNT), lds, st, a); break;
@@ -575,9 +634,13 @@ int y_group_lanes(int Ny) {
     return 0;
 }
 
+// two FFT/transpose buffers of M lines, or the solve's lane-private stash of
+// three vectors per mode (3 * M * 4L = 12 * YNT slots), whichever is larger
 size_t y_lds_bytes(int Ny, int ld) {
     int L = y_group_lanes(Ny);
-    return L ? 2 * (size_t)(YNT / L) * ld * sizeof(double2) : 0;
+    if (!L) return 0;
+    size_t fft = 2 * (size_t)(YNT / L) * ld, stash = 12 * (size_t)YNT;
+    return (fft > stash ? fft : stash) * sizeof(double2);
 }
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
