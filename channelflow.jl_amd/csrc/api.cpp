@@ -115,7 +115,13 @@ static int make_plan(cf_problem* h, int n, FFTPlan& plan) {
     if ((int)radix.size() > MAX_FACTORS) return fail(CF_ERR_UNSUPPORTED, "FFT length has too many factors");
     plan.n = n;
     plan.nf = (int)radix.size();
-    for (int i = 0; i < plan.nf; ++i) plan.radix[i] = radix[i];
+    int Ns = 1;
+    for (int i = 0; i < plan.nf; ++i) {
+        plan.radix[i] = radix[i];
+        plan.mul_m[i] = recip_mul((unsigned)(n / radix[i]));
+        plan.mul_ns[i] = recip_mul((unsigned)Ns);
+        Ns *= radix[i];
+    }
     if (n == 1) plan.nf = 0;
     std::vector<double> tw;
     twiddles(n, tw);
@@ -252,7 +258,8 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
     h->KT = std::min(h->KT, gd.Nxh);
     h->xld = M + 1;
     {
-        size_t per_line = 2 * (size_t)h->xld * sizeof(double2) + 6 * (size_t)gd.Nxp * sizeof(double);
+        size_t per_line = (gd.Nxp % 2 == 0) ? 12 * (size_t)h->xld * sizeof(double2)
+                                            : 2 * (size_t)h->xld * sizeof(double2) + 6 * (size_t)gd.Nxp * sizeof(double);
         h->LPB = (int)std::max<size_t>(1, std::min<size_t>(8, LDS_MAX / per_line));
     }
     if (y_lds_bytes(gd.Ny, h->yld) > LDS_MAX) return bail(fail(CF_ERR_UNSUPPORTED, "wall-normal tile exceeds LDS"));

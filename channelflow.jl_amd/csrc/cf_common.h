@@ -7,12 +7,21 @@ namespace cf {
 constexpr int MAX_FACTORS = 16;
 
 // FFT of length n factored into radices; tw[t] = exp(-2 pi i t/n) (device).
+// Per pass f (radix R, span Ns = product of earlier radices): reciprocal
+// multipliers q(x) = (x * mul) >> 32 with mul = floor(2^32/d) + 1 for the
+// divisors d = n/R (mul_m) and d = Ns (mul_ns); exact for x*d < 2^32.
 struct FFTPlan {
     int n;
     int nf;
     int radix[MAX_FACTORS];
+    unsigned long long mul_m[MAX_FACTORS], mul_ns[MAX_FACTORS];
     const double2* tw;
 };
+
+__host__ __device__ inline unsigned long long recip_mul(unsigned d) { return (1ull << 32) / d + 1ull; }
+__host__ __device__ __forceinline__ unsigned fast_div(unsigned x, unsigned long long mul) {
+    return (unsigned)(((unsigned long long)x * mul) >> 32);
+}
 
 // Grid sizes (src/grids.jl:22-74): resolved (Nx, Ny, Nz), padded Nxp, Nzp,
 // half spectrum Nxh; P = Ny - 1; B = Nxh*Nz modes; S = B*Ny coefficients.

@@ -372,29 +372,31 @@ __device__ __forceinline__ void dft_small(double2* v, int sign) {
 
 template <int R>
 __device__ __forceinline__ void stockham_pass_fixed(const double2* src, double2* dst, int n, int nlines, int ld, int Ns,
+                                                    unsigned long long mul_m, unsigned long long mul_ns,
                                                     const double2* tw, int sign, int tid, int nthr) {
     const int m = n / R;
     const int twstep = n / (Ns * R);
     const int total = nlines * m;
     for (int idx = tid; idx < total; idx += nthr) {
-        int line = idx / m;
+        int line = (int)fast_div((unsigned)idx, mul_m);
         int j = idx - line * m;
+        int jq = (int)fast_div((unsigned)j, mul_ns);
+        int jm = j - jq * Ns;
         const double2* s = src + (size_t)line * ld;
         double2* d = dst + (size_t)line * ld;
-        int jm = j % Ns;
         double2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             double2 x = s[j + r * m];
             if (r > 0) {
-                double2 w = tw[(r * jm * twstep) % n];
+                double2 w = tw[r * jm * twstep];  // r*jm*twstep < n
                 if (sign > 0) w = cconj(w);
                 x = cmul(x, w);
             }
             v[r] = x;
         }
         dft_small<R>(v, sign);
-        int idxD = (j / Ns) * Ns * R + jm;
+        int idxD = jq * Ns * R + jm;
 #pragma unroll
         for (int r = 0; r < R; ++r) d[idxD + r * Ns] = v[r];
     }
@@ -438,12 +440,13 @@ __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPla
     double2* dst = b;
     for (int f = 0; f < plan.nf; ++f) {
         int R = plan.radix[f];
+        const unsigned long long mm = plan.mul_m[f], mn = plan.mul_ns[f];
         switch (R) {
-            case 2: stockham_pass_fixed<2>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
-            case 3: stockham_pass_fixed<3>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
-            case 4: stockham_pass_fixed<4>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
-            case 5: stockham_pass_fixed<5>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
-            case 8: stockham_pass_fixed<8>(src, dst, plan.n, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+            case 2: stockham_pass_fixed<2>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
+            case 3: stockham_pass_fixed<3>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
+            case 4: stockham_pass_fixed<4>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
+            case 5: stockham_pass_fixed<5>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
+            case 8: stockham_pass_fixed<8>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
             default: stockham_pass_generic(src, dst, plan.n, R, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
         }
         __syncthreads();

@@ -171,8 +171,66 @@ struct XCtx {
     }
 };
 
+// Rotational x pass for even Nxp: the six fields of LPB lines are one batch
+// of 6*LPB length-M FFTs; the product is formed from the transformed batch
+// in LDS and packed directly as the input of the 3*LPB forward FFTs.
+__device__ __forceinline__ void xpass_rotational_even(const XArgs& a, double2* smem) {
+    const int Nxp = a.gd.Nxp, Nxh = a.gd.Nxh, M = Nxp / 2, LPB = a.LPB, ld = a.ld, tid = threadIdx.x;
+    const long long gl0 = (long long)blockIdx.x * LPB, nlines = (long long)a.gd.Nzp * a.gd.Ny;
+    const bool xeven = (a.gd.Nx % 2 == 0);
+    double2* bufA = smem;
+    double2* bufB = smem + 6 * LPB * ld;
+    const int nX = M + 1;
+    // stage X_k, k = 0..M (padding, Nyquist plane and Im of DC/Nyquist removed)
+    for (int idx = tid; idx < 6 * LPB * nX; idx += XNT) {
+        int q = idx / nX, k = idx - q * nX;
+        int f = q / LPB, ln = q - f * LPB;
+        long long gl = gl0 + ln;
+        double2 v = czero();
+        if (gl < nlines && k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = a.in[f][(size_t)gl * Nxh + k];
+        if (k == 0 || k == M) v.y = 0.0;
+        bufB[q * ld + k] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 6 * LPB * M; idx += XNT) {
+        int q = idx / M, k = idx - q * M;
+        double2 Xk = bufB[q * ld + k];
+        double2 Xc = cconj(bufB[q * ld + (M - k)]);
+        double2 w = cconj(a.twx[k]);  // exp(+2 pi i k / Nxp)
+        bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
+    }
+    __syncthreads();
+    double2* res = lds_fft(bufA, bufB, a.plan, 6 * LPB, ld, +1, tid, XNT);
+    double2* nxt = (res == bufA) ? bufB : bufA;
+    // u x w at x = 2j (real parts) and x = 2j+1 (imaginary parts)
+    for (int idx = tid; idx < LPB * M; idx += XNT) {
+        int ln = idx / M, j = idx - ln * M;
+        double2 u0 = res[(0 * LPB + ln) * ld + j], u1 = res[(1 * LPB + ln) * ld + j], u2 = res[(2 * LPB + ln) * ld + j];
+        double2 w0 = res[(3 * LPB + ln) * ld + j], w1 = res[(4 * LPB + ln) * ld + j], w2 = res[(5 * LPB + ln) * ld + j];
+        nxt[(0 * LPB + ln) * ld + j] = cmk(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+        nxt[(1 * LPB + ln) * ld + j] = cmk(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+        nxt[(2 * LPB + ln) * ld + j] = cmk(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+    }
+    __syncthreads();
+    double2* r2 = lds_fft(nxt, res, a.plan, 3 * LPB, ld, -1, tid, XNT);
+    for (int idx = tid; idx < 3 * LPB * Nxh; idx += XNT) {
+        int q = idx / Nxh, k = idx - q * Nxh;
+        int c = q / LPB, ln = q - c * LPB;
+        long long gl = gl0 + ln;
+        if (gl >= nlines) continue;
+        double2 Zk = r2[q * ld + k];
+        double2 Zc = cconj(r2[q * ld + ((M - k) % M)]);
+        double2 w = a.twx[k];  // exp(-2 pi i k / Nxp)
+        a.out[c][(size_t)gl * Nxh + k] = csub(cscale(cadd(Zk, Zc), 0.5), cscale(cmuli(cmul(w, csub(Zk, Zc))), 0.5));
+    }
+}
+
 __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
     extern __shared__ double2 smem[];
+    if (a.mode == X_ROTATIONAL && a.gd.Nxp % 2 == 0) {
+        xpass_rotational_even(a, smem);
+        return;
+    }
     XCtx c(a, smem);
     double* phys = reinterpret_cast<double*>(smem + 2 * a.LPB * a.ld);
     const int Nxp = c.Nxp, LPB = c.LPB;
@@ -221,6 +279,7 @@ hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st) {
 }
 
 size_t x_lds_bytes(const XArgs& a) {
+    if (a.mode == X_ROTATIONAL && a.gd.Nxp % 2 == 0) return 2 * 6 * (size_t)a.LPB * a.ld * sizeof(double2);
     int nphys = (a.mode == X_ROTATIONAL) ? 6 : 1;
     return 2 * (size_t)a.LPB * a.ld * sizeof(double2) + (size_t)nphys * a.LPB * a.gd.Nxp * sizeof(double);
 }
