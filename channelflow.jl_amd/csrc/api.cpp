@@ -262,7 +262,7 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
                                             : 2 * (size_t)h->xld * sizeof(double2) + 6 * (size_t)gd.Nxp * sizeof(double);
         h->LPB = (int)std::max<size_t>(1, std::min<size_t>(8, LDS_MAX / per_line));
     }
-    if (y_lds_bytes(gd.Ny, h->yld) > LDS_MAX) return bail(fail(CF_ERR_UNSUPPORTED, "wall-normal tile exceeds LDS"));
+    if (y_lds_bytes(gd.Ny, h->yld) > 160 * 1024) return bail(fail(CF_ERR_UNSUPPORTED, "wall-normal tile exceeds LDS"));
     // workspaces
     if ((rc = dalloc(h, &h->six, 6 * (size_t)gd.S))) return bail(rc);
     if ((rc = dalloc(h, &h->zbuf, 6 * (size_t)gd.Sz))) return bail(rc);

@@ -606,11 +606,22 @@ __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
 int y_group_lanes(int Ny);
 size_t y_lds_bytes(int Ny, int ld);
 
+template <typename K>
+static void allow_lds(K kernel, size_t lds) {
+    if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 template <int L>
 static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     constexpr int M = YNT / L;
     dim3 grid((unsigned)((a.gd.B + M - 1) / M));
     size_t lds = y_lds_bytes(a.gd.Ny, a.ld);
+    allow_lds(ystage_kernel<L>, lds);
+    allow_lds(yprologue_kernel<L>, lds);
+    allow_lds(ysolve_kernel<L>, lds);
+    allow_lds(yneglap_kernel<L>, lds);
+    allow_lds(ypressure_kernel<L>, lds);
+    allow_lds(ydct_kernel<L>, lds);
     switch (which) {
         case Y_STAGE: hipLaunchKernelGGL(ystage_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_PROLOGUE: hipLaunchKernelGGL(yprologue_kernel<L>, grid, dim3(YNT), lds, st, a); break;
