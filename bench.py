@@ -10,7 +10,9 @@ A step is one CNRK2 step! call (three stages) on device-resident fields.
 
 value = steps/s on one GPU.  roofline = the dominant kernel class (by device
 time) with its algorithmic bytes per launch over its event-timed mean launch
-duration.  cpu_baseline = the oracle (C++ restatement of the reference CPU
+duration; roofline.traffic = that kernel's HBM bytes per launch measured by
+rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated), read from
+profiles/hbm_traffic_<workload>.json when present.  cpu_baseline = the oracle (C++ restatement of the reference CPU
 step, all host cores) on a bounded sample of the same workload.
 """
 import argparse
@@ -149,6 +151,14 @@ def main():
                                            (per_step_ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
     step_gbs = step_bytes * steps_per_s / 1e9
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.workload}.json")
+    if os.path.exists(tpath):  # rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated
+        with open(tpath) as f:
+            rec = json.load(f)["kernels"].get(CLASSES[dom])
+        if rec:
+            traffic = rec["fetch"] + rec["write"]
+
     cpu = None
     if args.cpu_steps > 0:
         cps, cores, secs = cpu_baseline(w, prof, kw, U0, P0, args.cpu_steps)
@@ -176,7 +186,7 @@ def main():
                    "kernel_breakdown": breakdown},
         "roofline": {"bound": "hbm", "kernel": CLASSES[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

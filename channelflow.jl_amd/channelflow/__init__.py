@@ -154,8 +154,9 @@ class CNRK2:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
-            _lib.load().cf_problem_destroy(h)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None
+        if h and lib is not None:  # at interpreter exit the module may be torn down first
+            lib.cf_problem_destroy(h)
             self.h = None
 
 

@@ -431,10 +431,23 @@ __device__ __noinline__ void stockham_pass_generic(const double2* src, double2* 
     }
 }
 
-// Run all passes; returns the buffer holding the result.  Must be called by
-// all threads of the block (contains __syncthreads).
-__device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
-                                   int nthr) {
+// Ordering of LDS accesses among the lanes of one wavefront: LDS operations
+// of a wave are performed in issue order, so a compiler-level barrier with
+// wavefront-scope fences is all a wave-private exchange needs.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Run all passes; returns the buffer holding the result.  WAVE = false: the
+// lines are shared by the block and every thread of the block must call
+// (passes end in __syncthreads); WAVE = true: the lines are private to one
+// wavefront, tid/nthr are its lane/64 and passes end in wave_sync().
+// tw: twiddle table exp(-2 pi i t/n) (global or LDS).
+template <bool WAVE = false>
+__device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, const double2* tw, int nlines, int ld,
+                                            int sign, int tid, int nthr) {
     int Ns = 1;
     double2* src = a;
     double2* dst = b;
@@ -442,20 +455,25 @@ __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPla
         int R = plan.radix[f];
         const unsigned long long mm = plan.mul_m[f], mn = plan.mul_ns[f];
         switch (R) {
-            case 2: stockham_pass_fixed<2>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
-            case 3: stockham_pass_fixed<3>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
-            case 4: stockham_pass_fixed<4>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
-            case 5: stockham_pass_fixed<5>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
-            case 8: stockham_pass_fixed<8>(src, dst, plan.n, nlines, ld, Ns, mm, mn, plan.tw, sign, tid, nthr); break;
-            default: stockham_pass_generic(src, dst, plan.n, R, nlines, ld, Ns, plan.tw, sign, tid, nthr); break;
+            case 2: stockham_pass_fixed<2>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            case 3: stockham_pass_fixed<3>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            case 4: stockham_pass_fixed<4>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            case 5: stockham_pass_fixed<5>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            case 8: stockham_pass_fixed<8>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            default: stockham_pass_generic(src, dst, plan.n, R, nlines, ld, Ns, tw, sign, tid, nthr); break;
         }
-        __syncthreads();
+        if constexpr (WAVE) wave_sync();
+        else __syncthreads();
         Ns *= R;
         double2* t = src;
         src = dst;
         dst = t;
     }
     return src;
+}
+__device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
+                                            int nthr) {
+    return lds_fft<false>(a, b, plan, plan.tw, nlines, ld, sign, tid, nthr);
 }
 
 }  // namespace cf

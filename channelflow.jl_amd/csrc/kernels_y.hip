@@ -41,101 +41,116 @@ __device__ __forceinline__ ModeInfo mode_info(const GridDims& gd, long long s) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile movement between HBM (k,l,n) rows and group registers via LDS.
+// Per-wavefront tile.  Wave w of block b owns the W = 64/L consecutive modes
+// starting at b*M + w*W, and a private LDS region holding two FFT lines per
+// mode (the DCT-I staging) that also serves as the solve's stash.  Spectral
+// rows move between HBM and the group registers directly (lane t of a mode
+// reads/writes degrees 4t..4t+3; the W modes of one row are contiguous), so
+// no block-level barrier is needed anywhere after the twiddle table load.
 // ---------------------------------------------------------------------------
 template <int L>
 struct Tile {
-    static constexpr int M = YNT / L;
+    static constexpr int M = YNT / L;  // modes per block
+    static constexpr int W = 64 / L;   // modes per wavefront
     const GridDims& gd;
-    long long s0;
-    int tid, mloc, ld;
-    double2* buf;
-    __device__ __forceinline__ Tile(const GridDims& g_, double2* b, int ld_) : gd(g_), ld(ld_), buf(b) {
-        tid = threadIdx.x;
-        mloc = tid / L;
-        s0 = (long long)blockIdx.x * M;
+    long long s, sw0;  // this group's mode, the wave's first mode
+    int lane, mw, ld;
+    double2 *bufA, *bufB, *stash;
+    const double2* tw;
+    // region: LDS slots per wave (y_region_slots); smem starts with the
+    // twiddle table of the DCT plan (ntw entries)
+    __device__ __forceinline__ Tile(const GridDims& g_, double2* smem, int ntw, int ld_, int region)
+        : gd(g_), ld(ld_) {
+        const int tid = threadIdx.x, wave = tid >> 6;
+        lane = tid & 63;
+        mw = lane / L;
+        sw0 = (long long)blockIdx.x * M + wave * W;
+        s = sw0 + mw;
+        tw = smem;
+        bufA = smem + ntw + (size_t)wave * region;
+        bufB = bufA + W * ld;
+        stash = bufA + mw * 12 * L;
     }
-    // spectral field rows n = 0..P -> registers (zero beyond P and for absent modes)
-    __device__ __forceinline__ CVec load(const double2* __restrict__ src, const Group<L>& g) {
-        const int Ny = gd.Ny;
-        for (int idx = tid; idx < Ny * M; idx += YNT) {
-            int n = idx / M, m = idx - n * M;
-            long long s = s0 + m;
-            buf[m * ld + n] = (s < gd.B) ? src[s + gd.B * n] : czero();
-        }
-        __syncthreads();
+    // spectral field column of this group's mode (zero beyond P / absent mode)
+    __device__ __forceinline__ CVec load(const double2* __restrict__ src, const Group<L>& g) const {
         CVec v;
+        const bool ok = s < gd.B;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int n = 4 * g.t + e;
-            v.v[e] = (n < Ny) ? buf[mloc * ld + n] : czero();
+            v.v[e] = (ok && n < gd.Ny) ? src[s + gd.B * n] : czero();
         }
-        __syncthreads();
         return v;
     }
-    __device__ __forceinline__ void store(double2* __restrict__ dst, const Group<L>& g, const CVec& v) {
-        const int Ny = gd.Ny;
+    __device__ __forceinline__ void store(double2* __restrict__ dst, const Group<L>& g, const CVec& v) const {
+        const bool ok = s < gd.B;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int n = 4 * g.t + e;
-            if (n < Ny) buf[mloc * ld + n] = v.v[e];
+            if (ok && n < gd.Ny) dst[s + gd.B * n] = v.v[e];
         }
-        __syncthreads();
-        for (int idx = tid; idx < Ny * M; idx += YNT) {
-            int n = idx / M, m = idx - n * M;
-            long long s = s0 + m;
-            if (s < gd.B) dst[s + gd.B * n] = buf[m * ld + n];
-        }
-        __syncthreads();
     }
     // forward Chebyshev transform of y-physical rows (values at y_j, j=0..P):
     // REDFT00, times norm, endpoint 1/2, Nyquist modes zeroed (normalize_forward!).
-    __device__ __forceinline__ CVec dct_forward(const double2* __restrict__ src, double2* bufB, const FFTPlan& plan, double norm,
-                                const Group<L>& g, bool zero_mode) {
+    __device__ __forceinline__ CVec dct_forward(const double2* __restrict__ src, const FFTPlan& plan, double norm,
+                                                const Group<L>& g, bool zero_mode) {
         const int P = gd.P;
-        for (int idx = tid; idx < (P + 1) * M; idx += YNT) {
-            int n = idx / M, m = idx - n * M;
-            long long s = s0 + m;
-            double2 v = (s < gd.B) ? src[s + gd.B * n] : czero();
-            buf[m * ld + n] = v;
-            if (n >= 1 && n <= P - 1) buf[m * ld + 2 * P - n] = v;
+        for (int idx = lane; idx < (P + 1) * W; idx += 64) {
+            int n = idx / W, m = idx - n * W;
+            long long sm = sw0 + m;
+            double2 v = (sm < gd.B) ? src[sm + gd.B * n] : czero();
+            bufA[m * ld + n] = v;
+            if (n >= 1 && n <= P - 1) bufA[m * ld + 2 * P - n] = v;
         }
-        __syncthreads();
-        double2* res = lds_fft(buf, bufB, plan, M, ld, -1, tid, YNT);
+        wave_sync();
+        double2* res = lds_fft<true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
         CVec v;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int n = 4 * g.t + e;
             double w = (n == 0 || n == P) ? 0.5 * norm : norm;
-            v.v[e] = (n <= P && !zero_mode) ? cscale(res[mloc * ld + n], w) : czero();
+            v.v[e] = (n <= P && !zero_mode) ? cscale(res[mw * ld + n], w) : czero();
         }
-        __syncthreads();
+        wave_sync();
         return v;
     }
     // inverse: coefficient registers -> values at y_j (endpoints doubled,
     // REDFT00, x1/2: InverseFFT!'s chebyshev + copy_to_padded! scale).
-    __device__ __forceinline__ void dct_inverse(double2* __restrict__ dst, double2* bufB, const FFTPlan& plan, const Group<L>& g,
-                                const CVec& a) {
+    __device__ __forceinline__ void dct_inverse(double2* __restrict__ dst, const FFTPlan& plan, const Group<L>& g,
+                                                const CVec& a) {
         const int P = gd.P;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int n = 4 * g.t + e;
             if (n <= P) {
                 double2 x = a.v[e];
-                buf[mloc * ld + n] = (n == 0 || n == P) ? cscale(x, 2.0) : x;
-                if (n >= 1 && n <= P - 1) buf[mloc * ld + 2 * P - n] = x;
+                bufA[mw * ld + n] = (n == 0 || n == P) ? cscale(x, 2.0) : x;
+                if (n >= 1 && n <= P - 1) bufA[mw * ld + 2 * P - n] = x;
             }
         }
-        __syncthreads();
-        double2* res = lds_fft(buf, bufB, plan, M, ld, -1, tid, YNT);
-        for (int idx = tid; idx < (P + 1) * M; idx += YNT) {
-            int n = idx / M, m = idx - n * M;
-            long long s = s0 + m;
-            if (s < gd.B) dst[s + gd.B * n] = cscale(res[m * ld + n], 0.5);
+        wave_sync();
+        double2* res = lds_fft<true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
+        for (int idx = lane; idx < (P + 1) * W; idx += 64) {
+            int n = idx / W, m = idx - n * W;
+            long long sm = sw0 + m;
+            if (sm < gd.B) dst[sm + gd.B * n] = cscale(res[m * ld + n], 0.5);
         }
-        __syncthreads();
+        wave_sync();
     }
 };
+
+// LDS slots of one wave's region: two FFT lines per mode, or the solve's
+// stash of three coefficient vectors (4L slots each) per mode.
+template <int L>
+__host__ __device__ constexpr int y_region_slots(int ld) {
+    return (2 * (64 / L) * ld > 12 * 64) ? 2 * (64 / L) * ld : 12 * 64;
+}
+
+// Block prologue of every y kernel: the DCT twiddle table into LDS.
+__device__ __forceinline__ void load_twiddles(double2* smem, const FFTPlan& plan) {
+    for (int i = threadIdx.x; i < plan.n; i += YNT) smem[i] = plan.tw[i];
+    __syncthreads();
+}
 
 // Lane-private LDS stash of a coefficient vector (each lane writes and reads
 // back only its own four slots, so no barrier is needed between the two).
@@ -175,7 +190,7 @@ __device__ __forceinline__ void load_fac(const double* __restrict__ f, long long
 // V = U + Ub e_x at (0,0); Omega = curl V (operators.jl:172-190); values
 // of the six fields at the Lobatto points into out6.
 template <int L>
-__device__ __forceinline__ void inverse_six(Tile<L>& tile, double2* bufB, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
+__device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
                             const CVec U[3]) {
     CVec V[3] = {U[0], U[1], U[2]};
     if (mi.mean && mi.valid) {
@@ -185,22 +200,22 @@ __device__ __forceinline__ void inverse_six(Tile<L>& tile, double2* bufB, const 
             if (n < a.gd.Ny) V[0].v[e] = cadd(V[0].v[e], cmk(a.base[n], 0.0));
         }
     }
-    tile.dct_inverse(a.out6[0], bufB, a.dct, g, V[0]);
-    tile.dct_inverse(a.out6[1], bufB, a.dct, g, V[1]);
-    tile.dct_inverse(a.out6[2], bufB, a.dct, g, V[2]);
+    tile.dct_inverse(a.out6[0], a.dct, g, V[0]);
+    tile.dct_inverse(a.out6[1], a.dct, g, V[1]);
+    tile.dct_inverse(a.out6[2], a.dct, g, V[2]);
     CVec d3 = cheb_deriv(g, V[2]);
     CVec W;
 #pragma unroll
     for (int e = 0; e < 4; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V[1].v[e], mi.kz)));  // dy V3 - i l V2
-    tile.dct_inverse(a.out6[3], bufB, a.dct, g, W);
+    tile.dct_inverse(a.out6[3], a.dct, g, W);
 #pragma unroll
     for (int e = 0; e < 4; ++e)  // i (l V1 - k V3)
         W.v[e] = cmuli(csub(cscale(V[0].v[e], mi.kz), cscale(V[2].v[e], mi.kx)));
-    tile.dct_inverse(a.out6[4], bufB, a.dct, g, W);
+    tile.dct_inverse(a.out6[4], a.dct, g, W);
     CVec d1 = cheb_deriv(g, V[0]);
 #pragma unroll
     for (int e = 0; e < 4; ++e) W.v[e] = csub(cmuli(cscale(V[1].v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
-    tile.dct_inverse(a.out6[5], bufB, a.dct, g, W);
+    tile.dct_inverse(a.out6[5], a.dct, g, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -295,9 +310,11 @@ __device__ __forceinline__ void solve_nonzero(const Group<L>& g, const YArgs& a,
 
 // mean mode (meanmode.jl:103-150); returns the new gradients
 template <int L>
-__device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, const Bank& bk, const CVec R[3], CVec Uout[3], CVec& Pout,
-                           double gout[2]) {
+__device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, const Bank& bk, const double2* Rs0,
+                                           const double2* Rs1, const double2* Rs2, CVec Uout[3], CVec& Pout,
+                                           double gout[2]) {
     const int P = a.gd.P, Ny = a.gd.Ny;
+    const CVec R[3] = {unstash(Rs0, g), unstash(Rs1, g), unstash(Rs2, g)};
     // pressure: integrate Ry through degree P-1, zero bulk mean
     {
         int up1 = g.up(1), dn1 = g.down(1);
@@ -363,14 +380,14 @@ __device__ __forceinline__ CVec laplacian(const Group<L>& g, const CVec& a, doub
 // Stokes solve, state write, inverse transforms for the next stage.
 // ---------------------------------------------------------------------------
 template <int L>
-__global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
+// two waves per SIMD (<= 256 registers per lane) at the price of a small
+// spill: one wave per SIMD leaves the wave-serial scans and LDS FFTs unhidden
+__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    const int ld = a.ld;
-    double2* bufA = smem;
-    double2* bufB = smem + (Tile<L>::M) * ld;
-    Tile<L> tile(a.gd, bufA, ld);
+    load_twiddles(smem, a.dct);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g;
-    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny;
     const bool meanmode = mi.mean && mi.valid;
 
@@ -381,7 +398,7 @@ __global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
     CVec R[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        CVec Qc = tile.dct_forward(a.Nphys[c], bufB, a.dct, a.norm, g, mi.nyq);
+        CVec Qc = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
         double bN = 0.0;
         if (a.bulk && meanmode && c != 1) bN = bulk_mean(g, Qc).x;
         if (a.stage != 0) {
@@ -411,6 +428,10 @@ __global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
             x = cfma(a.nu, lap.v[e], x);
             x = csub(x, dP.v[e]);
             R[c].v[e] = cfma(a.weight, Qc.v[e], x);
+            // materialise R_c here: otherwise the RHS arithmetic is sunk into
+            // the solve branches below and every input of all three
+            // components stays live until then
+            asm volatile("" : "+v"(R[c].v[e].x), "+v"(R[c].v[e].y));
         }
     }
     if (meanmode) {
@@ -424,10 +445,14 @@ __global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
             R[2].v[0] = csub(R[2].v[0], cmk(gold1, 0.0));
         }
     }
+    double2* st = tile.stash;
+    stash(st, g, R[0]);
+    stash(st + 4 * L, g, R[1]);
+    stash(st + 8 * L, g, R[2]);
     CVec Un[3], Pn;
     if (meanmode) {
         double gnew[2] = {0.0, 0.0};
-        solve_mean(g, a, a.bank, R, Un, Pn, gnew);
+        solve_mean(g, a, a.bank, st, st + 4 * L, st + 8 * L, Un, Pn, gnew);
         if (g.t == 0 && a.grad_out) {
             a.grad_out[0] = gnew[0];
             a.grad_out[1] = gnew[1];
@@ -437,52 +462,49 @@ __global__ void __launch_bounds__(YNT) ystage_kernel(YArgs a) {
         for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
         zero_vec<L>(Pn);
     } else {
-        double2* st = smem + (size_t)tile.mloc * 3 * (4 * L);
-        stash(st, g, R[0]);
-        stash(st + 4 * L, g, R[1]);
-        stash(st + 8 * L, g, R[2]);
         solve_nonzero(g, a, a.bank, mi, st, st + 4 * L, st + 8 * L, Un, Pn);
     }
-    __syncthreads();  // LDS stash regions are reused by the stores below
+    wave_sync();  // the stash aliases the wave's FFT lines reused below
     tile.store(a.U[0], g, Un[0]);
     tile.store(a.U[1], g, Un[1]);
     tile.store(a.U[2], g, Un[2]);
     tile.store(a.P, g, Pn);
-    if (a.out6[0]) inverse_six(tile, bufB, a, g, mi, Un);
+    if (a.out6[0]) inverse_six(tile, a, g, mi, Un);
 }
 
 // prologue: state U -> six y-physical fields (velocity and vorticity)
 template <int L>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    const int ld = a.ld;
-    double2* bufA = smem;
-    double2* bufB = smem + (Tile<L>::M) * ld;
-    Tile<L> tile(a.gd, bufA, ld);
+    load_twiddles(smem, a.dct);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g;
-    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    ModeInfo mi = mode_info(a.gd, tile.s);
     CVec U[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
-    inverse_six(tile, bufB, a, g, mi, U);
+    inverse_six(tile, a, g, mi, U);
 }
 
 // Stokes solve! of a bank: R (3 spectral) -> U (3), P (stokes.jl:81-108)
 template <int L>
 __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    const int ld = a.ld;
-    double2* bufA = smem;
-    Tile<L> tile(a.gd, bufA, ld);
+    load_twiddles(smem, a.dct);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g;
-    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    ModeInfo mi = mode_info(a.gd, tile.s);
     CVec R[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) R[c] = tile.load(a.Rin[c], g);
     CVec Un[3], Pn;
     double gnew[2] = {0.0, 0.0};
+    double2* st = tile.stash;
+    stash(st, g, R[0]);
+    stash(st + 4 * L, g, R[1]);
+    stash(st + 8 * L, g, R[2]);
     if (mi.mean && mi.valid) {
-        solve_mean(g, a, a.bank, R, Un, Pn, gnew);
+        solve_mean(g, a, a.bank, st, st + 4 * L, st + 8 * L, Un, Pn, gnew);
         if (g.t == 0 && a.grad_out) {
             a.grad_out[0] = gnew[0];
             a.grad_out[1] = gnew[1];
@@ -491,13 +513,8 @@ __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
         for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
         zero_vec<L>(Pn);
     } else {
-        double2* st = smem + (size_t)tile.mloc * 3 * (4 * L);
-        stash(st, g, R[0]);
-        stash(st + 4 * L, g, R[1]);
-        stash(st + 8 * L, g, R[2]);
         solve_nonzero(g, a, a.bank, mi, st, st + 4 * L, st + 8 * L, Un, Pn);
     }
-    __syncthreads();
 #pragma unroll
     for (int c = 0; c < 3; ++c) tile.store(a.U[c], g, Un[c]);
     tile.store(a.P, g, Pn);
@@ -507,9 +524,10 @@ __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    Tile<L> tile(a.gd, smem, a.ld);
+    load_twiddles(smem, a.dct);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g;
-    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < 3; ++c) {
         CVec U = tile.load(a.U[c], g);
         CVec lap = laplacian(g, U, mi.kappa2);
@@ -524,9 +542,10 @@ __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    Tile<L> tile(a.gd, smem, a.ld);
+    load_twiddles(smem, a.dct);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g;
-    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    ModeInfo mi = mode_info(a.gd, tile.s);
     const int P = a.gd.P, Ny = a.gd.Ny;
     CVec A[3];
     for (int c = 0; c < 3; ++c) A[c] = tile.load(a.Rin[c], g);
@@ -583,19 +602,17 @@ __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
     extern __shared__ double2 smem[];
-    const int ld = a.ld;
-    double2* bufA = smem;
-    double2* bufB = smem + (Tile<L>::M) * ld;
-    Tile<L> tile(a.gd, bufA, ld);
+    load_twiddles(smem, a.dct);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g;
-    ModeInfo mi = mode_info(a.gd, tile.s0 + tile.mloc);
+    ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < a.nfields; ++c) {
         if (dir < 0) {
-            CVec v = tile.dct_forward(a.Nphys[c], bufB, a.dct, a.norm, g, mi.nyq);
+            CVec v = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
             tile.store(a.Rout[c], g, v);
         } else {
             CVec v = tile.load(a.Rin[c], g);
-            tile.dct_inverse(a.out6[c], bufB, a.dct, g, v);
+            tile.dct_inverse(a.out6[c], a.dct, g, v);
         }
     }
 }
@@ -645,13 +662,19 @@ int y_group_lanes(int Ny) {
     return 0;
 }
 
-// two FFT/transpose buffers of M lines, or the solve's lane-private stash of
-// three vectors per mode (3 * M * 4L = 12 * YNT slots), whichever is larger
+// DCT twiddle table (2P entries) followed by one region per wavefront
 size_t y_lds_bytes(int Ny, int ld) {
     int L = y_group_lanes(Ny);
-    if (!L) return 0;
-    size_t fft = 2 * (size_t)(YNT / L) * ld, stash = 12 * (size_t)YNT;
-    return (fft > stash ? fft : stash) * sizeof(double2);
+    int region = 0;
+    switch (L) {
+        case 4: region = y_region_slots<4>(ld); break;
+        case 8: region = y_region_slots<8>(ld); break;
+        case 16: region = y_region_slots<16>(ld); break;
+        case 32: region = y_region_slots<32>(ld); break;
+        case 64: region = y_region_slots<64>(ld); break;
+        default: return 0;
+    }
+    return (2 * (size_t)(Ny - 1) + (YNT / 64) * (size_t)region) * sizeof(double2);
 }
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
