@@ -23,9 +23,12 @@ from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 
 __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State", "zero_state", "random_state",
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
-           "require_device", "RotatingForm"]
+           "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm"]
 
+# nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
+ConvectiveForm = "convective"
+DivergenceForm = "divergence"
 
 
 def _torch():
@@ -161,7 +164,9 @@ class CNRK2:
 
 
 class NonLinearTerm:
-    """Rotational-form nonlinear operator N = FFT(V x curl V) (nonlinear.jl:173-193)."""
+    """NonLinearTerm of the problem's form (nonlinear.jl:113-193): rotating
+    N = FFT(V x curl V), convective N = -FFT((V . grad) V), divergence
+    N = -div FFT(V V), with V = U + Ub e_x."""
 
     def __init__(self, scheme):
         self.scheme = scheme

@@ -74,11 +74,12 @@ struct cf_problem {
     FFTPlan pz{}, py{}, px{};
     double2* d_twx = nullptr;
     double2* six = nullptr;   // 6 S
-    double2* zbuf = nullptr;  // 6 Sz
+    double2* zbuf = nullptr;  // 6 Sz (9 Sz for the convective form)
     double2* Q = nullptr;     // 3 S
     double2* R = nullptr;     // 3 S
     double* d_grad = nullptr;
     int KT = 1, zld = 0, LPB = 1, xld = 0, yld = 0;
+    int xwaves = 4;  // even Nxp: wavefronts per block of the nonlinear x pass
 };
 
 struct DeviceGuard {
@@ -183,7 +184,8 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         return fail(CF_ERR_ARGUMENT, "unknown mean-flow constraint");
     if (!(std::isfinite(cfg->constraint_values[0]) && std::isfinite(cfg->constraint_values[1])))
         return fail(CF_ERR_ARGUMENT, "mean-flow constraint values must be finite");
-    if (cfg->form != CF_FORM_ROTATING) return fail(CF_ERR_UNSUPPORTED, "only the rotational form runs on the device");
+    if (cfg->form != CF_FORM_ROTATING && cfg->form != CF_FORM_CONVECTIVE && cfg->form != CF_FORM_DIVERGENCE)
+        return fail(CF_ERR_ARGUMENT, "unknown nonlinear form");
     if (y_group_lanes(cfg->Ny) == 0) return fail(CF_ERR_UNSUPPORTED, "Ny above 255 is not supported");
 
     cf_problem* h = new cf_problem();
@@ -258,14 +260,24 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
     h->KT = std::min(h->KT, gd.Nxh);
     h->xld = M + 1;
     {
-        size_t per_line = (gd.Nxp % 2 == 0) ? 12 * (size_t)h->xld * sizeof(double2)
-                                            : 2 * (size_t)h->xld * sizeof(double2) + 6 * (size_t)gd.Nxp * sizeof(double);
+        // odd Nxp: two FFT lines plus the form's physical lines per (zp, y) line
+        const int xmode = h->form == CF_FORM_CONVECTIVE ? X_CONVECTIVE : h->form == CF_FORM_DIVERGENCE ? X_DIVERGENCE : X_ROTATIONAL;
+        size_t per_line = 2 * (size_t)h->xld * sizeof(double2) + (size_t)x_phys_lines(xmode) * gd.Nxp * sizeof(double);
         h->LPB = (int)std::max<size_t>(1, std::min<size_t>(8, LDS_MAX / per_line));
     }
     if (y_lds_bytes(gd.Ny, h->yld) > 160 * 1024) return bail(fail(CF_ERR_UNSUPPORTED, "wall-normal tile exceeds LDS"));
+    if (gd.Nxp % 2 == 0) {
+        XArgs xa;
+        std::memset(&xa, 0, sizeof(xa));
+        xa.gd = gd;
+        xa.ld = h->xld;
+        xa.mode = h->form == CF_FORM_CONVECTIVE ? X_CONVECTIVE : h->form == CF_FORM_DIVERGENCE ? X_DIVERGENCE : X_ROTATIONAL;
+        h->xwaves = x_max_waves(xa);
+        if (h->xwaves < 1) return bail(fail(CF_ERR_UNSUPPORTED, "streamwise line exceeds LDS"));
+    }
     // workspaces
     if ((rc = dalloc(h, &h->six, 6 * (size_t)gd.S))) return bail(rc);
-    if ((rc = dalloc(h, &h->zbuf, 6 * (size_t)gd.Sz))) return bail(rc);
+    if ((rc = dalloc(h, &h->zbuf, (h->form == CF_FORM_CONVECTIVE ? 9 : 6) * (size_t)gd.Sz))) return bail(rc);
     if ((rc = dalloc(h, &h->Q, 3 * (size_t)gd.S))) return bail(rc);
     if ((rc = dalloc(h, &h->R, 3 * (size_t)gd.S))) return bail(rc);
     if ((rc = dalloc(h, &h->d_grad, 2))) return bail(rc);
@@ -312,12 +324,15 @@ static YArgs base_yargs(cf_problem* h) {
     a.curv_mean = h->curv_mean;
     a.facP = h->d_facP;
     a.facPN = h->d_facP;
+    a.form = h->form;
+    a.nl_sign = 1.0;
     return a;
 }
 
 static double2* fld(double2* base, long long n, int i) { return base + (size_t)n * i; }
 
-static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* const* out, hipStream_t st) {
+static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* const* out, hipStream_t st,
+                 unsigned dz_mask = 0) {
     ZArgs z;
     std::memset(&z, 0, sizeof(z));
     z.gd = h->gd;
@@ -325,6 +340,7 @@ static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* co
     z.KT = h->KT;
     z.ld = h->zld;
     z.nfields = nf;
+    z.dz_mask = dz_mask;
     for (int i = 0; i < nf; ++i) {
         z.in[i] = in[i];
         z.out[i] = out[i];
@@ -341,30 +357,57 @@ static XArgs base_xargs(cf_problem* h) {
     x.twx = h->d_twx;
     x.LPB = h->LPB;
     x.ld = h->xld;
+    x.waves = h->xwaves;
     return x;
 }
 
-// y-physical (V, Omega) in six -> y-physical nonlinear term in six[0..2]
+// y-physical fields of the form in six (V, Omega | V, dV/dy | V) -> y-physical
+// nonlinear term in six[0..2] (rotating, convective: V x Omega, (V.grad)V) or
+// the six flux products V_i V_j in six[0..5] (divergence)
 struct Timer;
 static int tmark(Timer* tm, int c, hipStream_t st);
 static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st, Timer* tm = nullptr) {
     const GridDims& gd = h->gd;
     double2* six[6];
-    double2* zb[6];
-    for (int i = 0; i < 6; ++i) {
-        six[i] = fld(h->six, gd.S, i);
-        zb[i] = fld(h->zbuf, gd.Sz, i);
-    }
-    if (int rc = zpass(h, +1, 6, six, zb, st)) return rc;
-    if (int rc = tmark(tm, 1, st)) return rc;
+    double2* zb[9];
+    for (int i = 0; i < 6; ++i) six[i] = fld(h->six, gd.S, i);
+    const int nzb = h->form == CF_FORM_CONVECTIVE ? 9 : 6;
+    for (int i = 0; i < nzb; ++i) zb[i] = fld(h->zbuf, gd.Sz, i);
     XArgs x = base_xargs(h);
-    x.mode = X_ROTATIONAL;
-    for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
-    for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a block reads all six lines first
+    if (h->form == CF_FORM_ROTATING) {
+        if (int rc = zpass(h, +1, 6, six, zb, st)) return rc;
+        x.mode = X_ROTATIONAL;
+        for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
+        for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a wavefront reads its line first
+    } else if (h->form == CF_FORM_CONVECTIVE) {
+        // V, dV/dy and (from V again) dV/dz
+        double2* zin[9] = {six[0], six[1], six[2], six[3], six[4], six[5], six[0], six[1], six[2]};
+        if (int rc = zpass(h, +1, 9, zin, zb, st, 0x1C0u)) return rc;
+        x.mode = X_CONVECTIVE;
+        for (int i = 0; i < 9; ++i) x.in[i] = zb[i];
+        for (int i = 0; i < 3; ++i) x.out[i] = zb[i];
+    } else {
+        if (int rc = zpass(h, +1, 3, six, zb, st)) return rc;
+        x.mode = X_DIVERGENCE;
+        for (int i = 0; i < 3; ++i) x.in[i] = zb[i];
+        for (int i = 0; i < 6; ++i) x.out[i] = zb[i];
+    }
+    if (int rc = tmark(tm, 1, st)) return rc;
     HIPTRY(launch_xpass(x, st));
     if (int rc = tmark(tm, 2, st)) return rc;
-    if (int rc = zpass(h, -1, 3, zb, six, st)) return rc;
+    const int nout = h->form == CF_FORM_DIVERGENCE ? 6 : 3;
+    if (int rc = zpass(h, -1, nout, zb, six, st)) return rc;
     if (int rc = tmark(tm, 3, st)) return rc;
+    return 0;
+}
+
+// the transformed nonlinear term's sign (nonlinear.jl:139, 166, 192) and, for
+// the divergence form, its assembly in coefficient space into Ncoef[0..2]
+static int enqueue_nonlinear_coef(cf_problem* h, double2* const Ncoef[3], hipStream_t st) {
+    YArgs a = base_yargs(h);
+    for (int i = 0; i < 6; ++i) a.Nphys[i] = fld(h->six, h->gd.S, i);
+    for (int i = 0; i < 3; ++i) a.Rout[i] = Ncoef[i];
+    HIPTRY(launch_ykernel(Y_DIVERGENCE, a, st));
     return 0;
 }
 
@@ -435,7 +478,11 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
         if (int rc = tmark(tm, 0, st)) return rc;
         for (int j = 0; j < 3; ++j) {
             if (int rc = enqueue_nonlinear_phys(h, st, tm)) return rc;
+            double2* Ncoef[3] = {fld(h->R, gd.S, 0), fld(h->R, gd.S, 1), fld(h->R, gd.S, 2)};
+            if (h->form == CF_FORM_DIVERGENCE)
+                if (int rc = enqueue_nonlinear_coef(h, Ncoef, st)) return rc;
             YArgs a = base_yargs(h);
+            a.nl_sign = (h->form == CF_FORM_CONVECTIVE) ? -1.0 : 1.0;
             a.stage = j;
             a.store_Q = (j < 2);
             a.lambda = 1.0 / (CNRK2_C[j] * h->dt);
@@ -447,6 +494,7 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
             a.bank = h->stage[j].view();
             for (int i = 0; i < 3; ++i) {
                 a.Nphys[i] = fld(h->six, gd.S, i);
+                a.Ncoef[i] = (h->form == CF_FORM_DIVERGENCE) ? Ncoef[i] : nullptr;
                 a.U[i] = U[i];
                 a.Q[i] = fld(h->Q, gd.S, i);
             }
@@ -561,7 +609,9 @@ extern "C" int cf_inverse(cf_problem* h, const double* U, double* u, void* strea
 static int enqueue_nonlinear_spec(cf_problem* h, double2* const U[3], double2* const N[3], hipStream_t st) {
     if (int rc = enqueue_prologue(h, U, st)) return rc;
     if (int rc = enqueue_nonlinear_phys(h, st)) return rc;
+    if (h->form == CF_FORM_DIVERGENCE) return enqueue_nonlinear_coef(h, N, st);
     YArgs a = base_yargs(h);
+    a.nl_sign = (h->form == CF_FORM_CONVECTIVE) ? -1.0 : 1.0;
     a.nfields = 3;
     for (int i = 0; i < 3; ++i) {
         a.Nphys[i] = fld(h->six, h->gd.S, i);

@@ -1,7 +1,7 @@
 // Device-side building blocks for the CNRK2 step on gfx950 (CDNA4, wave64).
 //
 //  * complex FP64 arithmetic on double2
-//  * lane-group cross-lane primitives (ds_bpermute) and scans used by the
+//  * lane-group cross-lane primitives (DPP / permlane) and scans used by the
 //    wall-normal (Chebyshev) recurrences: one mode is held by a group of L
 //    lanes, lane t owning coefficients n = 4t .. 4t+3
 //  * a batched Stockham autosort FFT staged in LDS (mixed radix 2/3/4/5/8 +
@@ -28,45 +28,219 @@ __device__ __forceinline__ double2 cfma(double s, double2 a, double2 b) { return
 __device__ __forceinline__ double2 czero() { return cmk(0.0, 0.0); }
 
 // ---------------------------------------------------------------------------
-// cross-lane moves (ds_bpermute): value from absolute lane `src`
+// cross-lane moves.  A wave64 is four DPP rows of 16 lanes.  Within a row:
+// DPP row shifts, quad permutes and mirrors; across rows: row_bcast:15/31,
+// row_newbcast:0 (64-bit DPP), v_permlane16_swap / v_permlane32_swap and
+// readlane.  All are VALU operations (no LDS round trip).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double bperm(double v, int src) {
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_ds_bpermute(src << 2, lo);
-    hi = __builtin_amdgcn_ds_bpermute(src << 2, hi);
-    return __hiloint2double(hi, lo);
+namespace dpp {
+enum : int {
+    QUAD_XOR1 = 0xB1,        // quad_perm [1,0,3,2]
+    QUAD_XOR2 = 0x4E,        // quad_perm [2,3,0,1]
+    ROW_SHL = 0x100,         // + d: lane r <- r+d within the row
+    ROW_SHR = 0x110,         // + d: lane r <- r-d within the row
+    ROW_MIRROR = 0x140,      // lane r <- 15-r
+    ROW_HALF_MIRROR = 0x141, // lane r <- 7-r within each half row
+    ROW_BCAST15 = 0x142,     // row k+1 <- lane 15 of row k (rows selected by the row mask)
+    ROW_BCAST31 = 0x143,     // rows 2,3 <- lane 31 (rows selected by the row mask)
+    ROW_NEWBCAST0 = 0x150,   // every lane <- first lane of its row (64-bit)
+};
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double mov(double x) {
+    return __longlong_as_double(__builtin_amdgcn_update_dpp(0ll, __double_as_longlong(x), CTRL, ROWMASK, 0xF, false));
 }
-__device__ __forceinline__ double2 bperm(double2 v, int src) { return cmk(bperm(v.x, src), bperm(v.y, src)); }
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double2 mov(double2 x) {
+    return make_double2(mov<CTRL, ROWMASK>(x.x), mov<CTRL, ROWMASK>(x.y));
+}
+// value held by the odd row of this lane's row pair (rows 0/1, 2/3), same column
+__device__ __forceinline__ double odd_row(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double((int)h[1], (int)l[1]);
+}
+__device__ __forceinline__ void row_pair(double x, double& even, double& odd) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    even = __hiloint2double((int)h[0], (int)l[0]);
+    odd = __hiloint2double((int)h[1], (int)l[1]);
+}
+__device__ __forceinline__ void wave_halves(double x, double& lower, double& upper) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    lower = __hiloint2double((int)h[0], (int)l[0]);
+    upper = __hiloint2double((int)h[1], (int)l[1]);
+}
+__device__ __forceinline__ double readlane(double x, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                            __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+__device__ __forceinline__ double2 readlane(double2 x, int lane) { return make_double2(readlane(x.x, lane), readlane(x.y, lane)); }
+__device__ __forceinline__ double2 odd_row(double2 x) { return make_double2(odd_row(x.x), odd_row(x.y)); }
+}  // namespace dpp
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-// Group of L lanes (power of two, <= 64); t = index within group.
+// Group of L lanes (power of two, 4 <= L <= 64) owning one mode; t = index
+// within the group, r = index within the DPP row.  Groups of L <= 16 lie in
+// one row; L = 32 is a row pair, L = 64 the whole wave.  Every lane of a group
+// must call a group operation together (values of other groups are never used).
 template <int L>
 struct Group {
-    int lane, t, base;
+    int lane, t, r;
     __device__ __forceinline__ Group() {
         lane = lane_id();
         t = lane & (L - 1);
-        base = lane - t;
+        r = lane & 15;
     }
-    // value from group lane t+d (or 0/identity if out of range: caller masks)
-    __device__ __forceinline__ int down(int d) const { return (t + d < L) ? lane + d : lane; }
-    __device__ __forceinline__ int up(int d) const { return (t >= d) ? lane - d : lane; }
-    __device__ __forceinline__ int at(int tt) const { return base + tt; }
 
-    __device__ __forceinline__ double sum(double v) const {
-#pragma unroll
-        for (int d = 1; d < L; d <<= 1) v += bperm(v, lane ^ d);
-        return v;
+    // ---- neighbours: group lane t-1 (undefined at t = 0), t+1 (undefined at t = L-1)
+    template <typename V>
+    __device__ __forceinline__ V up1(V x) const {
+        V in_row = dpp::mov<dpp::ROW_SHR + 1>(x);
+        if constexpr (L <= 16) return in_row;
+        else {
+            V across = dpp::mov<dpp::ROW_BCAST15, (L == 32 ? 0xA : 0xE)>(x);
+            return r == 0 ? across : in_row;
+        }
     }
-    __device__ __forceinline__ double2 sum(double2 v) const {
-#pragma unroll
-        for (int d = 1; d < L; d <<= 1) {
-            int s = lane ^ d;
-            v.x += bperm(v.x, s);
-            v.y += bperm(v.y, s);
+    template <typename V>
+    __device__ __forceinline__ V down1(V x) const {
+        V in_row = dpp::mov<dpp::ROW_SHL + 1>(x);
+        if constexpr (L <= 16) return in_row;
+        else {
+            V across = dpp::mov<dpp::ROW_NEWBCAST0>(dpp::odd_row(x));  // first lane of the next row (rows 0, 2)
+            if constexpr (L == 64) {
+                V mid = dpp::readlane(x, 32);  // lane 31 <- lane 32
+                if (lane == 31) across = mid;
+            }
+            return r == 15 ? across : in_row;
+        }
+    }
+
+    // ---- all-reduce over the group
+    __device__ __forceinline__ double sum(double v) const {
+        v += dpp::mov<dpp::QUAD_XOR1>(v);
+        v += dpp::mov<dpp::QUAD_XOR2>(v);
+        if constexpr (L >= 8) v += dpp::mov<dpp::ROW_HALF_MIRROR>(v);
+        if constexpr (L >= 16) v += dpp::mov<dpp::ROW_MIRROR>(v);
+        if constexpr (L >= 32) {
+            double e, o;
+            dpp::row_pair(v, e, o);
+            v = e + o;
+        }
+        if constexpr (L >= 64) {
+            double lo, hi;
+            dpp::wave_halves(v, lo, hi);
+            v = lo + hi;
         }
         return v;
+    }
+    __device__ __forceinline__ double2 sum(double2 v) const { return make_double2(sum(v.x), sum(v.y)); }
+
+    // ---- inclusive suffix sum over group lanes t..L-1
+    template <int D>
+    __device__ __forceinline__ bool suffix_ok() const {
+        return (L <= 16) ? (t + D < L) : (r + D < 16);
+    }
+    template <int D>
+    __device__ __forceinline__ bool prefix_ok() const {
+        return ((L <= 16) ? t : r) >= D;
+    }
+    template <int D>
+    __device__ __forceinline__ void suffix_sum_step(double2& v) const {
+        if constexpr (D < L && D < 16) {
+            double2 o = dpp::mov<dpp::ROW_SHL + D>(v);
+            if (suffix_ok<D>()) v = cadd(v, o);
+        }
+    }
+    __device__ __forceinline__ double2 suffix_sum(double2 v) const {
+        suffix_sum_step<1>(v);
+        suffix_sum_step<2>(v);
+        suffix_sum_step<4>(v);
+        suffix_sum_step<8>(v);
+        if constexpr (L >= 32) {  // even rows add the next row's total (its first lane)
+            double2 o = dpp::mov<dpp::ROW_NEWBCAST0>(dpp::odd_row(v));
+            if ((lane & 16) == 0) v = cadd(v, o);
+        }
+        if constexpr (L == 64) {  // lower half adds the upper half's total (lane 32)
+            double2 o = dpp::readlane(v, 32);
+            if (lane < 32) v = cadd(v, o);
+        }
+        return v;
+    }
+
+    // ---- affine compositions x -> A + T x.  Suffix: lane t ends with
+    // f_t o f_{t+1} o ... o f_{L-1}; prefix: f_t o f_{t-1} o ... o f_0.
+    // Combining with a later/earlier composite (oT, oA): (T oT, A + T oA).
+    template <int D>
+    __device__ __forceinline__ void suffix_affine_step(double& T, double2& A) const {
+        if constexpr (D < L && D < 16) {
+            double oT = dpp::mov<dpp::ROW_SHL + D>(T);
+            double2 oA = dpp::mov<dpp::ROW_SHL + D>(A);
+            if (suffix_ok<D>()) {
+                A = cfma(T, oA, A);
+                T *= oT;
+            }
+        }
+    }
+    __device__ __forceinline__ void suffix_affine(double& T, double2& A) const {
+        suffix_affine_step<1>(T, A);
+        suffix_affine_step<2>(T, A);
+        suffix_affine_step<4>(T, A);
+        suffix_affine_step<8>(T, A);
+        if constexpr (L >= 32) {
+            double oT = dpp::mov<dpp::ROW_NEWBCAST0>(dpp::odd_row(T));
+            double2 oA = dpp::mov<dpp::ROW_NEWBCAST0>(dpp::odd_row(A));
+            if ((lane & 16) == 0) {
+                A = cfma(T, oA, A);
+                T *= oT;
+            }
+        }
+        if constexpr (L == 64) {
+            double oT = dpp::readlane(T, 32);
+            double2 oA = dpp::readlane(A, 32);
+            if (lane < 32) {
+                A = cfma(T, oA, A);
+                T *= oT;
+            }
+        }
+    }
+    template <int D>
+    __device__ __forceinline__ void prefix_affine_step(double& T, double2& A) const {
+        if constexpr (D < L && D < 16) {
+            double oT = dpp::mov<dpp::ROW_SHR + D>(T);
+            double2 oA = dpp::mov<dpp::ROW_SHR + D>(A);
+            if (prefix_ok<D>()) {
+                A = cfma(T, oA, A);
+                T *= oT;
+            }
+        }
+    }
+    __device__ __forceinline__ void prefix_affine(double& T, double2& A) const {
+        prefix_affine_step<1>(T, A);
+        prefix_affine_step<2>(T, A);
+        prefix_affine_step<4>(T, A);
+        prefix_affine_step<8>(T, A);
+        if constexpr (L >= 32) {  // odd rows take the previous row's total (its lane 15)
+            double oT = dpp::mov<dpp::ROW_BCAST15, 0xA>(T);
+            double2 oA = dpp::mov<dpp::ROW_BCAST15, 0xA>(A);
+            if (lane & 16) {
+                A = cfma(T, oA, A);
+                T *= oT;
+            }
+        }
+        if constexpr (L == 64) {  // upper half takes the lower half's total (lane 31)
+            double oT = dpp::mov<dpp::ROW_BCAST31, 0xC>(T);
+            double2 oA = dpp::mov<dpp::ROW_BCAST31, 0xC>(A);
+            if (lane & 32) {
+                A = cfma(T, oA, A);
+                T *= oT;
+            }
+        }
     }
 };
 
@@ -88,14 +262,7 @@ __device__ __forceinline__ CVec cheb_deriv(const Group<L>& g, const CVec& a) {
     for (int e = 0; e < 4; ++e) c[e] = cscale(a.v[e], 2.0 * (n0 + e));
     double2 E = cadd(c[0], c[2]), O = cadd(c[1], c[3]);
     // inclusive suffix sums over lanes of E and O
-    double2 sE = E, sO = O;
-#pragma unroll
-    for (int d = 1; d < L; d <<= 1) {
-        int src = g.down(d);
-        bool ok = (g.t + d < L);
-        double2 tE = bperm(sE, src), tO = bperm(sO, src);
-        if (ok) { sE = cadd(sE, tE); sO = cadd(sO, tO); }
-    }
+    double2 sE = g.suffix_sum(E), sO = g.suffix_sum(O);
     // exclusive
     sE = csub(sE, E);
     sO = csub(sO, O);
@@ -160,9 +327,8 @@ __device__ __forceinline__ CVec helm_solve(const Group<L>& g, int P, double t1, 
     // neighbouring coefficients of f for the 3-term tau stencil
     double2 fm2[4], fp2[4];
     {
-        int up1 = g.up(1), dn1 = g.down(1);
-        double2 a2 = bperm(f.v[2], up1), a3 = bperm(f.v[3], up1);
-        double2 b0 = bperm(f.v[0], dn1), b1 = bperm(f.v[1], dn1);
+        double2 a2 = g.up1(f.v[2]), a3 = g.up1(f.v[3]);
+        double2 b0 = g.down1(f.v[0]), b1 = g.down1(f.v[1]);
         bool hasprev = g.t > 0, hasnext = g.t + 1 < L;
         fm2[0] = hasprev ? a2 : czero();
         fm2[1] = hasprev ? a3 : czero();
@@ -209,27 +375,14 @@ __device__ __forceinline__ CVec helm_solve(const Group<L>& g, int P, double t1, 
     }
     // inclusive suffix scan of composites (f_t o f_{t+1} o ...)
 #pragma unroll
-    for (int d = 1; d < L; d <<= 1) {
-        int src = g.down(d);
-        bool ok = g.t + d < L;
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            double oT = bperm(cT[p], src);
-            double2 oA = bperm(cA[p], src);
-            if (ok) {
-                cA[p] = cfma(cT[p], oA, cA[p]);
-                cT[p] = cT[p] * oT;
-            }
-        }
-    }
+    for (int p = 0; p < 2; ++p) g.suffix_affine(cT[p], cA[p]);
     // carry = beta at first j of next lane = A-part of inclusive scan at lane t+1
     double2 beta[4];
     {
-        int src = g.down(1);
         bool ok = g.t + 1 < L;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            double2 carry = bperm(cA[p], src);
+            double2 carry = g.down1(cA[p]);
             if (!ok) carry = czero();
             beta[p + 2] = cfma(T[p + 2], carry, A[p + 2]);
             beta[p] = cfma(T[p], beta[p + 2], A[p]);
@@ -245,28 +398,15 @@ __device__ __forceinline__ CVec helm_solve(const Group<L>& g, int P, double t1, 
         fA[p] = cfma(alpha[p + 2], beta[p], beta[p + 2]);
     }
 #pragma unroll
-    for (int d = 1; d < L; d <<= 1) {
-        int src = g.up(d);
-        bool ok = g.t >= d;
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            double oT = bperm(fT[p], src);
-            double2 oA = bperm(fA[p], src);
-            if (ok) {  // (later) o (earlier): (T, A) o (oT, oA) = (T oT, A + T oA)
-                fA[p] = cfma(fT[p], oA, fA[p]);
-                fT[p] = fT[p] * oT;
-            }
-        }
-    }
+    for (int p = 0; p < 2; ++p) g.prefix_affine(fT[p], fA[p]);  // (later) o (earlier)
     double2 dl[4];
     double gm[4];
     {
-        int src = g.up(1);
         bool ok = g.t >= 1;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-            double2 carryA = bperm(fA[p], src);
-            double carryT = bperm(fT[p], src);
+            double2 carryA = g.up1(fA[p]);
+            double carryT = g.up1(fT[p]);
             if (!ok) { carryA = czero(); carryT = 1.0; }
             dl[p] = cfma(alpha[p], carryA, beta[p]);
             gm[p] = alpha[p] * carryT;

@@ -15,7 +15,10 @@ struct Bank {
     double lambda, nu;
 };
 
-enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_INV };
+enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_INV, Y_DIVERGENCE };
+
+// nonlinear forms (src/nonlinear.jl:113-193); values match CF_FORM_* of the C ABI
+enum { FORM_ROTATING = 0, FORM_CONVECTIVE = 1, FORM_DIVERGENCE = 2 };
 
 struct YArgs {
     GridDims gd;
@@ -35,11 +38,14 @@ struct YArgs {
     const double* facP;  // (B, Ny) factors of D^2 - kappa^2 (Dirichlet; kappa^2=1 placeholder at (0,0))
     const double* facPN; // (B, Ny) factors of D^2 - kappa^2 with Neumann rows (pressure init)
     Bank bank;
-    const double2* Nphys[3];  // y-physical nonlinear term
+    int form;                 // FORM_*: which y-physical fields the epilogue emits
+    double nl_sign;           // sign applied to the transformed nonlinear term
+    const double2* Nphys[6];  // y-physical nonlinear term (Y_DIVERGENCE: the six products)
+    const double2* Ncoef[3];  // nonlinear term already in coefficients (skip the DCT) or null
     double2* U[3];
     double2* P;
     double2* Q[3];
-    double2* out6[6];  // y-physical outputs (V, Omega) or inverse transforms
+    double2* out6[6];  // y-physical outputs: (V, Omega), (V, dV/dy) or V by form; or inverse transforms
     const double2* Rin[3];
     double2* Rout[3];
     double* grad_out;  // 2 doubles (device) or null
@@ -50,19 +56,21 @@ struct ZArgs {
     GridDims gd;
     FFTPlan plan;  // length Nzp
     int KT, ld, nfields;
-    const double2* in[6];
-    double2* out[6];
+    unsigned dz_mask;  // bit f: inverse pass multiplies field f by i*beta*l (z derivative)
+    const double2* in[9];
+    double2* out[9];
 };
 
-enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD };
+enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD, X_CONVECTIVE, X_DIVERGENCE };
 
 struct XArgs {
     GridDims gd;
     FFTPlan plan;         // length Nxp/2 (even Nxp) or Nxp (odd)
     const double2* twx;   // exp(-2 pi i t / Nxp), t < Nxp
     int LPB, ld, mode, nfields;
-    const double2* in[6];  // (k, zp, y) complex, Nxh per line
-    double2* out[3];
+    int waves;             // even Nxp: wavefronts (lines) per block, 1..4
+    const double2* in[9];  // (k, zp, y) complex, Nxh per line
+    double2* out[6];
     const double* rin[3];  // (x, zp, y) real
     double* rout[6];
 };
@@ -74,5 +82,7 @@ hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st);
 size_t z_lds_bytes(const ZArgs& a);
 hipError_t launch_xpass(const XArgs& a, hipStream_t st);
 size_t x_lds_bytes(const XArgs& a);
+int x_phys_lines(int mode);  // physical lines per (zp, y) line of the odd-Nxp x pass
+int x_max_waves(const XArgs& a);  // even Nxp: wavefronts per block that fit the LDS (0: none)
 
 }  // namespace cf

@@ -19,7 +19,18 @@ namespace cf {
 constexpr int ZNT = 256;
 constexpr int XNT = 256;
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): block b runs on XCD b % 8 as that XCD's (b / 8)-th block.  The
+// returned logical index gives XCD x a contiguous range of logical blocks,
+// so neighbouring tiles that share cache lines share an L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned nblocks) {
+    unsigned q = nblocks / NXCD, r = nblocks % NXCD, x = b % NXCD, slot = b / NXCD;
+    return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 // z inverse (dir=+1): (k,l,y) -> (k,zp,y); z forward (dir=-1): (k,zp,y) -> (k,l,y)
+// 1-D grid of ntiles * Ny * nfields blocks, k tile fastest in logical order.
 template <int DIR>
 __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
     extern __shared__ double2 smem[];
@@ -27,14 +38,18 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
     const int KT = a.KT, ld = a.ld;
     double2* bufA = smem;
     double2* bufB = smem + KT * ld;
-    const int y = blockIdx.y, f = blockIdx.z;
-    const int k0 = blockIdx.x * KT;
+    const unsigned ntiles = (unsigned)((gd.Nxh + KT - 1) / KT);
+    const unsigned lb = xcd_logical(blockIdx.x, gridDim.x);
+    const unsigned tile = lb % ntiles, plane = lb / ntiles;
+    const int y = (int)(plane % (unsigned)gd.Ny), f = (int)(plane / (unsigned)gd.Ny);
+    const int k0 = (int)tile * KT;
     const int tid = threadIdx.x;
     const int Nz = gd.Nz, Nzp = gd.Nzp, Nxh = gd.Nxh;
     const int npos = (Nz >> 1) + 1, nneg = Nz - npos;
     const bool xeven = (gd.Nx % 2 == 0), zeven = (Nz % 2 == 0);
     if (DIR > 0) {
         const double2* in = a.in[f] + (size_t)Nxh * Nz * y;
+        const bool dz = (a.dz_mask >> f) & 1u;  // emit d/dz: multiply by i beta l
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
             int z = idx / KT, kk = idx - z * KT;
             int k = k0 + kk;
@@ -43,7 +58,10 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
                 int l = -1;
                 if (z < npos) l = (zeven && z == Nz / 2) ? -1 : z;
                 else if (z >= Nzp - nneg) l = npos + (z - (Nzp - nneg));
-                if (l >= 0) v = in[k + (size_t)Nxh * l];
+                if (l >= 0) {
+                    v = in[k + (size_t)Nxh * l];
+                    if (dz) v = cmuli(cscale(v, gd.beta * (l <= Nz / 2 ? l : l - Nz)));
+                }
             }
             bufA[kk * ld + z] = v;
         }
@@ -99,7 +117,7 @@ struct XCtx {
     }
     // inverse real transform of LPB lines of `in` (Nxh complex per line)
     // into phys[line * Nxp + x]
-    __device__ __forceinline__ void c2r(const double2* __restrict__ in, double* phys) {
+    __device__ __forceinline__ void c2r(const double2* __restrict__ in, double* phys, bool dx = false) {
         const bool xeven = (a.gd.Nx % 2 == 0);
         // stage X_k (k = 0..M) in bufB (zero padding, Nyquist plane and Im(DC) removed)
         const int nX = even ? M + 1 : Nxp / 2 + 1;
@@ -110,6 +128,7 @@ struct XCtx {
             if (gl < nlines && k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = in[(size_t)gl * Nxh + k];
             if (k == 0) v.y = 0.0;
             if (even && k == M) v.y = 0.0;
+            if (dx) v = cmuli(cscale(v, a.gd.alpha * k));
             bufB[ln * ld + k] = v;
         }
         __syncthreads();
@@ -171,70 +190,148 @@ struct XCtx {
     }
 };
 
-// Rotational x pass for even Nxp: the six fields of LPB lines are one batch
-// of 6*LPB length-M FFTs; the product is formed from the transformed batch
-// in LDS and packed directly as the input of the 3*LPB forward FFTs.
-__device__ __forceinline__ void xpass_rotational_even(const XArgs& a, double2* smem) {
-    const int Nxp = a.gd.Nxp, Nxh = a.gd.Nxh, M = Nxp / 2, LPB = a.LPB, ld = a.ld, tid = threadIdx.x;
-    const long long gl0 = (long long)blockIdx.x * LPB, nlines = (long long)a.gd.Nzp * a.gd.Ny;
+// Nonlinear x pass for even Nxp.  Each wavefront owns one (zp, y) line and a
+// private LDS region: the fields the form needs are one batch of length-M
+// complex FFTs (half-length real transforms), the pointwise products are
+// formed in LDS and packed directly as the input of the forward FFTs.
+//   rotating   in V, Omega (6)             out V x Omega (3)          (nonlinear.jl:187-189)
+//   convective in V, dV/dy, dV/dz (9);     out (V . grad) V (3)       (nonlinear.jl:124-133)
+//              dV/dx is formed here as i alpha k V
+//   divergence in V (3)                    out V_i V_j, i <= j (6)    (nonlinear.jl:157-160)
+// Only wave-level ordering is needed after the block loads the twiddles.
+// A block has a.waves wavefronts (64 * a.waves threads).
+__host__ __device__ constexpr int x_region_slots(int mode, int ld) { return (mode == X_CONVECTIVE ? 24 : 12) * ld; }
+__host__ __device__ constexpr int x_fft_lines_in(int mode) { return mode == X_CONVECTIVE ? 12 : (mode == X_DIVERGENCE ? 3 : 6); }
+__host__ __device__ constexpr int x_fields_in(int mode) { return mode == X_CONVECTIVE ? 9 : (mode == X_DIVERGENCE ? 3 : 6); }
+__host__ __device__ constexpr int x_fields_out(int mode) { return mode == X_DIVERGENCE ? 6 : 3; }
+
+template <int MODE>
+__device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
+    constexpr int NIN = x_fields_in(MODE), NFFT = x_fft_lines_in(MODE), NOUT = x_fields_out(MODE);
+    constexpr int NB = NFFT > NOUT ? NFFT : NOUT;  // lines per buffer (x_region_slots = 2 NB ld)
+    const int Nxp = a.gd.Nxp, Nxh = a.gd.Nxh, M = Nxp / 2, ld = a.ld;
+    const long long nlines = (long long)a.gd.Nzp * a.gd.Ny;
     const bool xeven = (a.gd.Nx % 2 == 0);
-    double2* bufA = smem;
-    double2* bufB = smem + 6 * LPB * ld;
+    // twiddles: FFT table (M) and pack/unpack table exp(-2 pi i k / Nxp), k <= M
+    double2* tw = smem;
+    double2* twx = smem + M;
+    for (int i = threadIdx.x; i < M; i += blockDim.x) tw[i] = a.plan.tw[i];
+    for (int i = threadIdx.x; i <= M; i += blockDim.x) twx[i] = a.twx[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long gl = (long long)blockIdx.x * a.waves + wave;
+    if (gl >= nlines) return;  // whole wavefront leaves; no block barrier follows
+    double2* bufA = smem + 2 * M + 1 + (size_t)wave * x_region_slots(MODE, ld);
+    double2* bufB = bufA + NB * ld;
     const int nX = M + 1;
     // stage X_k, k = 0..M (padding, Nyquist plane and Im of DC/Nyquist removed)
-    for (int idx = tid; idx < 6 * LPB * nX; idx += XNT) {
-        int q = idx / nX, k = idx - q * nX;
-        int f = q / LPB, ln = q - f * LPB;
-        long long gl = gl0 + ln;
+    for (int idx = lane; idx < NIN * nX; idx += 64) {
+        int f = idx / nX, k = idx - f * nX;
         double2 v = czero();
-        if (gl < nlines && k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = a.in[f][(size_t)gl * Nxh + k];
+        if (k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = a.in[f][(size_t)gl * Nxh + k];
         if (k == 0 || k == M) v.y = 0.0;
-        bufB[q * ld + k] = v;
+        bufB[f * ld + k] = v;
     }
-    __syncthreads();
-    for (int idx = tid; idx < 6 * LPB * M; idx += XNT) {
+    wave_sync();
+    // pack for the half-length inverse transform; FFT line q reads staged
+    // field src(q), multiplied by i alpha k for the convective x derivatives
+    for (int idx = lane; idx < NFFT * M; idx += 64) {
         int q = idx / M, k = idx - q * M;
-        double2 Xk = bufB[q * ld + k];
-        double2 Xc = cconj(bufB[q * ld + (M - k)]);
-        double2 w = cconj(a.twx[k]);  // exp(+2 pi i k / Nxp)
+        int f = q;
+        bool dx = false;
+        if constexpr (MODE == X_CONVECTIVE) {
+            dx = (q >= 3 && q < 6);
+            f = q < 3 ? q : (q < 6 ? q - 3 : q - 3);
+        }
+        double2 Xk = bufB[f * ld + k];
+        double2 Xm = bufB[f * ld + (M - k)];
+        if (dx) {
+            Xk = cmuli(cscale(Xk, a.gd.alpha * k));
+            Xm = cmuli(cscale(Xm, a.gd.alpha * (M - k)));
+        }
+        double2 Xc = cconj(Xm);
+        double2 w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
         bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
     }
-    __syncthreads();
-    double2* res = lds_fft(bufA, bufB, a.plan, 6 * LPB, ld, +1, tid, XNT);
+    wave_sync();
+    double2* res = lds_fft<true>(bufA, bufB, a.plan, tw, NFFT, ld, +1, lane, 64);
     double2* nxt = (res == bufA) ? bufB : bufA;
-    // u x w at x = 2j (real parts) and x = 2j+1 (imaginary parts)
-    for (int idx = tid; idx < LPB * M; idx += XNT) {
-        int ln = idx / M, j = idx - ln * M;
-        double2 u0 = res[(0 * LPB + ln) * ld + j], u1 = res[(1 * LPB + ln) * ld + j], u2 = res[(2 * LPB + ln) * ld + j];
-        double2 w0 = res[(3 * LPB + ln) * ld + j], w1 = res[(4 * LPB + ln) * ld + j], w2 = res[(5 * LPB + ln) * ld + j];
-        nxt[(0 * LPB + ln) * ld + j] = cmk(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
-        nxt[(1 * LPB + ln) * ld + j] = cmk(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
-        nxt[(2 * LPB + ln) * ld + j] = cmk(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+    // pointwise products at x = 2j (real parts) and x = 2j+1 (imaginary parts)
+    for (int j = lane; j < M; j += 64) {
+        if constexpr (MODE == X_ROTATIONAL) {
+            double2 u0 = res[0 * ld + j], u1 = res[1 * ld + j], u2 = res[2 * ld + j];
+            double2 w0 = res[3 * ld + j], w1 = res[4 * ld + j], w2 = res[5 * ld + j];
+            nxt[0 * ld + j] = cmk(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+            nxt[1 * ld + j] = cmk(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+            nxt[2 * ld + j] = cmk(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+        } else if constexpr (MODE == X_CONVECTIVE) {
+            double2 u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                double2 gx = res[(3 + i) * ld + j], gy = res[(6 + i) * ld + j], gz = res[(9 + i) * ld + j];
+                nxt[i * ld + j] = cmk(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
+            }
+        } else {
+            double2 u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
+            nxt[0 * ld + j] = cmk(u.x * u.x, u.y * u.y);
+            nxt[1 * ld + j] = cmk(u.x * v.x, u.y * v.y);
+            nxt[2 * ld + j] = cmk(u.x * w.x, u.y * w.y);
+            nxt[3 * ld + j] = cmk(v.x * v.x, v.y * v.y);
+            nxt[4 * ld + j] = cmk(v.x * w.x, v.y * w.y);
+            nxt[5 * ld + j] = cmk(w.x * w.x, w.y * w.y);
+        }
     }
-    __syncthreads();
-    double2* r2 = lds_fft(nxt, res, a.plan, 3 * LPB, ld, -1, tid, XNT);
-    for (int idx = tid; idx < 3 * LPB * Nxh; idx += XNT) {
-        int q = idx / Nxh, k = idx - q * Nxh;
-        int c = q / LPB, ln = q - c * LPB;
-        long long gl = gl0 + ln;
-        if (gl >= nlines) continue;
-        double2 Zk = r2[q * ld + k];
-        double2 Zc = cconj(r2[q * ld + ((M - k) % M)]);
-        double2 w = a.twx[k];  // exp(-2 pi i k / Nxp)
+    wave_sync();
+    double2* r2 = lds_fft<true>(nxt, res, a.plan, tw, NOUT, ld, -1, lane, 64);
+    for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
+        int c = idx / Nxh, k = idx - c * Nxh;
+        double2 Zk = r2[c * ld + k];
+        double2 Zc = cconj(r2[c * ld + ((M - k) % M)]);
+        double2 w = twx[k];  // exp(-2 pi i k / Nxp)
         a.out[c][(size_t)gl * Nxh + k] = csub(cscale(cadd(Zk, Zc), 0.5), cscale(cmuli(cmul(w, csub(Zk, Zc))), 0.5));
     }
 }
 
 __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
     extern __shared__ double2 smem[];
-    if (a.mode == X_ROTATIONAL && a.gd.Nxp % 2 == 0) {
-        xpass_rotational_even(a, smem);
-        return;
+    if (a.gd.Nxp % 2 == 0) {
+        if (a.mode == X_ROTATIONAL) { xpass_wave<X_ROTATIONAL>(a, smem); return; }
+        if (a.mode == X_CONVECTIVE) { xpass_wave<X_CONVECTIVE>(a, smem); return; }
+        if (a.mode == X_DIVERGENCE) { xpass_wave<X_DIVERGENCE>(a, smem); return; }
     }
     XCtx c(a, smem);
     double* phys = reinterpret_cast<double*>(smem + 2 * a.LPB * a.ld);
     const int Nxp = c.Nxp, LPB = c.LPB;
-    if (a.mode == X_ROTATIONAL) {
+    if (a.mode == X_CONVECTIVE) {  // odd Nxp: phys lines V, dV/dx, dV/dy, dV/dz
+        for (int f = 0; f < 3; ++f) c.c2r(a.in[f], phys + (size_t)f * LPB * Nxp);
+        for (int f = 0; f < 3; ++f) c.c2r(a.in[f], phys + (size_t)(3 + f) * LPB * Nxp, true);
+        for (int f = 3; f < 9; ++f) c.c2r(a.in[f], phys + (size_t)(3 + f) * LPB * Nxp);
+        for (int idx = c.tid; idx < LPB * Nxp; idx += XNT) {
+            double u = phys[idx], v = phys[1 * LPB * Nxp + idx], w = phys[2 * LPB * Nxp + idx];
+            double n[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                n[i] = u * phys[(3 + i) * LPB * Nxp + idx] + v * phys[(6 + i) * LPB * Nxp + idx] +
+                       w * phys[(9 + i) * LPB * Nxp + idx];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) phys[i * LPB * Nxp + idx] = n[i];
+        }
+        __syncthreads();
+        for (int f = 0; f < 3; ++f) c.r2c(phys + (size_t)f * LPB * Nxp, a.out[f]);
+    } else if (a.mode == X_DIVERGENCE) {  // odd Nxp: products into phys lines 3..8
+        for (int f = 0; f < 3; ++f) c.c2r(a.in[f], phys + (size_t)f * LPB * Nxp);
+        for (int idx = c.tid; idx < LPB * Nxp; idx += XNT) {
+            double u = phys[idx], v = phys[1 * LPB * Nxp + idx], w = phys[2 * LPB * Nxp + idx];
+            phys[3 * LPB * Nxp + idx] = u * u;
+            phys[4 * LPB * Nxp + idx] = u * v;
+            phys[5 * LPB * Nxp + idx] = u * w;
+            phys[6 * LPB * Nxp + idx] = v * v;
+            phys[7 * LPB * Nxp + idx] = v * w;
+            phys[8 * LPB * Nxp + idx] = w * w;
+        }
+        __syncthreads();
+        for (int f = 0; f < 6; ++f) c.r2c(phys + (size_t)(3 + f) * LPB * Nxp, a.out[f]);
+    } else if (a.mode == X_ROTATIONAL) {
         for (int f = 0; f < 6; ++f) c.c2r(a.in[f], phys + (size_t)f * LPB * Nxp);
         for (int idx = c.tid; idx < LPB * Nxp; idx += XNT) {
             double u0 = phys[0 * LPB * Nxp + idx], u1 = phys[1 * LPB * Nxp + idx], u2 = phys[2 * LPB * Nxp + idx];
@@ -271,23 +368,46 @@ __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
 size_t z_lds_bytes(const ZArgs& a) { return 2 * (size_t)a.KT * a.ld * sizeof(double2); }
 
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st) {
-    dim3 grid((unsigned)((a.gd.Nxh + a.KT - 1) / a.KT), (unsigned)a.gd.Ny, (unsigned)a.nfields);
+    dim3 grid((unsigned)((a.gd.Nxh + a.KT - 1) / a.KT) * (unsigned)a.gd.Ny * (unsigned)a.nfields);
     size_t lds = z_lds_bytes(a);
     if (dir > 0) hipLaunchKernelGGL(zpass_kernel<+1>, grid, dim3(ZNT), lds, st, a);
     else hipLaunchKernelGGL(zpass_kernel<-1>, grid, dim3(ZNT), lds, st, a);
     return hipGetLastError();
 }
 
+static bool x_wave_path(const XArgs& a) {
+    return a.gd.Nxp % 2 == 0 && (a.mode == X_ROTATIONAL || a.mode == X_CONVECTIVE || a.mode == X_DIVERGENCE);
+}
+
+int x_phys_lines(int mode) {
+    return mode == X_ROTATIONAL ? 6 : mode == X_CONVECTIVE ? 12 : mode == X_DIVERGENCE ? 9 : 1;
+}
+
+static size_t x_wave_lds(const XArgs& a, int waves) {
+    return ((size_t)a.gd.Nxp + 1 + (size_t)waves * x_region_slots(a.mode, a.ld)) * sizeof(double2);
+}
+
+int x_max_waves(const XArgs& a) {
+    for (int w = XNT / 64; w >= 1; --w)
+        if (x_wave_lds(a, w) <= 160 * 1024) return w;
+    return 0;
+}
+
 size_t x_lds_bytes(const XArgs& a) {
-    if (a.mode == X_ROTATIONAL && a.gd.Nxp % 2 == 0) return 2 * 6 * (size_t)a.LPB * a.ld * sizeof(double2);
-    int nphys = (a.mode == X_ROTATIONAL) ? 6 : 1;
+    if (x_wave_path(a)) return x_wave_lds(a, a.waves);
+    int nphys = x_phys_lines(a.mode);
     return 2 * (size_t)a.LPB * a.ld * sizeof(double2) + (size_t)nphys * a.LPB * a.gd.Nxp * sizeof(double);
 }
 
 hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
     long long nlines = (long long)a.gd.Nzp * a.gd.Ny;
-    dim3 grid((unsigned)((nlines + a.LPB - 1) / a.LPB));
-    hipLaunchKernelGGL(xpass_kernel, grid, dim3(XNT), x_lds_bytes(a), st, a);
+    const bool wave = x_wave_path(a);
+    if (wave && (a.waves < 1 || a.waves > XNT / 64)) return hipErrorInvalidValue;
+    const int per_block = wave ? a.waves : a.LPB;
+    dim3 grid((unsigned)((nlines + per_block - 1) / per_block));
+    size_t lds = x_lds_bytes(a);
+    if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)xpass_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(xpass_kernel, grid, dim3(wave ? 64 * a.waves : XNT), lds, st, a);
     return hipGetLastError();
 }
 

@@ -187,8 +187,12 @@ __device__ __forceinline__ void load_fac(const double* __restrict__ f, long long
     }
 }
 
-// V = U + Ub e_x at (0,0); Omega = curl V (operators.jl:172-190); values
-// of the six fields at the Lobatto points into out6.
+// Total velocity V = U + Ub e_x at (0,0) and the y-physical fields each
+// nonlinear form needs, at the Lobatto points, into out6:
+//   rotating   V, Omega = curl V   (nonlinear.jl:173-193, operators.jl:172-190)
+//   convective V, dV/dy            (nonlinear.jl:113-140; x and z derivatives
+//                                   are formed by the periodic passes)
+//   divergence V                   (nonlinear.jl:146-167)
 template <int L>
 __device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
                             const CVec U[3]) {
@@ -203,6 +207,13 @@ __device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const
     tile.dct_inverse(a.out6[0], a.dct, g, V[0]);
     tile.dct_inverse(a.out6[1], a.dct, g, V[1]);
     tile.dct_inverse(a.out6[2], a.dct, g, V[2]);
+    if (a.form == FORM_DIVERGENCE) return;
+    if (a.form == FORM_CONVECTIVE) {
+        tile.dct_inverse(a.out6[3], a.dct, g, cheb_deriv(g, V[0]));
+        tile.dct_inverse(a.out6[4], a.dct, g, cheb_deriv(g, V[1]));
+        tile.dct_inverse(a.out6[5], a.dct, g, cheb_deriv(g, V[2]));
+        return;
+    }
     CVec d3 = cheb_deriv(g, V[2]);
     CVec W;
 #pragma unroll
@@ -317,9 +328,8 @@ __device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, co
     const CVec R[3] = {unstash(Rs0, g), unstash(Rs1, g), unstash(Rs2, g)};
     // pressure: integrate Ry through degree P-1, zero bulk mean
     {
-        int up1 = g.up(1), dn1 = g.down(1);
-        double2 prev3 = bperm(R[1].v[3], up1);
-        double2 next0 = bperm(R[1].v[0], dn1);
+        double2 prev3 = g.up1(R[1].v[3]);
+        double2 next0 = g.down1(R[1].v[0]);
         CVec r = R[1];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -398,7 +408,14 @@ __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) y
     CVec R[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        CVec Qc = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
+        CVec Qc;
+        if (a.Ncoef[c]) {
+            Qc = tile.load(a.Ncoef[c], g);
+        } else {
+            Qc = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Qc.v[e] = cscale(Qc.v[e], a.nl_sign);
+        }
         double bN = 0.0;
         if (a.bulk && meanmode && c != 1) bN = bulk_mean(g, Qc).x;
         if (a.stage != 0) {
@@ -552,9 +569,8 @@ __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
     CVec v = tile.load(a.U[1], g);
     CVec Pn;
     if (mi.mean && mi.valid) {
-        int up1 = g.up(1), dn1 = g.down(1);
-        double2 prev3 = bperm(A[1].v[3], up1);
-        double2 next0 = bperm(A[1].v[0], dn1);
+        double2 prev3 = g.up1(A[1].v[3]);
+        double2 next0 = g.down1(A[1].v[0]);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int n = 4 * g.t + e;
@@ -609,12 +625,49 @@ __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
     for (int c = 0; c < a.nfields; ++c) {
         if (dir < 0) {
             CVec v = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v.v[e] = cscale(v.v[e], a.nl_sign);
             tile.store(a.Rout[c], g, v);
         } else {
             CVec v = tile.load(a.Rin[c], g);
             tile.dct_inverse(a.out6[c], a.dct, g, v);
         }
     }
+}
+
+// Divergence form (nonlinear.jl:146-167, div! of the flux tensor): the six
+// y-physical products T = (VxVx, VxVy, VxVz, VyVy, VyVz, VzVz) are forward
+// transformed and N_i = -(i k T_ix + d/dy T_iy + i l T_iz) is written to Rout.
+template <int L>
+__global__ void __launch_bounds__(YNT) ydivergence_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    load_twiddles(smem, a.dct);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Group<L> g;
+    ModeInfo mi = mode_info(a.gd, tile.s);
+    CVec N[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) zero_vec<L>(N[c]);
+    // tensor entry (i, j) for products p = 0..5
+    const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        CVec T = tile.dct_forward(a.Nphys[p], a.dct, a.norm, g, mi.nyq);
+        CVec dT = cheb_deriv(g, T);
+        // contribution to N_i from d_j T_ij and, off the diagonal, to N_j from d_i T_ij
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            if (side == 1 && pi[p] == pj[p]) continue;
+            const int i = side == 0 ? pi[p] : pj[p], j = side == 0 ? pj[p] : pi[p];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                double2 d = (j == 1) ? dT.v[e] : cmuli(cscale(T.v[e], j == 0 ? mi.kx : mi.kz));
+                N[i].v[e] = csub(N[i].v[e], d);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tile.store(a.Rout[c], g, N[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -639,6 +692,7 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     allow_lds(yneglap_kernel<L>, lds);
     allow_lds(ypressure_kernel<L>, lds);
     allow_lds(ydct_kernel<L>, lds);
+    allow_lds(ydivergence_kernel<L>, lds);
     switch (which) {
         case Y_STAGE: hipLaunchKernelGGL(ystage_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_PROLOGUE: hipLaunchKernelGGL(yprologue_kernel<L>, grid, dim3(YNT), lds, st, a); break;
@@ -647,6 +701,7 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
         case Y_PRESSURE: hipLaunchKernelGGL(ypressure_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_DCT_FWD: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, -1); break;
         case Y_DCT_INV: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, +1); break;
+        case Y_DIVERGENCE: hipLaunchKernelGGL(ydivergence_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

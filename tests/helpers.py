@@ -105,6 +105,44 @@ def dense_stokes_mode(Ny, k, l, nu, lam, Rx, Ry, Rz):
     return x[:Ny], x[Ny:2 * Ny], x[2 * Ny:3 * Ny], x[3 * Ny:]
 
 
+def stokes_mode_exact(Ny, k, l, nu, lam, Rx, Ry, Rz, refinements=6):
+    """The same square system as dense_stokes_mode, assembled in long double
+    and solved by FP64 LU with iterative refinement on long-double residuals:
+    accurate to ~cond x 1e-19, i.e. far below FP64 solver roundoff.  Returns
+    (u, v, w, p) as clongdouble."""
+    P = Ny - 1
+    D = diffmatrix(Ny).astype(np.longdouble)
+    I = np.eye(Ny, dtype=np.longdouble)
+    k, l, lam, nu = (np.longdouble(x) for x in (k, l, lam, nu))
+    H = (lam + nu * (k * k + l * l)) * I - nu * (D @ D)
+    n = np.arange(Ny)
+    walls = [np.ones(Ny, dtype=np.longdouble), ((-1.0) ** n).astype(np.longdouble)]
+    z = np.zeros(Ny, dtype=np.clongdouble)
+    rows, rhs = [], []
+    for c, (gp, src) in enumerate(zip([1j * k * I, D, 1j * l * I], [Rx, Ry, Rz])):
+        for m in range(P - 1):
+            row = [z, z, z, gp[m]]
+            row[c] = H[m]
+            rows.append(np.concatenate(row))
+            rhs.append(src[m])
+    for m in range(Ny):
+        rows.append(np.concatenate([1j * k * I[m], D[m], 1j * l * I[m], z]))
+        rhs.append(0.0)
+    for c in range(3):
+        for b in walls:
+            row = [z, z, z, z]
+            row[c] = b
+            rows.append(np.concatenate(row))
+            rhs.append(0.0)
+    A = np.array(rows, dtype=np.clongdouble)
+    b = np.array(rhs, dtype=np.clongdouble)
+    A64 = A.astype(complex)
+    x = np.linalg.solve(A64, b.astype(complex)).astype(np.clongdouble)
+    for _ in range(refinements):
+        x = x + np.linalg.solve(A64, (b - A @ x).astype(complex)).astype(np.clongdouble)
+    return x[:Ny], x[Ny:2 * Ny], x[2 * Ny:3 * Ny], x[3 * Ny:]
+
+
 def spectral_derivative(a):
     """Derivative coefficients by the dense matrix (independent of the oracle)."""
     return diffmatrix(len(a)) @ a

@@ -53,8 +53,9 @@ def test_create_rejects_bad_configs_without_device():
     assert make(dt=0.0)[0] == _lib.CF_ERR_ARGUMENT         # cnrk2.jl:60
     assert make(dt=float("inf"))[0] == _lib.CF_ERR_ARGUMENT
     assert make(vals=(float("nan"), 0.0))[0] == _lib.CF_ERR_ARGUMENT  # problem.jl:86-87
-    assert make(form=1)[0] == _lib.CF_ERR_UNSUPPORTED
-    assert b"Gibson" in L.cf_last_error() or True
+    assert make(form=3)[0] == _lib.CF_ERR_ARGUMENT          # forms: rotating, convective, divergence
+    assert b"form" in L.cf_last_error()
+    assert make(Ny=257)[0] == _lib.CF_ERR_UNSUPPORTED      # a mode's column must fit one wavefront
 
 
 def test_python_mirror_validates_constraints():

@@ -40,6 +40,49 @@ def test_waleffe_csv_on_gpu(gpu, branch, Ny, N):
             assert _close(row[key], r[key], tol), (key, row[key], r[key])
 
 
+@pytest.mark.parametrize("name", ["Poiseuille", "Couette"])
+@pytest.mark.parametrize("n", [1, 3])
+def test_viscous_decay_csv_on_gpu(gpu, name, n):
+    """validation/results/decay.csv (ConvectiveForm, 5x33x8) reproduced by the
+    GPU trajectory with the oracle's own bounds."""
+    import gpu_backend as GB
+    say("decay csv gpu", name, n)
+    ref = [r for r in GC.read_csv("decay.csv") if r["flow"] == name and int(r["n"]) == n]
+    errors = []
+    for dt in (0.5, 0.25, 0.125):
+        rows = GC.decay_case(GB, name, n, dt)
+        for row in rows:
+            r = [x for x in ref if float(x["dt"]) == dt and float(x["t"]) == row["t"]][0]
+            say(f"  dt={dt} t={row['t']} energy ratio gpu {row['energy_ratio']!r} csv {r['energy_ratio']}")
+            assert _close(row["mu"], r["mu"], 1e-15)
+            assert _close(row["energy_ratio"], r["energy_ratio"], 1e-13), (row["energy_ratio"], r["energy_ratio"])
+            assert _close(row["error"], r["error"], 1e-6), (row["error"], r["error"])
+        errors.append(rows[-1]["error"])
+    assert 3.5 < errors[0] / errors[1] < 4.5 and 3.5 < errors[1] / errors[2] < 4.5
+
+
+def test_tollmien_schlichting_csv_on_gpu(gpu):
+    """validation/results/ts_history.csv and ts_convergence.csv (ConvectiveForm,
+    8x81x1) reproduced by the GPU trajectory with the oracle's own bounds."""
+    import gpu_backend as GB
+    hist = GC.read_csv("ts_history.csv")
+    conv = GC.read_csv("ts_convergence.csv")
+    errors = []
+    for dt in (0.4, 0.2, 0.1):
+        say("ts csv gpu", dt)
+        rows, c, _ = GC.ts_case(GB, dt)
+        for row in rows:
+            r = [h for h in hist if float(h["dt"]) == dt and float(h["t"]) == row["t"]][0]
+            assert _close(row["growth"], r["growth"], 1e-9)
+            assert _close(row["frequency"], r["frequency"], 1e-9)
+            assert _close(row["energy_ratio"], r["energy_ratio"], 1e-9)
+        rc = [x for x in conv if float(x["dt"]) == dt][0]
+        say(f"  error gpu {c['error']!r} csv {rc['error']}")
+        assert abs(c["error"] - float(rc["error"])) < 5e-11
+        errors.append(c["error"])
+    assert errors[0] / errors[1] > 3.5 and errors[1] / errors[2] > 3.5 and errors[-1] < 1e-5
+
+
 def _decay_rotating(B, name, n, dt, T=4.0):
     Re, A = 100.0, 0.1
     nu = 1 / Re

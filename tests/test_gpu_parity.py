@@ -17,6 +17,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 ENVELOPE_FACTOR = 16.0
+ACCURACY_FACTOR = 4.0  # HIP error vs the exact discrete Stokes solution <= 4 x the oracle's
 
 
 def rel(a, b):
@@ -65,7 +66,11 @@ def hermitian_random(g, rng, n=None, scale=1.0):
     return np.stack([O.forward(g, scale * rng.standard_normal(shape)) for _ in range(n)])
 
 
-GRIDS = [(8, 17, 8), (9, 17, 9), (6, 9, 8), (7, 9, 9), (16, 33, 12), (32, 33, 32), (12, 35, 10), (1, 9, 5), (5, 9, 1)]
+# wall-normal group sizes covered: Ny=9 -> 4 lanes per mode, 17 -> 8, 33/35 -> 16,
+# 65/97 -> 32 (two modes per wavefront, cross-row scans), 129/255 -> 64 (one
+# mode per wavefront; 255 = largest supported, DCT length 508 = 4 x 127)
+GRIDS = [(8, 17, 8), (9, 17, 9), (6, 9, 8), (7, 9, 9), (16, 33, 12), (32, 33, 32), (12, 35, 10), (1, 9, 5), (5, 9, 1),
+         (8, 65, 6), (10, 97, 12), (6, 129, 4), (4, 255, 3)]
 
 
 def make(cf, dims, L=(5.3, 7.1), profile=lambda y: y, nu=1 / 400, dt=0.01, **kw):
@@ -91,17 +96,23 @@ def test_transforms(gpu, dims):
     assert rel(v, O.inverse(go, V)) < 1e-13
 
 
+FORMS = ["rotating", "convective", "divergence"]
+
+
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("dims", GRIDS)
-def test_nonlinear(gpu, dims):
+def test_nonlinear(gpu, dims, form):
     import torch
     cf = gpu
-    say("nonlinear", dims)
-    gc, go, prob = make(cf, dims)
+    say("nonlinear", dims, form)
+    gc, go, prob = make(cf, dims, form=form)
     rng = np.random.default_rng(3 + sum(dims))
     U = hermitian_random(go, rng, 3)
     N = prob.nlterm(0.0, torch.as_tensor(U, device="cuda")).cpu().numpy()
-    ref = O.Problem(go, lambda y: y, 1 / 400, 0.01).nonlinear(U)
-    assert rel(N, ref) < 1e-12
+    ref = O.Problem(go, lambda y: y, 1 / 400, 0.01, form=form).nonlinear(U)
+    err = rel(N, ref)
+    say(f"  rel err {err:.3e}")
+    assert err < 1e-12
 
 
 @pytest.mark.parametrize("dims", GRIDS)
@@ -117,15 +128,51 @@ def test_stokes_solve(gpu, dims, lam, constraint):
     nu = 1 / 400
     kw = {"bulkvelocity": (0.7, -0.1)} if constraint == "bulk" else {"pressuregradient": (-0.3, 0.2)}
     U, P, grads = cf.stokes_solve(prob, nu, lam, torch.as_tensor(R, device="cuda"), **kw)
+    U, P = U.cpu().numpy(), P.cpu().numpy()
     basec = O.chebcoeffs(1 - go.y ** 2)
     solve = lambda R_: O.stokes(go, nu, lam, R_, baseflow=basec, **kw)[:2]
     Ur, Pr, gr = O.stokes(go, nu, lam, R, baseflow=basec, **kw)
     env = roundoff_envelope(solve, [R], [Ur, Pr])
-    assert_parity("stokes", [U.cpu().numpy(), P.cpu().numpy()], [Ur, Pr], env)
+    # Accuracy of both FP64 solvers against the exact discrete solution (long
+    # double refined dense solve) on the non-mean, non-Nyquist modes: at large
+    # Ny and lambda the tau system's conditioning (~1e7) puts both at ~1e-11,
+    # above the input-perturbation envelope.  The HIP solve must be at least
+    # as accurate as the oracle (up to 2x) and agree with it within the
+    # oracle's own accuracy.  ACCURACY_FACTOR allows for the different
+    # summation orders (parallel scans, re-solve form) of two FP64 solvers.
+    err_gpu, err_ora = exact_errors(go, nu, lam, R, [U, P], [Ur, Pr])
+    for i in range(2):
+        say(f"stokes[{i}] vs exact: hip {err_gpu[i]:.3e}  oracle {err_ora[i]:.3e}")
+        assert err_gpu[i] <= max(TOL, ACCURACY_FACTOR * err_ora[i]), ("accuracy", i, err_gpu[i], err_ora[i])
+    env = [max(e, ACCURACY_FACTOR * a / ENVELOPE_FACTOR) for e, a in zip(env, err_ora)]
+    assert_parity("stokes", [U, P], [Ur, Pr], env)
     assert np.allclose(grads, gr, rtol=1e-11, atol=1e-14)
 
 
-@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 9, 7), (32, 33, 32)])
+def exact_errors(go, nu, lam, R, got, ref, max_modes=24):
+    """Relative max errors of `got` and `ref` (U, P) against the exact
+    discrete Stokes solution over (up to max_modes) non-mean, non-Nyquist modes."""
+    from helpers import stokes_mode_exact
+    Nx, Ny, Nz = go.Nx, go.Ny, go.Nz
+    modes = [(iz, ix) for iz in range(Nz) for ix in range(Nx // 2 + 1)
+             if not (ix == 0 and iz == 0) and not (Nx % 2 == 0 and ix == Nx // 2)
+             and not (Nz % 2 == 0 and iz == Nz // 2)]
+    modes = modes[:: max(1, len(modes) // max_modes)]
+    diff_g, diff_r, scale = np.zeros(2), np.zeros(2), np.zeros(2)
+    for iz, ix in modes:
+        l = iz if iz <= Nz // 2 else iz - Nz
+        k, lw = 2 * np.pi / go.Lx * ix, 2 * np.pi / go.Lz * l
+        u, v, w, p = stokes_mode_exact(Ny, k, lw, nu, lam, *(R[c, :, iz, ix] for c in range(3)))
+        Ue = np.stack([u, v, w])
+        for i, (ex, a, b) in enumerate([(Ue, got[0][:, :, iz, ix], ref[0][:, :, iz, ix]),
+                                        (p, got[1][:, iz, ix], ref[1][:, iz, ix])]):
+            scale[i] = max(scale[i], float(np.abs(ex).max()))
+            diff_g[i] = max(diff_g[i], float(np.abs(a - ex).max()))
+            diff_r[i] = max(diff_r[i], float(np.abs(b - ex).max()))
+    return diff_g / scale, diff_r / scale
+
+
+@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 9, 7), (32, 33, 32), (8, 97, 6), (4, 129, 4)])
 @pytest.mark.parametrize("bulk", [False, True])
 def test_project_and_pressure(gpu, dims, bulk):
     import torch
@@ -167,11 +214,34 @@ def _split(up):
     return [U[0], U[1], U[2], P]
 
 
-@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 17, 9), (16, 17, 16), (6, 9, 5)])
+@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 17, 9), (16, 17, 16), (6, 9, 5), (8, 97, 6), (4, 129, 6)])
 def test_step_small(gpu, dims):
     say("step small", dims)
     U, P, Ur, Pr, _, _, env = run_pair(gpu, dims, 5)
     assert_parity(f"step{dims}", _split((U, P)), _split((Ur, Pr)), env)
+
+
+@pytest.mark.parametrize("form", ["convective", "divergence"])
+@pytest.mark.parametrize("dims", [(8, 17, 8), (7, 9, 9), (6, 33, 4), (8, 97, 6)])
+def test_step_forms(gpu, dims, form):
+    say("step", form, dims)
+    U, P, Ur, Pr, _, _, env = run_pair(gpu, dims, 5, form=form)
+    assert_parity(f"step {form} {dims}", _split((U, P)), _split((Ur, Pr)), env)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_project_pressure_forms(gpu, form):
+    import torch
+    cf = gpu
+    say("pressure", form)
+    dims = (8, 33, 6)
+    gc, go, prob = make(cf, dims, profile=lambda y: 1 - y ** 2, form=form, pressuregradient=(-2 / 400, 0.0))
+    ref = O.Problem(go, lambda y: 1 - y ** 2, 1 / 400, 0.01, form=form, pressuregradient=(-2 / 400, 0.0))
+    U = ref.project(hermitian_random(go, np.random.default_rng(5), 3, 0.1))
+    P = cf.pressure(torch.as_tensor(U, device="cuda"), prob).cpu().numpy()
+    Pref = ref.pressure(U)
+    env = roundoff_envelope(lambda U_: [ref.pressure(U_)], [U], [Pref])
+    assert_parity(f"pressure {form}", [P], [Pref], env)
 
 
 def test_step_couette_32_100steps(gpu):
@@ -182,15 +252,17 @@ def test_step_couette_32_100steps(gpu):
     assert_parity("couette32x100", _split((U, P)), _split((Ur, Pr)), env)
 
 
+@pytest.mark.parametrize("dims", [(16, 33, 16), (8, 97, 8)])
 @pytest.mark.parametrize("bulk", [False, True])
-def test_step_poiseuille(gpu, bulk):
-    say("step poiseuille bulk", bulk)
+def test_step_poiseuille(gpu, dims, bulk):
+    say("step poiseuille", dims, "bulk", bulk)
     nu = 1 / 2800
     kw = {"bulkvelocity": (2 / 3, 0.0)} if bulk else {"pressuregradient": (-2 * nu, 0.0)}
-    U, P, Ur, Pr, prob, ref, env = run_pair(gpu, (16, 33, 16), 10, L=(4 * np.pi, 4 * np.pi / 3),
+    U, P, Ur, Pr, prob, ref, env = run_pair(gpu, dims, 10, L=(4 * np.pi, 4 * np.pi / 3),
                                             profile=lambda y: 1 - y ** 2, nu=nu, dt=0.005, eps=0.05, **kw)
     assert_parity(f"poiseuille bulk={bulk}", _split((U, P)), _split((Ur, Pr)), env)
     import channelflow as cf
     g = cf.last_gradient(prob.scheme)
     gr = ref.last_gradient()
-    assert np.allclose(g, gr, rtol=1e-10, atol=1e-15)
+    # the gradient is a functional of the velocity: same relative envelope
+    assert np.allclose(g, gr, rtol=max(1e-10, ENVELOPE_FACTOR * env[0]), atol=1e-15)
