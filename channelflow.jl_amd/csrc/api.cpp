@@ -187,6 +187,10 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
     if (cfg->form != CF_FORM_ROTATING && cfg->form != CF_FORM_CONVECTIVE && cfg->form != CF_FORM_DIVERGENCE)
         return fail(CF_ERR_ARGUMENT, "unknown nonlinear form");
     if (y_group_lanes(cfg->Ny) == 0) return fail(CF_ERR_UNSUPPORTED, "Ny above 255 is not supported");
+    // device indexing: mode counts and padded line counts in 32 bits
+    if ((long long)(cfg->Nx / 2 + 1) * cfg->Nz * cfg->Ny >= (1ll << 31) ||
+        (long long)((3 * cfg->Nx + 1) / 2) * ((3 * cfg->Nz + 1) / 2) * cfg->Ny >= (1ll << 31))
+        return fail(CF_ERR_UNSUPPORTED, "grid too large for one device");
 
     cf_problem* h = new cf_problem();
     h->device = cfg->device;

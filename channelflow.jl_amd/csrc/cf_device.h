@@ -88,10 +88,14 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 // within the group, r = index within the DPP row.  Groups of L <= 16 lie in
 // one row; L = 32 is a row pair, L = 64 the whole wave.  Every lane of a group
 // must call a group operation together (values of other groups are never used).
+//
+// tab: per-degree constants (see build_cheb_table), NT = 4L entries per array.
 template <int L>
 struct Group {
+    static constexpr int NT = 4 * L;
     int lane, t, r;
-    __device__ __forceinline__ Group() {
+    const double* tab;
+    __device__ __forceinline__ explicit Group(const double* tab_ = nullptr) : tab(tab_) {
         lane = lane_id();
         t = lane & (L - 1);
         r = lane & 15;
@@ -244,6 +248,25 @@ struct Group {
     }
 };
 
+// Per-degree constants of the Chebyshev tau rows and of the bulk mean,
+// built once per block (LDS) so that the solves need no FP64 divisions:
+//   [0, nt)    c1(n) = c_{n-2} / (4 n (n-1))      (n >= 2; c_0 = 2, else 1)
+//   [nt, 2nt)  c2(n) = -1 / (2 (n^2 - 1))        (n >= 2)
+//   [2nt, 3nt) c3(n) = 1 / (4 n (n+1))           (n >= 2)
+//   [3nt, 4nt) bw(n) = 1 / (1 - n^2), even n; 0 for odd n
+// (tau rows: docs/src/spatial.md:448-470; bulk mean: meanmode.jl:13-19)
+constexpr int CHEB_TAB_ARRAYS = 4;
+__device__ __forceinline__ void build_cheb_table(double* tab, int nt, int tid, int nthr) {
+    for (int n = tid; n < nt; n += nthr) {
+        const double nn = n;
+        const bool row = n >= 2;
+        tab[n] = row ? (n == 2 ? 2.0 : 1.0) / (4.0 * nn * (nn - 1.0)) : 0.0;
+        tab[nt + n] = row ? -1.0 / (2.0 * (nn * nn - 1.0)) : 0.0;
+        tab[2 * nt + n] = row ? 1.0 / (4.0 * nn * (nn + 1.0)) : 0.0;
+        tab[3 * nt + n] = (n % 2 == 0) ? 1.0 / (1.0 - nn * nn) : 0.0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Chebyshev coefficient vectors in the blocked group layout: lane t holds
 // v[e] = a_{4t+e}.  Entries above degree P must be zero.
@@ -299,11 +322,9 @@ __device__ __forceinline__ double2 wall_deriv_minus(const Group<L>& g, const CVe
 template <int L>
 __device__ __forceinline__ double2 bulk_mean(const Group<L>& g, const CVec& a) {  // a_0 + sum_{even m>=2} a_m/(1-m^2)
     double2 s = czero();
+    const double* bw = g.tab + 3 * Group<L>::NT;
 #pragma unroll
-    for (int e = 0; e < 4; e += 2) {
-        double n = 4 * g.t + e;
-        s = cfma(1.0 / (1.0 - n * n), a.v[e], s);
-    }
+    for (int e = 0; e < 4; e += 2) s = cfma(bw[4 * g.t + e], a.v[e], s);
     return g.sum(s);
 }
 
@@ -342,23 +363,24 @@ __device__ __forceinline__ CVec helm_solve(const Group<L>& g, int P, double t1, 
     // per element: elimination coefficients; A = s*rhs, T = -up*s (backward), alpha = -lo*s
     double T[4], alpha[4];
     double2 A[4];
+    constexpr int NT = Group<L>::NT;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         int n = n0 + e;
-        double nn = (double)n;
         bool row = (n >= 2) && (n <= P);
         double s = row ? fac[e] : 0.0;
         double en2 = (n + 2 <= P - 2) ? 1.0 : 0.0;
-        double lo = -t1 * cfac_(n - 2) / (4.0 * nn * (nn - 1.0));
-        double up = -t1 * en2 / (4.0 * nn * (nn + 1.0));
+        const double c1 = g.tab[n], c2 = g.tab[NT + n], c3 = g.tab[2 * NT + n];
+        double lo = -t1 * c1;
+        double up = -t1 * en2 * c3;
         double2 Fm = (n - 2 >= 0 && n - 2 <= P - 2) ? fm2[e] : czero();
         double2 F0 = (n <= P - 2) ? f.v[e] : czero();
         double2 Fp = (n + 2 <= P - 2) ? fp2[e] : czero();
         double2 rhs = czero();
         if (row) {
-            rhs = cscale(Fm, cfac_(n - 2) / (4.0 * nn * (nn - 1.0)));
-            rhs = cfma(-1.0 / (2.0 * (nn * nn - 1.0)), F0, rhs);
-            rhs = cfma(1.0 / (4.0 * nn * (nn + 1.0)), Fp, rhs);
+            rhs = cscale(Fm, c1);
+            rhs = cfma(c2, F0, rhs);
+            rhs = cfma(c3, Fp, rhs);
         }
         A[e] = cscale(rhs, s);
         T[e] = row ? -up * s : 0.0;

@@ -28,9 +28,9 @@ __device__ __forceinline__ ModeInfo mode_info(const GridDims& gd, long long s) {
     ModeInfo mi;
     mi.s = s;
     mi.valid = s < gd.B;
-    long long ss = mi.valid ? s : 0;
-    mi.ix = (int)(ss % gd.Nxh);
-    mi.iz = (int)(ss / gd.Nxh);
+    const unsigned ss = mi.valid ? (unsigned)s : 0u;  // B = Nxh*Nz < 2^31
+    mi.iz = (int)(ss / (unsigned)gd.Nxh);
+    mi.ix = (int)ss - mi.iz * gd.Nxh;
     mi.l = mi.iz <= (gd.Nz >> 1) ? mi.iz : mi.iz - gd.Nz;
     mi.kx = gd.alpha * mi.ix;
     mi.kz = gd.beta * mi.l;
@@ -39,6 +39,10 @@ __device__ __forceinline__ ModeInfo mode_info(const GridDims& gd, long long s) {
     mi.mean = (ss == 0);
     return mi;
 }
+
+// LDS slots (double2) of the per-degree table: CHEB_TAB_ARRAYS arrays of 4L doubles
+template <int L>
+__host__ __device__ constexpr int y_table_slots() { return CHEB_TAB_ARRAYS * 4 * L / 2; }
 
 // ---------------------------------------------------------------------------
 // Per-wavefront tile.  Wave w of block b owns the W = 64/L consecutive modes
@@ -58,7 +62,9 @@ struct Tile {
     double2 *bufA, *bufB, *stash;
     const double2* tw;
     // region: LDS slots per wave (y_region_slots); smem starts with the
-    // twiddle table of the DCT plan (ntw entries)
+    // twiddle table of the DCT plan (ntw entries) and the per-degree table
+    // (y_table_slots<L>), see load_tables
+    const double* tab;
     __device__ __forceinline__ Tile(const GridDims& g_, double2* smem, int ntw, int ld_, int region)
         : gd(g_), ld(ld_) {
         const int tid = threadIdx.x, wave = tid >> 6;
@@ -67,7 +73,8 @@ struct Tile {
         sw0 = (long long)blockIdx.x * M + wave * W;
         s = sw0 + mw;
         tw = smem;
-        bufA = smem + ntw + (size_t)wave * region;
+        tab = reinterpret_cast<const double*>(smem + ntw);
+        bufA = smem + ntw + y_table_slots<L>() + (size_t)wave * region;
         bufB = bufA + W * ld;
         stash = bufA + mw * 12 * L;
     }
@@ -146,9 +153,12 @@ __host__ __device__ constexpr int y_region_slots(int ld) {
     return (2 * (64 / L) * ld > 12 * 64) ? 2 * (64 / L) * ld : 12 * 64;
 }
 
-// Block prologue of every y kernel: the DCT twiddle table into LDS.
-__device__ __forceinline__ void load_twiddles(double2* smem, const FFTPlan& plan) {
+// Block prologue of every y kernel: the DCT twiddle table and the
+// per-degree table into LDS.
+template <int L>
+__device__ __forceinline__ void load_tables(double2* smem, const FFTPlan& plan) {
     for (int i = threadIdx.x; i < plan.n; i += YNT) smem[i] = plan.tw[i];
+    build_cheb_table(reinterpret_cast<double*>(smem + plan.n), 4 * L, threadIdx.x, YNT);
     __syncthreads();
 }
 
@@ -394,9 +404,9 @@ template <int L>
 // spill: one wave per SIMD leaves the wave-serial scans and LDS FFTs unhidden
 __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_twiddles(smem, a.dct);
+    load_tables<L>(smem, a.dct);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
-    Group<L> g;
+    Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny;
     const bool meanmode = mi.mean && mi.valid;
@@ -493,9 +503,9 @@ __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) y
 template <int L>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_twiddles(smem, a.dct);
+    load_tables<L>(smem, a.dct);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
-    Group<L> g;
+    Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     CVec U[3];
 #pragma unroll
@@ -507,9 +517,9 @@ __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_twiddles(smem, a.dct);
+    load_tables<L>(smem, a.dct);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
-    Group<L> g;
+    Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     CVec R[3];
 #pragma unroll
@@ -541,9 +551,9 @@ __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_twiddles(smem, a.dct);
+    load_tables<L>(smem, a.dct);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
-    Group<L> g;
+    Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < 3; ++c) {
         CVec U = tile.load(a.U[c], g);
@@ -559,9 +569,9 @@ __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_twiddles(smem, a.dct);
+    load_tables<L>(smem, a.dct);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
-    Group<L> g;
+    Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int P = a.gd.P, Ny = a.gd.Ny;
     CVec A[3];
@@ -618,9 +628,9 @@ __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
     extern __shared__ double2 smem[];
-    load_twiddles(smem, a.dct);
+    load_tables<L>(smem, a.dct);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
-    Group<L> g;
+    Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < a.nfields; ++c) {
         if (dir < 0) {
@@ -641,9 +651,9 @@ __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ydivergence_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_twiddles(smem, a.dct);
+    load_tables<L>(smem, a.dct);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
-    Group<L> g;
+    Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     CVec N[3];
 #pragma unroll
@@ -729,7 +739,7 @@ size_t y_lds_bytes(int Ny, int ld) {
         case 64: region = y_region_slots<64>(ld); break;
         default: return 0;
     }
-    return (2 * (size_t)(Ny - 1) + (YNT / 64) * (size_t)region) * sizeof(double2);
+    return (2 * (size_t)(Ny - 1) + (size_t)CHEB_TAB_ARRAYS * 4 * L / 2 + (YNT / 64) * (size_t)region) * sizeof(double2);
 }
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
