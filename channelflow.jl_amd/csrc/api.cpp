@@ -69,6 +69,7 @@ struct cf_problem {
     double* d_base = nullptr;
     double* d_curv = nullptr;
     double* d_facP = nullptr;
+    double* d_tab = nullptr;  // per-degree tau / bulk-mean constants
     DevBank stage[3];
     std::vector<std::pair<std::pair<double, double>, DevBank>> extra;
     FFTPlan pz{}, py{}, px{};
@@ -242,6 +243,11 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
     std::vector<double> facP;
     build_pressure_factors(gd, facP);
     if ((rc = upload(h, &h->d_facP, facP))) return bail(rc);
+    {
+        std::vector<double> tab;
+        build_cheb_table(tab);
+        if ((rc = upload(h, &h->d_tab, tab))) return bail(rc);
+    }
     for (int j = 0; j < 3; ++j)
         if ((rc = make_bank(h, h->nu, 1.0 / (CNRK2_C[j] * h->dt), h->stage[j]))) return bail(rc);
     // plans
@@ -330,6 +336,8 @@ static YArgs base_yargs(cf_problem* h) {
     a.facPN = h->d_facP;
     a.form = h->form;
     a.nl_sign = 1.0;
+    a.cheb_tab = h->d_tab;
+    a.ntab = CHEB_TAB_MAX;
     return a;
 }
 

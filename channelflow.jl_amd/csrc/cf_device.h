@@ -255,17 +255,8 @@ struct Group {
 //   [2nt, 3nt) c3(n) = 1 / (4 n (n+1))           (n >= 2)
 //   [3nt, 4nt) bw(n) = 1 / (1 - n^2), even n; 0 for odd n
 // (tau rows: docs/src/spatial.md:448-470; bulk mean: meanmode.jl:13-19)
-constexpr int CHEB_TAB_ARRAYS = 4;
-__device__ __forceinline__ void build_cheb_table(double* tab, int nt, int tid, int nthr) {
-    for (int n = tid; n < nt; n += nthr) {
-        const double nn = n;
-        const bool row = n >= 2;
-        tab[n] = row ? (n == 2 ? 2.0 : 1.0) / (4.0 * nn * (nn - 1.0)) : 0.0;
-        tab[nt + n] = row ? -1.0 / (2.0 * (nn * nn - 1.0)) : 0.0;
-        tab[2 * nt + n] = row ? 1.0 / (4.0 * nn * (nn + 1.0)) : 0.0;
-        tab[3 * nt + n] = (n % 2 == 0) ? 1.0 / (1.0 - nn * nn) : 0.0;
-    }
-}
+// The host builds the table once per problem (build_cheb_table in setup.cpp);
+// every y block copies it into LDS.
 
 // ---------------------------------------------------------------------------
 // Chebyshev coefficient vectors in the blocked group layout: lane t holds

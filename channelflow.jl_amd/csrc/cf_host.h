@@ -25,6 +25,10 @@ void build_pressure_factors(const GridDims& gd, std::vector<double>& facP);
 bool build_bank(const GridDims& gd, double nu, double lambda, HostBank& hb, std::string& err);
 void plan_factors(int n, std::vector<int>& radix);
 void twiddles(int n, std::vector<double>& tw);
+// per-degree constants of the Chebyshev tau rows and the bulk mean, 4 arrays
+// of CHEB_TAB_MAX entries: c_{n-2}/(4n(n-1)), -1/(2(n^2-1)), 1/(4n(n+1)) for
+// n >= 2 (else 0) and 1/(1-n^2) for even n (else 0)
+void build_cheb_table(std::vector<double>& tab);
 void host_chebcoeffs(int Ny, const double* v, double* c);
 void host_diff(double* b, const double* a, int Ny);
 double host_bulkmean(const double* a, int Ny);

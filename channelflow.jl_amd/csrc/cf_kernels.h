@@ -37,6 +37,8 @@ struct YArgs {
     double curv_mean;    // bulk mean of Ub''
     const double* facP;  // (B, Ny) factors of D^2 - kappa^2 (Dirichlet; kappa^2=1 placeholder at (0,0))
     const double* facPN; // (B, Ny) factors of D^2 - kappa^2 with Neumann rows (pressure init)
+    const double* cheb_tab;  // CHEB_TAB_ARRAYS x ntab per-degree constants (build_cheb_table layout)
+    int ntab;                // entries per array (= 4L of the launch)
     Bank bank;
     int form;                 // FORM_*: which y-physical fields the epilogue emits
     double nl_sign;           // sign applied to the transformed nonlinear term

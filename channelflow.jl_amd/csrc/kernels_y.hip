@@ -156,9 +156,14 @@ __host__ __device__ constexpr int y_region_slots(int ld) {
 // Block prologue of every y kernel: the DCT twiddle table and the
 // per-degree table into LDS.
 template <int L>
-__device__ __forceinline__ void load_tables(double2* smem, const FFTPlan& plan) {
-    for (int i = threadIdx.x; i < plan.n; i += YNT) smem[i] = plan.tw[i];
-    build_cheb_table(reinterpret_cast<double*>(smem + plan.n), 4 * L, threadIdx.x, YNT);
+__device__ __forceinline__ void load_tables(double2* smem, const YArgs& a) {
+    for (int i = threadIdx.x; i < a.dct.n; i += YNT) smem[i] = a.dct.tw[i];
+    double* tab = reinterpret_cast<double*>(smem + a.dct.n);
+    constexpr int NT = 4 * L;
+    for (int i = threadIdx.x; i < CHEB_TAB_ARRAYS * NT; i += YNT) {
+        const int arr = i / NT, n = i - arr * NT;
+        tab[i] = a.cheb_tab[arr * a.ntab + n];
+    }
     __syncthreads();
 }
 
@@ -404,7 +409,7 @@ template <int L>
 // spill: one wave per SIMD leaves the wave-serial scans and LDS FFTs unhidden
 __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a.dct);
+    load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
@@ -503,7 +508,7 @@ __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) y
 template <int L>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a.dct);
+    load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
@@ -517,7 +522,7 @@ __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a.dct);
+    load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
@@ -551,7 +556,7 @@ __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a.dct);
+    load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
@@ -569,7 +574,7 @@ __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a.dct);
+    load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
@@ -628,7 +633,7 @@ __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a.dct);
+    load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
@@ -651,7 +656,7 @@ __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
 template <int L>
 __global__ void __launch_bounds__(YNT) ydivergence_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a.dct);
+    load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);

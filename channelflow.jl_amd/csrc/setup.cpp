@@ -226,6 +226,20 @@ void plan_factors(int n, std::vector<int>& radix) {
     if (radix.empty()) radix.push_back(1);
 }
 
+void build_cheb_table(std::vector<double>& tab) {
+    const int nt = CHEB_TAB_MAX;
+    tab.assign((size_t)CHEB_TAB_ARRAYS * nt, 0.0);
+    for (int n = 0; n < nt; ++n) {
+        const double nn = n;
+        if (n >= 2) {
+            tab[n] = (n == 2 ? 2.0 : 1.0) / (4.0 * nn * (nn - 1.0));
+            tab[nt + n] = -1.0 / (2.0 * (nn * nn - 1.0));
+            tab[2 * nt + n] = 1.0 / (4.0 * nn * (nn + 1.0));
+        }
+        if (n % 2 == 0) tab[3 * nt + n] = 1.0 / (1.0 - nn * nn);
+    }
+}
+
 void twiddles(int n, std::vector<double>& tw) {
     tw.resize(2 * (size_t)std::max(n, 1));
     const long double PI = 3.141592653589793238462643383279502884L;
