@@ -45,13 +45,28 @@ enum : int {
     ROW_BCAST31 = 0x143,     // rows 2,3 <- lane 31 (rows selected by the row mask)
     ROW_NEWBCAST0 = 0x150,   // every lane <- first lane of its row (64-bit)
 };
+// Lanes whose source lies outside the row, or whose row is not in ROWMASK,
+// receive an unspecified value (callers select them away).
 template <int CTRL, int ROWMASK = 0xF>
 __device__ __forceinline__ double mov(double x) {
-    return __longlong_as_double(__builtin_amdgcn_update_dpp(0ll, __double_as_longlong(x), CTRL, ROWMASK, 0xF, false));
+    return __longlong_as_double(__builtin_amdgcn_mov_dpp(__double_as_longlong(x), CTRL, ROWMASK, 0xF, false));
 }
 template <int CTRL, int ROWMASK = 0xF>
 __device__ __forceinline__ double2 mov(double2 x) {
     return make_double2(mov<CTRL, ROWMASK>(x.x), mov<CTRL, ROWMASK>(x.y));
+}
+// Lanes whose source lies outside the row receive 0 (the additive identity).
+template <int CTRL>
+__device__ __forceinline__ double mov0(double x) {
+    return __longlong_as_double(__builtin_amdgcn_mov_dpp(__double_as_longlong(x), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double2 mov0(double2 x) { return make_double2(mov0<CTRL>(x.x), mov0<CTRL>(x.y)); }
+// Lanes whose source lies outside the row receive 1 (the multiplicative identity).
+template <int CTRL>
+__device__ __forceinline__ double mov1(double x) {
+    return __longlong_as_double(
+        __builtin_amdgcn_update_dpp(0x3FF0000000000000ll, __double_as_longlong(x), CTRL, 0xF, 0xF, false));
 }
 // value held by the odd row of this lane's row pair (rows 0/1, 2/3), same column
 __device__ __forceinline__ double odd_row(double x) {
@@ -154,11 +169,18 @@ struct Group {
     __device__ __forceinline__ bool prefix_ok() const {
         return ((L <= 16) ? t : r) >= D;
     }
+    // Steps inside a DPP row.  For L >= 16 a group spans whole rows, so the
+    // lanes whose source falls outside the row are exactly those that must
+    // combine with the identity, which bound-controlled moves deliver.
     template <int D>
     __device__ __forceinline__ void suffix_sum_step(double2& v) const {
         if constexpr (D < L && D < 16) {
-            double2 o = dpp::mov<dpp::ROW_SHL + D>(v);
-            if (suffix_ok<D>()) v = cadd(v, o);
+            if constexpr (L >= 16) {
+                v = cadd(v, dpp::mov0<dpp::ROW_SHL + D>(v));
+            } else {
+                double2 o = dpp::mov<dpp::ROW_SHL + D>(v);
+                if (suffix_ok<D>()) v = cadd(v, o);
+            }
         }
     }
     __device__ __forceinline__ double2 suffix_sum(double2 v) const {
@@ -183,11 +205,18 @@ struct Group {
     template <int D>
     __device__ __forceinline__ void suffix_affine_step(double& T, double2& A) const {
         if constexpr (D < L && D < 16) {
-            double oT = dpp::mov<dpp::ROW_SHL + D>(T);
-            double2 oA = dpp::mov<dpp::ROW_SHL + D>(A);
-            if (suffix_ok<D>()) {
+            if constexpr (L >= 16) {
+                double oT = dpp::mov1<dpp::ROW_SHL + D>(T);
+                double2 oA = dpp::mov0<dpp::ROW_SHL + D>(A);
                 A = cfma(T, oA, A);
                 T *= oT;
+            } else {
+                double oT = dpp::mov<dpp::ROW_SHL + D>(T);
+                double2 oA = dpp::mov<dpp::ROW_SHL + D>(A);
+                if (suffix_ok<D>()) {
+                    A = cfma(T, oA, A);
+                    T *= oT;
+                }
             }
         }
     }
@@ -216,11 +245,18 @@ struct Group {
     template <int D>
     __device__ __forceinline__ void prefix_affine_step(double& T, double2& A) const {
         if constexpr (D < L && D < 16) {
-            double oT = dpp::mov<dpp::ROW_SHR + D>(T);
-            double2 oA = dpp::mov<dpp::ROW_SHR + D>(A);
-            if (prefix_ok<D>()) {
+            if constexpr (L >= 16) {
+                double oT = dpp::mov1<dpp::ROW_SHR + D>(T);
+                double2 oA = dpp::mov0<dpp::ROW_SHR + D>(A);
                 A = cfma(T, oA, A);
                 T *= oT;
+            } else {
+                double oT = dpp::mov<dpp::ROW_SHR + D>(T);
+                double2 oA = dpp::mov<dpp::ROW_SHR + D>(A);
+                if (prefix_ok<D>()) {
+                    A = cfma(T, oA, A);
+                    T *= oT;
+                }
             }
         }
     }
