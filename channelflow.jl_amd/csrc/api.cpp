@@ -81,6 +81,8 @@ struct cf_problem {
     double* d_grad = nullptr;
     int KT = 1, zld = 0, LPB = 1, xld = 0, yld = 0;
     int xwaves = 4;  // even Nxp: wavefronts per block of the nonlinear x pass
+    int zppt = 0;    // z pass in-place points per thread (0: two buffers)
+    int xppt = 0;    // even-Nxp x pass in-place points per lane (0: two buffers)
 };
 
 struct DeviceGuard {
@@ -264,10 +266,17 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
     }
     // launch geometry (LDS <= 64 KB per block)
     const size_t LDS_MAX = 64 * 1024;
-    h->yld = 2 * gd.P;
+    h->yld = 2 * gd.P + 1;  // odd stride: the two lines of a wave fall in different LDS banks
     h->zld = gd.Nzp + 1;
-    h->KT = (int)std::max<size_t>(1, std::min<size_t>(16, LDS_MAX / (2 * (size_t)h->zld * sizeof(double2))));
+    // z pass: one LDS buffer of ~32 KB (several blocks per CU) when the
+    // in-place FFT applies, else two buffers within LDS_MAX
+    h->KT = (int)std::max<size_t>(1, std::min<size_t>(16, (32 * 1024) / ((size_t)h->zld * sizeof(double2))));
     h->KT = std::min(h->KT, gd.Nxh);
+    h->zppt = z_inplace_ppt(h->pz, h->KT);
+    if (!h->zppt) {
+        h->KT = (int)std::max<size_t>(1, std::min<size_t>(16, LDS_MAX / (2 * (size_t)h->zld * sizeof(double2))));
+        h->KT = std::min(h->KT, gd.Nxh);
+    }
     h->xld = M + 1;
     {
         // odd Nxp: two FFT lines plus the form's physical lines per (zp, y) line
@@ -282,6 +291,9 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         xa.gd = gd;
         xa.ld = h->xld;
         xa.mode = h->form == CF_FORM_CONVECTIVE ? X_CONVECTIVE : h->form == CF_FORM_DIVERGENCE ? X_DIVERGENCE : X_ROTATIONAL;
+        xa.plan = h->px;
+        h->xppt = x_inplace_ppt(xa);
+        xa.ppt = h->xppt;
         h->xwaves = x_max_waves(xa);
         if (h->xwaves < 1) return bail(fail(CF_ERR_UNSUPPORTED, "streamwise line exceeds LDS"));
     }
@@ -352,6 +364,7 @@ static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* co
     z.KT = h->KT;
     z.ld = h->zld;
     z.nfields = nf;
+    z.ppt = h->zppt;
     z.dz_mask = dz_mask;
     for (int i = 0; i < nf; ++i) {
         z.in[i] = in[i];
@@ -370,6 +383,7 @@ static XArgs base_xargs(cf_problem* h) {
     x.LPB = h->LPB;
     x.ld = h->xld;
     x.waves = h->xwaves;
+    x.ppt = h->xppt;
     return x;
 }
 

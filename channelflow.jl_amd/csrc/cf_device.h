@@ -660,6 +660,83 @@ __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPla
     }
     return src;
 }
+// In-place variant: one buffer.  Each pass first gathers every butterfly of
+// this thread into registers, synchronises, then scatters the outputs.  PPT
+// bounds the points per thread (nlines * n <= PPT * nthr), so a thread holds
+// at most ceil(PPT / R) butterflies of radix R.  The plan must contain only
+// radices 2, 3, 4, 5 and 8 (fft_inplace_fits).
+template <int R, int PPT, bool WAVE>
+__device__ __forceinline__ void stockham_pass_inplace(double2* buf, int n, int nlines, int ld, int Ns,
+                                                      unsigned long long mul_m, unsigned long long mul_ns,
+                                                      const double2* tw, int sign, int tid, int nthr) {
+    constexpr int NB = (PPT + R - 1) / R;
+    const int m = n / R;
+    const int twstep = n / (Ns * R);
+    const int total = nlines * m;
+    double2 v[NB][R];
+    int dst[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int idx = tid + b * nthr;
+        dst[b] = -1;
+        if (idx < total) {
+            int line = (int)fast_div((unsigned)idx, mul_m);
+            int j = idx - line * m;
+            int jq = (int)fast_div((unsigned)j, mul_ns);
+            int jm = j - jq * Ns;
+            const double2* s = buf + (size_t)line * ld;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double2 x = s[j + r * m];
+                if (r > 0) {
+                    double2 w = tw[r * jm * twstep];
+                    if (sign > 0) w = cconj(w);
+                    x = cmul(x, w);
+                }
+                v[b][r] = x;
+            }
+            dft_small<R>(v[b], sign);
+            dst[b] = line * ld + jq * Ns * R + jm;
+        }
+    }
+    if constexpr (WAVE) wave_sync();
+    else __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+        if (dst[b] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[dst[b] + r * Ns] = v[b][r];
+        }
+    if constexpr (WAVE) wave_sync();
+    else __syncthreads();
+}
+
+template <int PPT, bool WAVE>
+__device__ __forceinline__ void lds_fft_inplace(double2* buf, const FFTPlan& plan, const double2* tw, int nlines, int ld,
+                                                int sign, int tid, int nthr) {
+    int Ns = 1;
+    for (int f = 0; f < plan.nf; ++f) {
+        const int R = plan.radix[f];
+        const unsigned long long mm = plan.mul_m[f], mn = plan.mul_ns[f];
+        switch (R) {
+            case 2: stockham_pass_inplace<2, PPT, WAVE>(buf, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            case 3: stockham_pass_inplace<3, PPT, WAVE>(buf, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            case 4: stockham_pass_inplace<4, PPT, WAVE>(buf, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            case 5: stockham_pass_inplace<5, PPT, WAVE>(buf, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+            default: stockham_pass_inplace<8, PPT, WAVE>(buf, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
+        }
+        Ns *= R;
+    }
+}
+
+__host__ __device__ inline bool fft_inplace_fits(const FFTPlan& plan, int nlines, int PPT, int nthr) {
+    for (int f = 0; f < plan.nf; ++f) {
+        const int R = plan.radix[f];
+        if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return false;
+    }
+    return (long long)nlines * plan.n <= (long long)PPT * nthr;
+}
+
 __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
                                             int nthr) {
     return lds_fft<false>(a, b, plan, plan.tw, nlines, ld, sign, tid, nthr);

@@ -58,6 +58,7 @@ struct ZArgs {
     GridDims gd;
     FFTPlan plan;  // length Nzp
     int KT, ld, nfields;
+    int ppt;           // > 0: in-place FFT in one LDS buffer with this many points per thread; 0: two buffers
     unsigned dz_mask;  // bit f: inverse pass multiplies field f by i*beta*l (z derivative)
     const double2* in[9];
     double2* out[9];
@@ -71,6 +72,7 @@ struct XArgs {
     const double2* twx;   // exp(-2 pi i t / Nxp), t < Nxp
     int LPB, ld, mode, nfields;
     int waves;             // even Nxp: wavefronts (lines) per block, 1..4
+    int ppt;               // even Nxp: in-place FFT points per lane (one LDS buffer), 0: two buffers
     const double2* in[9];  // (k, zp, y) complex, Nxh per line
     double2* out[6];
     const double* rin[3];  // (x, zp, y) real
@@ -82,9 +84,11 @@ int y_group_lanes(int Ny);
 size_t y_lds_bytes(int Ny, int ld);
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st);
 size_t z_lds_bytes(const ZArgs& a);
+int z_inplace_ppt(const FFTPlan& plan, int KT);  // 0: the plan needs two buffers
 hipError_t launch_xpass(const XArgs& a, hipStream_t st);
 size_t x_lds_bytes(const XArgs& a);
 int x_phys_lines(int mode);  // physical lines per (zp, y) line of the odd-Nxp x pass
 int x_max_waves(const XArgs& a);  // even Nxp: wavefronts per block that fit the LDS (0: none)
+int x_inplace_ppt(const XArgs& a);  // even Nxp: in-place points per lane for the mode (0: two buffers)
 
 }  // namespace cf

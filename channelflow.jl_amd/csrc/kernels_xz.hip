@@ -31,7 +31,7 @@ __device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned nblocks) {
 
 // z inverse (dir=+1): (k,l,y) -> (k,zp,y); z forward (dir=-1): (k,zp,y) -> (k,l,y)
 // 1-D grid of ntiles * Ny * nfields blocks, k tile fastest in logical order.
-template <int DIR>
+template <int DIR, int PPT>
 __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
     extern __shared__ double2 smem[];
     const GridDims& gd = a.gd;
@@ -74,7 +74,9 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
         }
     }
     __syncthreads();
-    double2* res = lds_fft(bufA, bufB, a.plan, KT, ld, DIR, tid, ZNT);
+    double2* res = bufA;
+    if constexpr (PPT > 0) lds_fft_inplace<PPT, false>(bufA, a.plan, a.plan.tw, KT, ld, DIR, tid, ZNT);
+    else res = lds_fft(bufA, bufB, a.plan, KT, ld, DIR, tid, ZNT);
     if (DIR > 0) {
         double2* out = a.out[f] + (size_t)Nxh * Nzp * y;
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
@@ -200,15 +202,28 @@ struct XCtx {
 //   divergence in V (3)                    out V_i V_j, i <= j (6)    (nonlinear.jl:157-160)
 // Only wave-level ordering is needed after the block loads the twiddles.
 // A block has a.waves wavefronts (64 * a.waves threads).
-__host__ __device__ constexpr int x_region_slots(int mode, int ld) { return (mode == X_CONVECTIVE ? 24 : 12) * ld; }
 __host__ __device__ constexpr int x_fft_lines_in(int mode) { return mode == X_CONVECTIVE ? 12 : (mode == X_DIVERGENCE ? 3 : 6); }
 __host__ __device__ constexpr int x_fields_in(int mode) { return mode == X_CONVECTIVE ? 9 : (mode == X_DIVERGENCE ? 3 : 6); }
 __host__ __device__ constexpr int x_fields_out(int mode) { return mode == X_DIVERGENCE ? 6 : 3; }
+__host__ __device__ constexpr int x_buffer_lines(int mode) {
+    return x_fft_lines_in(mode) > x_fields_out(mode) ? x_fft_lines_in(mode) : x_fields_out(mode);
+}
+// LDS slots of one wave's region: one buffer (in place) or two
+__host__ __device__ constexpr int x_region_slots(int mode, int ppt, int ld) { return (ppt > 0 ? 1 : 2) * x_buffer_lines(mode) * ld; }
 
-template <int MODE>
+// retained coefficient X_k of field f on line gl for the half-length inverse
+// transform (padding, Nyquist plane and Im of DC / of k = M removed)
+__device__ __forceinline__ double2 x_coef(const XArgs& a, int f, long long gl, int k, int M) {
+    double2 v = czero();
+    if (k < a.gd.Nxh && !((a.gd.Nx % 2 == 0) && k == a.gd.Nx / 2)) v = a.in[f][(size_t)gl * a.gd.Nxh + k];
+    if (k == 0 || k == M) v.y = 0.0;
+    return v;
+}
+
+template <int MODE, int PPT>
 __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
     constexpr int NIN = x_fields_in(MODE), NFFT = x_fft_lines_in(MODE), NOUT = x_fields_out(MODE);
-    constexpr int NB = NFFT > NOUT ? NFFT : NOUT;  // lines per buffer (x_region_slots = 2 NB ld)
+    constexpr int NB = x_buffer_lines(MODE);
     const int Nxp = a.gd.Nxp, Nxh = a.gd.Nxh, M = Nxp / 2, ld = a.ld;
     const long long nlines = (long long)a.gd.Nzp * a.gd.Ny;
     const bool xeven = (a.gd.Nx % 2 == 0);
@@ -221,7 +236,70 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long gl = (long long)blockIdx.x * a.waves + wave;
     if (gl >= nlines) return;  // whole wavefront leaves; no block barrier follows
-    double2* bufA = smem + 2 * M + 1 + (size_t)wave * x_region_slots(MODE, ld);
+    double2* bufA = smem + 2 * M + 1 + (size_t)wave * x_region_slots(MODE, PPT, ld);
+    if constexpr (PPT > 0) {
+        // pack straight from HBM into the single buffer; FFT line q reads
+        // field src(q), times i alpha k for the convective x derivatives
+        for (int idx = lane; idx < NFFT * M; idx += 64) {
+            int q = idx / M, k = idx - q * M;
+            int f = q;
+            bool dx = false;
+            if constexpr (MODE == X_CONVECTIVE) {
+                dx = (q >= 3 && q < 6);
+                f = q < 3 ? q : q - 3;
+            }
+            double2 Xk = x_coef(a, f, gl, k, M);
+            double2 Xm = x_coef(a, f, gl, M - k, M);
+            if (dx) {
+                Xk = cmuli(cscale(Xk, a.gd.alpha * k));
+                Xm = cmuli(cscale(Xm, a.gd.alpha * (M - k)));
+            }
+            double2 Xc = cconj(Xm);
+            double2 w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
+            bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
+        }
+        wave_sync();
+        lds_fft_inplace<PPT, true>(bufA, a.plan, tw, NFFT, ld, +1, lane, 64);
+        // products at x = 2j (real parts) and 2j+1 (imaginary parts), in place:
+        // each lane reads all lines at its j before writing them
+        for (int j = lane; j < M; j += 64) {
+            if constexpr (MODE == X_ROTATIONAL) {
+                double2 u0 = bufA[0 * ld + j], u1 = bufA[1 * ld + j], u2 = bufA[2 * ld + j];
+                double2 w0 = bufA[3 * ld + j], w1 = bufA[4 * ld + j], w2 = bufA[5 * ld + j];
+                bufA[0 * ld + j] = cmk(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+                bufA[1 * ld + j] = cmk(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+                bufA[2 * ld + j] = cmk(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+            } else if constexpr (MODE == X_CONVECTIVE) {
+                double2 u = bufA[0 * ld + j], v = bufA[1 * ld + j], w = bufA[2 * ld + j];
+                double2 n[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    double2 gx = bufA[(3 + i) * ld + j], gy = bufA[(6 + i) * ld + j], gz = bufA[(9 + i) * ld + j];
+                    n[i] = cmk(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) bufA[i * ld + j] = n[i];
+            } else {
+                double2 u = bufA[0 * ld + j], v = bufA[1 * ld + j], w = bufA[2 * ld + j];
+                bufA[0 * ld + j] = cmk(u.x * u.x, u.y * u.y);
+                bufA[1 * ld + j] = cmk(u.x * v.x, u.y * v.y);
+                bufA[2 * ld + j] = cmk(u.x * w.x, u.y * w.y);
+                bufA[3 * ld + j] = cmk(v.x * v.x, v.y * v.y);
+                bufA[4 * ld + j] = cmk(v.x * w.x, v.y * w.y);
+                bufA[5 * ld + j] = cmk(w.x * w.x, w.y * w.y);
+            }
+        }
+        wave_sync();
+        lds_fft_inplace<PPT, true>(bufA, a.plan, tw, NOUT, ld, -1, lane, 64);
+        for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
+            int c = idx / Nxh, k = idx - c * Nxh;
+            double2 Zk = bufA[c * ld + k];
+            double2 Zc = cconj(bufA[c * ld + ((M - k) % M)]);
+            double2 w = twx[k];  // exp(-2 pi i k / Nxp)
+            a.out[c][(size_t)gl * Nxh + k] = csub(cscale(cadd(Zk, Zc), 0.5), cscale(cmuli(cmul(w, csub(Zk, Zc))), 0.5));
+        }
+        return;
+    }
     double2* bufB = bufA + NB * ld;
     const int nX = M + 1;
     // stage X_k, k = 0..M (padding, Nyquist plane and Im of DC/Nyquist removed)
@@ -292,13 +370,15 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
     }
 }
 
+// one kernel per (form, in-place width) so each gets its own register budget
+template <int MODE, int PPT>
+__global__ void __launch_bounds__(XNT) xwave_kernel(XArgs a) {
+    extern __shared__ double2 smem[];
+    xpass_wave<MODE, PPT>(a, smem);
+}
+
 __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
     extern __shared__ double2 smem[];
-    if (a.gd.Nxp % 2 == 0) {
-        if (a.mode == X_ROTATIONAL) { xpass_wave<X_ROTATIONAL>(a, smem); return; }
-        if (a.mode == X_CONVECTIVE) { xpass_wave<X_CONVECTIVE>(a, smem); return; }
-        if (a.mode == X_DIVERGENCE) { xpass_wave<X_DIVERGENCE>(a, smem); return; }
-    }
     XCtx c(a, smem);
     double* phys = reinterpret_cast<double*>(smem + 2 * a.LPB * a.ld);
     const int Nxp = c.Nxp, LPB = c.LPB;
@@ -365,13 +445,30 @@ __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
     }
 }
 
-size_t z_lds_bytes(const ZArgs& a) { return 2 * (size_t)a.KT * a.ld * sizeof(double2); }
+size_t z_lds_bytes(const ZArgs& a) { return (a.ppt > 0 ? 1 : 2) * (size_t)a.KT * a.ld * sizeof(double2); }
+
+// in-place instantiations (points per thread); the host picks the smallest that fits
+int z_inplace_ppt(const FFTPlan& plan, int KT) {
+    for (int p : {8, 16})
+        if (fft_inplace_fits(plan, KT, p, ZNT)) return p;
+    return 0;
+}
+
+template <int DIR>
+static void launch_z(const ZArgs& a, dim3 grid, size_t lds, hipStream_t st) {
+    switch (a.ppt) {
+        case 8: hipLaunchKernelGGL((zpass_kernel<DIR, 8>), grid, dim3(ZNT), lds, st, a); break;
+        case 16: hipLaunchKernelGGL((zpass_kernel<DIR, 16>), grid, dim3(ZNT), lds, st, a); break;
+        default: hipLaunchKernelGGL((zpass_kernel<DIR, 0>), grid, dim3(ZNT), lds, st, a); break;
+    }
+}
 
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st) {
+    if (a.ppt != 0 && a.ppt != 8 && a.ppt != 16) return hipErrorInvalidValue;
     dim3 grid((unsigned)((a.gd.Nxh + a.KT - 1) / a.KT) * (unsigned)a.gd.Ny * (unsigned)a.nfields);
     size_t lds = z_lds_bytes(a);
-    if (dir > 0) hipLaunchKernelGGL(zpass_kernel<+1>, grid, dim3(ZNT), lds, st, a);
-    else hipLaunchKernelGGL(zpass_kernel<-1>, grid, dim3(ZNT), lds, st, a);
+    if (dir > 0) launch_z<+1>(a, grid, lds, st);
+    else launch_z<-1>(a, grid, lds, st);
     return hipGetLastError();
 }
 
@@ -384,7 +481,14 @@ int x_phys_lines(int mode) {
 }
 
 static size_t x_wave_lds(const XArgs& a, int waves) {
-    return ((size_t)a.gd.Nxp + 1 + (size_t)waves * x_region_slots(a.mode, a.ld)) * sizeof(double2);
+    return ((size_t)a.gd.Nxp + 1 + (size_t)waves * x_region_slots(a.mode, a.ppt, a.ld)) * sizeof(double2);
+}
+
+int x_inplace_ppt(const XArgs& a) {
+    if (a.gd.Nxp % 2 != 0) return 0;
+    for (int p : {9, 18})
+        if (fft_inplace_fits(a.plan, x_buffer_lines(a.mode), p, 64)) return p;
+    return 0;
 }
 
 int x_max_waves(const XArgs& a) {
@@ -403,12 +507,35 @@ hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
     long long nlines = (long long)a.gd.Nzp * a.gd.Ny;
     const bool wave = x_wave_path(a);
     if (wave && (a.waves < 1 || a.waves > XNT / 64)) return hipErrorInvalidValue;
+    if (wave && a.ppt != 0 && a.ppt != 9 && a.ppt != 18) return hipErrorInvalidValue;
     const int per_block = wave ? a.waves : a.LPB;
     dim3 grid((unsigned)((nlines + per_block - 1) / per_block));
     size_t lds = x_lds_bytes(a);
-    if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)xpass_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(xpass_kernel, grid, dim3(wave ? 64 * a.waves : XNT), lds, st, a);
-    return hipGetLastError();
+    if (!wave) {
+        if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)xpass_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(xpass_kernel, grid, dim3(XNT), lds, st, a);
+        return hipGetLastError();
+    }
+    const dim3 block(64 * a.waves);
+#define CF_XWAVE(MODE, PPT)                                                                                      \
+    {                                                                                                            \
+        if (lds > 64 * 1024)                                                                                     \
+            (void)hipFuncSetAttribute((const void*)xwave_kernel<MODE, PPT>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                      (int)lds);                                                                 \
+        hipLaunchKernelGGL((xwave_kernel<MODE, PPT>), grid, block, lds, st, a);                                  \
+        return hipGetLastError();                                                                                \
+    }
+#define CF_XMODE(MODE)                        \
+    switch (a.ppt) {                          \
+        case 9: CF_XWAVE(MODE, 9)             \
+        case 18: CF_XWAVE(MODE, 18)           \
+        default: CF_XWAVE(MODE, 0)            \
+    }
+    if (a.mode == X_ROTATIONAL) CF_XMODE(X_ROTATIONAL)
+    if (a.mode == X_CONVECTIVE) CF_XMODE(X_CONVECTIVE)
+    CF_XMODE(X_DIVERGENCE)
+#undef CF_XMODE
+#undef CF_XWAVE
 }
 
 }  // namespace cf
