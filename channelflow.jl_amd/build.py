@@ -23,6 +23,7 @@ FLAGS = [
     # no packed-FP32 code generation (v_pk_*_f32) anywhere in this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-Werror=return-type", "-Werror=uninitialized",
 ]
 
 

@@ -88,11 +88,11 @@ struct cf_problem {
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) == hipSuccess && prev != dev) hipSetDevice(dev);
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
         else prev = -1;
     }
     ~DeviceGuard() {
-        if (prev >= 0) hipSetDevice(prev);
+        if (prev >= 0) (void)hipSetDevice(prev);
     }
 };
 
@@ -167,7 +167,7 @@ static int get_bank(cf_problem* h, double nu, double lambda, DevBank** out) {
 }
 
 static void free_all(cf_problem* h) {
-    for (void* p : h->allocs) hipFree(p);
+    for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
 }
 
@@ -311,7 +311,7 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
 extern "C" int cf_problem_destroy(cf_problem* h) {
     if (!h) return CF_OK;
     DeviceGuard dg(h->device);
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     free_all(h);
     delete h;
     return CF_OK;
@@ -366,6 +366,7 @@ static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* co
     z.nfields = nf;
     z.ppt = h->zppt;
     z.dz_mask = dz_mask;
+    z.mul_kt = recip_mul((unsigned)h->KT);
     for (int i = 0; i < nf; ++i) {
         z.in[i] = in[i];
         z.out[i] = out[i];
@@ -384,6 +385,9 @@ static XArgs base_xargs(cf_problem* h) {
     x.ld = h->xld;
     x.waves = h->xwaves;
     x.ppt = h->xppt;
+    x.mul_m = recip_mul((unsigned)std::max(1, h->gd.Nxp / 2));
+    x.mul_nxh = recip_mul((unsigned)h->gd.Nxh);
+    x.mul_nx = recip_mul((unsigned)(std::max(1, h->gd.Nxp / 2) + 1));
     return x;
 }
 

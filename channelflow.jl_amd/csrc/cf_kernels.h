@@ -60,6 +60,7 @@ struct ZArgs {
     int KT, ld, nfields;
     int ppt;           // > 0: in-place FFT in one LDS buffer with this many points per thread; 0: two buffers
     unsigned dz_mask;  // bit f: inverse pass multiplies field f by i*beta*l (z derivative)
+    unsigned long long mul_kt;  // recip_mul(KT): index splits without integer division
     const double2* in[9];
     double2* out[9];
 };
@@ -73,6 +74,7 @@ struct XArgs {
     int LPB, ld, mode, nfields;
     int waves;             // even Nxp: wavefronts (lines) per block, 1..4
     int ppt;               // even Nxp: in-place FFT points per lane (one LDS buffer), 0: two buffers
+    unsigned long long mul_m, mul_nxh, mul_nx;  // recip_mul of Nxp/2, Nxh and Nxp/2+1 (index splits)
     const double2* in[9];  // (k, zp, y) complex, Nxh per line
     double2* out[6];
     const double* rin[3];  // (x, zp, y) real

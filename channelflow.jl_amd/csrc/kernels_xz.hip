@@ -47,11 +47,13 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
     const int Nz = gd.Nz, Nzp = gd.Nzp, Nxh = gd.Nxh;
     const int npos = (Nz >> 1) + 1, nneg = Nz - npos;
     const bool xeven = (gd.Nx % 2 == 0), zeven = (Nz % 2 == 0);
+    if constexpr (PPT > 0)  // in-place mode: the second buffer holds the twiddle table
+        for (int i = tid; i < Nzp; i += ZNT) bufB[i] = a.plan.tw[i];
     if (DIR > 0) {
         const double2* in = a.in[f] + (size_t)Nxh * Nz * y;
         const bool dz = (a.dz_mask >> f) & 1u;  // emit d/dz: multiply by i beta l
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
-            int z = idx / KT, kk = idx - z * KT;
+            int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
             int k = k0 + kk;
             double2 v = czero();
             if (k < Nxh && !(xeven && k == gd.Nx / 2)) {
@@ -68,26 +70,26 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
     } else {
         const double2* in = a.in[f] + (size_t)Nxh * Nzp * y;
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
-            int z = idx / KT, kk = idx - z * KT;
+            int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
             int k = k0 + kk;
             bufA[kk * ld + z] = (k < Nxh) ? in[k + (size_t)Nxh * z] : czero();
         }
     }
     __syncthreads();
     double2* res = bufA;
-    if constexpr (PPT > 0) lds_fft_inplace<PPT, false>(bufA, a.plan, a.plan.tw, KT, ld, DIR, tid, ZNT);
+    if constexpr (PPT > 0) lds_fft_inplace<PPT, false>(bufA, a.plan, bufB, KT, ld, DIR, tid, ZNT);  // bufB: twiddles
     else res = lds_fft(bufA, bufB, a.plan, KT, ld, DIR, tid, ZNT);
     if (DIR > 0) {
         double2* out = a.out[f] + (size_t)Nxh * Nzp * y;
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
-            int z = idx / KT, kk = idx - z * KT;
+            int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
             int k = k0 + kk;
             if (k < Nxh) out[k + (size_t)Nxh * z] = res[kk * ld + z];
         }
     } else {
         double2* out = a.out[f] + (size_t)Nxh * Nz * y;
         for (int idx = tid; idx < KT * Nz; idx += ZNT) {
-            int l = idx / KT, kk = idx - l * KT;
+            int l = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - l * KT;
             int k = k0 + kk;
             int z = (l < npos) ? l : Nzp - nneg + (l - npos);
             if (k < Nxh) out[k + (size_t)Nxh * l] = res[kk * ld + z];
@@ -241,7 +243,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
         // pack straight from HBM into the single buffer; FFT line q reads
         // field src(q), times i alpha k for the convective x derivatives
         for (int idx = lane; idx < NFFT * M; idx += 64) {
-            int q = idx / M, k = idx - q * M;
+            int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
             int f = q;
             bool dx = false;
             if constexpr (MODE == X_CONVECTIVE) {
@@ -292,7 +294,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
         wave_sync();
         lds_fft_inplace<PPT, true>(bufA, a.plan, tw, NOUT, ld, -1, lane, 64);
         for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
-            int c = idx / Nxh, k = idx - c * Nxh;
+            int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
             double2 Zk = bufA[c * ld + k];
             double2 Zc = cconj(bufA[c * ld + ((M - k) % M)]);
             double2 w = twx[k];  // exp(-2 pi i k / Nxp)
@@ -304,7 +306,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
     const int nX = M + 1;
     // stage X_k, k = 0..M (padding, Nyquist plane and Im of DC/Nyquist removed)
     for (int idx = lane; idx < NIN * nX; idx += 64) {
-        int f = idx / nX, k = idx - f * nX;
+        int f = (int)fast_div((unsigned)idx, a.mul_nx), k = idx - f * nX;
         double2 v = czero();
         if (k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = a.in[f][(size_t)gl * Nxh + k];
         if (k == 0 || k == M) v.y = 0.0;
@@ -314,7 +316,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
     // pack for the half-length inverse transform; FFT line q reads staged
     // field src(q), multiplied by i alpha k for the convective x derivatives
     for (int idx = lane; idx < NFFT * M; idx += 64) {
-        int q = idx / M, k = idx - q * M;
+        int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
         int f = q;
         bool dx = false;
         if constexpr (MODE == X_CONVECTIVE) {
@@ -362,7 +364,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
     wave_sync();
     double2* r2 = lds_fft<true>(nxt, res, a.plan, tw, NOUT, ld, -1, lane, 64);
     for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
-        int c = idx / Nxh, k = idx - c * Nxh;
+        int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
         double2 Zk = r2[c * ld + k];
         double2 Zc = cconj(r2[c * ld + ((M - k) % M)]);
         double2 w = twx[k];  // exp(-2 pi i k / Nxp)
@@ -445,7 +447,10 @@ __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
     }
 }
 
-size_t z_lds_bytes(const ZArgs& a) { return (a.ppt > 0 ? 1 : 2) * (size_t)a.KT * a.ld * sizeof(double2); }
+size_t z_lds_bytes(const ZArgs& a) {
+    // in place: one buffer plus the twiddle table (Nzp entries); else two buffers
+    return (a.ppt > 0 ? (size_t)a.KT * a.ld + a.gd.Nzp : 2 * (size_t)a.KT * a.ld) * sizeof(double2);
+}
 
 // in-place instantiations (points per thread); the host picks the smallest that fits
 int z_inplace_ppt(const FFTPlan& plan, int KT) {
