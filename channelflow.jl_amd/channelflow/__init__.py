@@ -23,7 +23,8 @@ from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 
 __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State", "zero_state", "random_state",
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
-           "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm"]
+           "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
+           "power_input"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -393,26 +394,49 @@ def step_timed(scheme, U, P, t, pressuregradient=None, bulkvelocity=None, stream
 # ---------------------------------------------------------------------------
 # diagnostics (postprocessing.jl:21-95), torch reductions on the device
 # ---------------------------------------------------------------------------
-def _mass_matrix(Ny, device):
-    torch = _torch()
-    n = np.arange(Ny)
-    mom = lambda k: np.where(k % 2 == 0, 1.0 / (1.0 - k.astype(float) ** 2 + (k == 1)), 0.0)
-    M = (mom(n[:, None] + n[None, :]) + mom(np.abs(n[:, None] - n[None, :]))) / 2
-    return torch.as_tensor(M, device=device, dtype=torch.complex128)
+def _scalar():
+    return C.c_double(0.0)
 
 
-def spectral_dot(grid, U, V):
-    torch = _torch()
-    M = _mass_matrix(grid.Ny, U.device)
-    w = torch.full((grid.Nz, grid.Nxh), 2.0, dtype=torch.float64, device=U.device)
-    w[:, 0] = 1.0
-    if grid.Nx % 2 == 0:
-        w[:, grid.Nx // 2] = 0.0
-    if grid.Nz % 2 == 0:
-        w[grid.Nz // 2, :] = 0.0
-    MV = torch.einsum("mn,nzx->mzx", M, V)
-    return float((torch.conj(U) * MV).real.sum(0).mul(w).sum())
+def dot(problem, U, V):
+    """dot(U::VectorField, V::VectorField) for spectral fields (postprocessing.jl:30-60):
+    exact volume average <U . V> (device kernel, Chebyshev mass matrix)."""
+    g = problem.grid
+    _check_vec(g, U)
+    _check_vec(g, V)
+    out = _scalar()
+    _lib.check(_lib.load().cf_dot(problem.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(V[0]), _ptr(V[1]),
+                                  _ptr(V[2]), C.byref(out), _stream()))
+    return out.value
 
 
-def kinetic_energy(grid, U):
-    return sum(spectral_dot(grid, U[i], U[i]) for i in range(3)) / 2
+def norm(problem, U):
+    """norm(U::VectorField): sqrt(<|U|^2>) (postprocessing.jl:70-71)."""
+    return math.sqrt(dot(problem, U, U))
+
+
+def dissipation_rate(problem, U, nu=None):
+    """dissipation_rate(U, nu) = nu <|curl U|^2> (postprocessing.jl:87-91)."""
+    g = problem.grid
+    _check_vec(g, U)
+    out = _scalar()
+    nu = problem.scheme.nu if nu is None else float(nu)
+    _lib.check(_lib.load().cf_dissipation(problem.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), nu, C.byref(out),
+                                          _stream()))
+    return out.value
+
+
+def power_input(problem, U, nu=None, pressuregradient=(0.0, 0.0)):
+    """power_input(U, nu; pressuregradient) (postprocessing.jl:113-128)."""
+    g = problem.grid
+    _check_vec(g, U)
+    out = _scalar()
+    nu = problem.scheme.nu if nu is None else float(nu)
+    grad = (C.c_double * 2)(*[float(x) for x in pressuregradient])
+    _lib.check(_lib.load().cf_power_input(problem.scheme.h, _ptr(U[0]), _ptr(U[2]), nu, grad, C.byref(out), _stream()))
+    return out.value
+
+
+def kinetic_energy(problem, U):
+    """kinetic_energy(U) = <|U|^2>/2 (postprocessing.jl:77)."""
+    return dot(problem, U, U) / 2

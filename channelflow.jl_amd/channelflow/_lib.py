@@ -36,7 +36,7 @@ _lib = None
 SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_problem_sizes", "cf_step", "cf_advance",
            "cf_step_timed",
            "cf_last_gradient", "cf_forward", "cf_inverse", "cf_nonlinear", "cf_stokes_solve", "cf_project",
-           "cf_pressure"]
+           "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input"]
 
 
 def load(path=LIB_PATH):
@@ -62,6 +62,9 @@ def load(path=LIB_PATH):
     L.cf_stokes_solve.argtypes = [vp, d, d, vp, vp, vp, vp, vp, vp, vp, i, dp, dp, vp]
     L.cf_project.argtypes = [vp, vp, vp, vp, vp]
     L.cf_pressure.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.cf_dot.argtypes = [vp, vp, vp, vp, vp, vp, vp, dp, vp]
+    L.cf_dissipation.argtypes = [vp, vp, vp, vp, d, dp, vp]
+    L.cf_power_input.argtypes = [vp, vp, vp, d, dp, dp, vp]
     for name in SYMBOLS:
         getattr(L, name).restype = C.c_int if name != "cf_last_error" else C.c_char_p
     _lib = L

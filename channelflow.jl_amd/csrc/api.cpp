@@ -79,6 +79,7 @@ struct cf_problem {
     double2* Q = nullptr;     // 3 S
     double2* R = nullptr;     // 3 S
     double* d_grad = nullptr;
+    double* d_scalar = nullptr;  // reduction result of the diagnostics
     int KT = 1, zld = 0, LPB = 1, xld = 0, yld = 0;
     int xwaves = 4;  // even Nxp: wavefronts per block of the nonlinear x pass
     int zppt = 0;    // z pass in-place points per thread (0: two buffers)
@@ -303,6 +304,7 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
     if ((rc = dalloc(h, &h->Q, 3 * (size_t)gd.S))) return bail(rc);
     if ((rc = dalloc(h, &h->R, 3 * (size_t)gd.S))) return bail(rc);
     if ((rc = dalloc(h, &h->d_grad, 2))) return bail(rc);
+    if ((rc = dalloc(h, &h->d_scalar, 2))) return bail(rc);
     if (hipMemset(h->d_grad, 0, 2 * sizeof(double)) != hipSuccess) return bail(fail(CF_ERR_DEVICE, "hipMemset"));
     *out = h;
     return CF_OK;
@@ -753,5 +755,96 @@ extern "C" int cf_pressure(cf_problem* h, const double* U0, const double* U1, co
     a.U[1] = U[1];
     a.P = (double2*)P;
     HIPTRY(launch_ykernel(Y_PRESSURE, a, st));
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics (postprocessing.jl:30-128)
+// ---------------------------------------------------------------------------
+static double host_bulkmean_real(const double2* m, int Ny) {
+    std::vector<double> re((size_t)Ny);
+    for (int n = 0; n < Ny; ++n) re[n] = m[n].x;
+    return host_bulkmean(re.data(), Ny);
+}
+
+static int enqueue_dot(cf_problem* h, int which, double2* const U[3], const double2* const V[3], double scale, double* out,
+                       hipStream_t st) {
+    YArgs a = base_yargs(h);
+    for (int i = 0; i < 3; ++i) {
+        a.U[i] = U[i];
+        a.Rin[i] = V ? V[i] : nullptr;
+    }
+    double* part = reinterpret_cast<double*>(h->R);  // >= one double per block
+    a.partial = part;
+    HIPTRY(launch_ykernel(which, a, st));
+    HIPTRY(launch_reduce(part, (int)y_blocks(h->gd), h->d_scalar, st));
+    double r = 0.0;
+    HIPTRY(hipMemcpyAsync(&r, h->d_scalar, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    *out = scale * r;
+    return CF_OK;
+}
+
+extern "C" int cf_dot(cf_problem* h, const double* U0, const double* U1, const double* U2, const double* V0, const double* V1,
+                      const double* V2, double* out, void* stream) {
+    if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    const void* fu[3] = {U0, U1, U2};
+    const void* fv[3] = {V0, V1, V2};
+    if (int rc = check_fields(fu, 3)) return rc;
+    if (int rc = check_fields(fv, 3)) return rc;
+    DeviceGuard dg(h->device);
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    const double2* V[3] = {(const double2*)V0, (const double2*)V1, (const double2*)V2};
+    return enqueue_dot(h, Y_DOT, U, V, 1.0, out, (hipStream_t)stream);
+}
+
+extern "C" int cf_dissipation(cf_problem* h, const double* U0, const double* U1, const double* U2, double nu, double* out,
+                              void* stream) {
+    if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    const void* fu[3] = {U0, U1, U2};
+    if (int rc = check_fields(fu, 3)) return rc;
+    if (!std::isfinite(nu)) return fail(CF_ERR_ARGUMENT, "nu must be finite");
+    DeviceGuard dg(h->device);
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    return enqueue_dot(h, Y_DISSIPATION, U, nullptr, nu, out, (hipStream_t)stream);
+}
+
+extern "C" int cf_power_input(cf_problem* h, const double* U0, const double* U2, double nu, const double grad[2],
+                              double* out, void* stream) {
+    if (!h || !U0 || !U2 || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    if (!std::isfinite(nu)) return fail(CF_ERR_ARGUMENT, "nu must be finite");
+    const double g[2] = {grad ? grad[0] : 0.0, grad ? grad[1] : 0.0};
+    if (!(std::isfinite(g[0]) && std::isfinite(g[1]))) return fail(CF_ERR_ARGUMENT, "pressure gradient must be finite");
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    const int Ny = h->gd.Ny;
+    // zero Fourier mode column: element (0, 0, n) at n * B
+    std::vector<double2> col(2 * (size_t)Ny);
+    HIPTRY(hipMemcpy2DAsync(col.data(), sizeof(double2), U0, (size_t)h->gd.B * sizeof(double2), sizeof(double2), Ny,
+                            hipMemcpyDeviceToHost, st));
+    HIPTRY(hipMemcpy2DAsync(col.data() + Ny, sizeof(double2), U2, (size_t)h->gd.B * sizeof(double2), sizeof(double2), Ny,
+                            hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    double input = 0.0;
+    for (int c = 0; c < 2; ++c) {
+        const double2* m = col.data() + (size_t)c * Ny;
+        // wall values and the one-sided derivatives of the mean profile
+        double up_r = 0, up_i = 0, lo_r = 0, lo_i = 0, dup_r = 0, dup_i = 0, dlo_r = 0, dlo_i = 0;
+        for (int n = 0; n < Ny; ++n) {
+            const double sgn = (n % 2 == 0) ? 1.0 : -1.0, n2 = (double)n * n;
+            up_r += m[n].x;
+            up_i += m[n].y;
+            lo_r += sgn * m[n].x;
+            lo_i += sgn * m[n].y;
+            dup_r += n2 * m[n].x;
+            dup_i += n2 * m[n].y;
+            dlo_r -= sgn * n2 * m[n].x;  // T_n'(-1) = (-1)^(n+1) n^2
+            dlo_i -= sgn * n2 * m[n].y;
+        }
+        // nu/2 Re(conj(u(+1)) u'(+1) - conj(u(-1)) u'(-1))
+        input += 0.5 * nu * ((up_r * dup_r + up_i * dup_i) - (lo_r * dlo_r + lo_i * dlo_i));
+        input -= g[c] * host_bulkmean_real(m, Ny);
+    }
+    *out = input;
     return CF_OK;
 }

@@ -15,7 +15,7 @@ struct Bank {
     double lambda, nu;
 };
 
-enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_INV, Y_DIVERGENCE };
+enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_INV, Y_DIVERGENCE, Y_DOT, Y_DISSIPATION };
 
 // nonlinear forms (src/nonlinear.jl:113-193); values match CF_FORM_* of the C ABI
 enum { FORM_ROTATING = 0, FORM_CONVECTIVE = 1, FORM_DIVERGENCE = 2 };
@@ -52,6 +52,7 @@ struct YArgs {
     double2* Rout[3];
     double* grad_out;  // 2 doubles (device) or null
     int nfields;
+    double* partial;   // Y_DOT / Y_DISSIPATION: one partial sum per block (device)
 };
 
 struct ZArgs {
@@ -82,6 +83,9 @@ struct XArgs {
 };
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st);
+unsigned y_blocks(const GridDims& gd);  // blocks of a y launch (partials of Y_DOT)
+// out[0] = sum of part[0..n) in a fixed order (one block)
+hipError_t launch_reduce(const double* part, int n, double* out, hipStream_t st);
 int y_group_lanes(int Ny);
 size_t y_lds_bytes(int Ny, int ld);
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st);

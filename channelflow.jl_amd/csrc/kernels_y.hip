@@ -685,6 +685,93 @@ __global__ void __launch_bounds__(YNT) ydivergence_kernel(YArgs a) {
     for (int c = 0; c < 3; ++c) tile.store(a.Rout[c], g, N[c]);
 }
 
+// Volume inner products (postprocessing.jl:30-60, 87-91).  Per mode,
+// Re(u^H M v) with the exact Chebyshev mass matrix
+// M_mn = (mu(m+n) + mu(|m-n|)) / 2, mu(k) = 1/(1-k^2) for even k (0 for odd),
+// weighted 1 for k = 0 and 2 for k > 0 (conjugate pairs), Nyquist planes
+// excluded.  Y_DOT: sum over the three components of U (U) and V (Rin).
+// Y_DISSIPATION: u = v = curl U (operators.jl:172-190); the caller scales by nu.
+// Each block writes its partial sum to partial[blockIdx.x].
+template <int L>
+__global__ void __launch_bounds__(YNT) ydot_kernel(YArgs a, int curl) {
+    extern __shared__ double2 smem[];
+    load_tables<L>(smem, a);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Group<L> g(tile.tab);
+    ModeInfo mi = mode_info(a.gd, tile.s);
+    const int Ny = a.gd.Ny;
+    double2* vline = tile.stash;  // this group's copy of v, indexed by degree
+    CVec W0, W1, W2;
+    if (curl) {
+        CVec U0 = tile.load(a.U[0], g), U1 = tile.load(a.U[1], g), U2 = tile.load(a.U[2], g);
+        CVec d2 = cheb_deriv(g, U2), d0 = cheb_deriv(g, U0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            W0.v[e] = csub(d2.v[e], cmuli(cscale(U1.v[e], mi.kz)));
+            W1.v[e] = cmuli(csub(cscale(U0.v[e], mi.kz), cscale(U2.v[e], mi.kx)));
+            W2.v[e] = csub(cmuli(cscale(U1.v[e], mi.kx)), d0.v[e]);
+        }
+    }
+    double acc = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        CVec u, v;
+        if (curl) {
+            u = (c == 0) ? W0 : (c == 1) ? W1 : W2;
+            v = u;
+        } else {
+            u = tile.load(a.U[c], g);
+            v = tile.load(a.Rin[c], g);
+        }
+        stash(vline, g, v);
+        wave_sync();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int m = 4 * g.t + e;
+            double2 s = czero();
+            if (m < Ny)
+                for (int n = m & 1; n < Ny; n += 2) {
+                    const double kp = m + n, km = m - n;
+                    const double Mmn = 0.5 * (1.0 / (1.0 - kp * kp) + 1.0 / (1.0 - km * km));
+                    s = cfma(Mmn, vline[n], s);
+                }
+            acc += u.v[e].x * s.x + u.v[e].y * s.y;  // Re(conj(u_m) (M v)_m)
+        }
+        wave_sync();
+    }
+    acc = g.sum(acc);
+    const bool use = mi.valid && !mi.nyq;
+    double* part = reinterpret_cast<double*>(smem);  // twiddle area is free now
+    __syncthreads();
+    constexpr int M = YNT / L;
+    const int mloc = threadIdx.x / L;
+    if (g.t == 0) part[mloc] = use ? (mi.ix == 0 ? 1.0 : 2.0) * acc : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int i = 0; i < M; ++i) b += part[i];
+        a.partial[blockIdx.x] = b;
+    }
+}
+
+__global__ void __launch_bounds__(256) reduce_kernel(const double* part, int n, double* out) {
+    __shared__ double s[256];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) v += part[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+hipError_t launch_reduce(const double* part, int n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, st, part, n, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -708,6 +795,7 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     allow_lds(ypressure_kernel<L>, lds);
     allow_lds(ydct_kernel<L>, lds);
     allow_lds(ydivergence_kernel<L>, lds);
+    allow_lds(ydot_kernel<L>, lds);
     switch (which) {
         case Y_STAGE: hipLaunchKernelGGL(ystage_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_PROLOGUE: hipLaunchKernelGGL(yprologue_kernel<L>, grid, dim3(YNT), lds, st, a); break;
@@ -717,9 +805,18 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
         case Y_DCT_FWD: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, -1); break;
         case Y_DCT_INV: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, +1); break;
         case Y_DIVERGENCE: hipLaunchKernelGGL(ydivergence_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_DOT: hipLaunchKernelGGL(ydot_kernel<L>, grid, dim3(YNT), lds, st, a, 0); break;
+        case Y_DISSIPATION: hipLaunchKernelGGL(ydot_kernel<L>, grid, dim3(YNT), lds, st, a, 1); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+unsigned y_blocks(const GridDims& gd) {
+    const int L = y_group_lanes(gd.Ny);
+    if (!L) return 0;
+    const long long M = YNT / L;
+    return (unsigned)((gd.B + M - 1) / M);
 }
 
 int y_group_lanes(int Ny) {

@@ -116,6 +116,23 @@ int cf_project(cf_problem* h, double* U0, double* U1, double* U2, void* stream);
  * divergence-free no-slip velocity (modified pressure for the rotational form) */
 int cf_pressure(cf_problem* h, const double* U0, const double* U1, const double* U2, double* P, void* stream);
 
+/* dot(U::VectorField, V::VectorField) for spectral fields (postprocessing.jl:30-60):
+ * the exact volume-averaged inner product <U . V> through the Chebyshev mass
+ * matrix, Nyquist planes excluded.  HOST output; synchronises the stream. */
+int cf_dot(cf_problem* h, const double* U0, const double* U1, const double* U2, const double* V0, const double* V1,
+           const double* V2, double* out, void* stream);
+
+/* dissipation_rate(U, nu) (postprocessing.jl:87-91): nu <|curl U|^2>.  HOST
+ * output; synchronises the stream. */
+int cf_dissipation(cf_problem* h, const double* U0, const double* U1, const double* U2, double nu, double* out,
+                   void* stream);
+
+/* power_input(U, nu; pressuregradient) (postprocessing.jl:113-128), from the
+ * zero Fourier mode of U0 and U2.  grad may be NULL (zero).  HOST output;
+ * synchronises the stream. */
+int cf_power_input(cf_problem* h, const double* U0, const double* U2, double nu, const double grad[2], double* out,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif
