@@ -24,7 +24,7 @@ from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State", "zero_state", "random_state",
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
            "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
-           "power_input"]
+           "power_input", "plane_statistics", "ProfileStatistics", "flow"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -266,6 +266,33 @@ def step_(scheme, nlterm, U, P, t, forcing=None, pressuregradient=None, bulkvelo
     return tout.value
 
 
+class Flow:
+    """Flows.flow(channel) (ext/ChannelFlowFlowsExt.jl:15-52): F(state, (t0, T))
+    advances the State in place from t0 to T with the problem's constant step,
+    shortening the final step (solvers factored for its actual duration)."""
+
+    def __init__(self, problem):
+        self.problem = problem
+
+    def __call__(self, state, span, stream=None):
+        t0, T = (float(x) for x in span)
+        p = self.problem
+        g = p.grid
+        U, P = state.velocity, state.stagepressure
+        _check_vec(g, U)
+        _check_spec(g, P)
+        kind, vals = _constraint(p.constraint.get("pressuregradient"), p.constraint.get("bulkvelocity"))
+        arr = (C.c_double * 2)(*vals)
+        tout = C.c_double()
+        _lib.check(_lib.load().cf_advance_to(p.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), t0, T, kind, arr,
+                                             C.byref(tout), _stream(stream)))
+        return state
+
+
+def flow(problem):
+    return Flow(problem)
+
+
 def last_gradient(scheme):
     out = (C.c_double * 2)()
     _lib.check(_lib.load().cf_last_gradient(scheme.h, out, _stream()))
@@ -435,6 +462,52 @@ def power_input(problem, U, nu=None, pressuregradient=(0.0, 0.0)):
     grad = (C.c_double * 2)(*[float(x) for x in pressuregradient])
     _lib.check(_lib.load().cf_power_input(problem.scheme.h, _ptr(U[0]), _ptr(U[2]), nu, grad, C.byref(out), _stream()))
     return out.value
+
+
+def plane_statistics(problem, U):
+    """x-z plane means and covariances of the TOTAL velocity U + Ub e_x at the
+    Lobatto points, from the spectral coefficients on the device.  Returns
+    (mean (Ny, 3): U V W, cov (Ny, 6): uu vv ww uv uw vw)."""
+    g = problem.grid
+    _check_vec(g, U)
+    out = np.zeros((g.Ny, 9))
+    _lib.check(_lib.load().cf_plane_statistics(problem.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]),
+                                               out.ctypes.data_as(C.POINTER(C.c_double)), _stream()))
+    return out[:, :3].copy(), out[:, 3:].copy()
+
+
+STRESS_PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+
+
+class ProfileStatistics:
+    """Plane- and time-averaged profiles (examples/mkm590/statistics.jl:9-43):
+    means of U, V, W and the six velocity covariances about the combined
+    plane-and-time mean, merged sample by sample (parallel-variance update)."""
+
+    def __init__(self, Ny):
+        self.count = 0
+        self.first_time = float("nan")
+        self.last_time = float("nan")
+        self.mean = np.zeros((Ny, 3))
+        self.moment = np.zeros((Ny, 6))
+
+    def sample(self, problem, U, t):
+        """sample!(stats, velocity, t) for a spectral perturbation velocity U."""
+        mu, cov = plane_statistics(problem, U)
+        delta = mu - self.mean
+        n = self.count + 1
+        for k, (i, j) in enumerate(STRESS_PAIRS):
+            self.moment[:, k] += cov[:, k] + (self.count / n) * delta[:, i] * delta[:, j]
+        self.mean += delta / n
+        if self.count == 0:
+            self.first_time = float(t)
+        self.count = n
+        self.last_time = float(t)
+        return self
+
+    def profiles(self):
+        """(mean (Ny, 3), covariances (Ny, 6)) of the samples so far."""
+        return self.mean.copy(), self.moment / max(self.count, 1)
 
 
 def kinetic_energy(problem, U):

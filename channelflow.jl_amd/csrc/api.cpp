@@ -14,6 +14,7 @@
 #include <cstring>
 #include <string>
 #include <utility>
+#include <list>
 #include <vector>
 
 #include "../../include/channelflow_hip.h"
@@ -71,7 +72,7 @@ struct cf_problem {
     double* d_facP = nullptr;
     double* d_tab = nullptr;  // per-degree tau / bulk-mean constants
     DevBank stage[3];
-    std::vector<std::pair<std::pair<double, double>, DevBank>> extra;
+    std::list<std::pair<std::pair<double, double>, DevBank>> extra;  // stable references
     FFTPlan pz{}, py{}, px{};
     double2* d_twx = nullptr;
     double2* six = nullptr;   // 6 S
@@ -501,9 +502,12 @@ struct Timer {
 };
 static int tmark(Timer* tm, int c, hipStream_t st) { return tm ? tm->mark(c, st) : 0; }
 
+// nsteps CNRK2 steps of length dt with the stage solver banks `banks`
+// (default: the problem's dt and its three stage banks)
 static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
-                         Timer* tm = nullptr) {
+                         Timer* tm = nullptr, double dt = 0.0, DevBank* const* banks = nullptr) {
     const GridDims& gd = h->gd;
+    if (!(dt > 0.0)) dt = h->dt;
     for (int step = 0; step < nsteps; ++step) {
         if (int rc = tmark(tm, -1, st)) return rc;
         if (int rc = enqueue_prologue(h, U, st)) return rc;
@@ -517,13 +521,13 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
             a.nl_sign = (h->form == CF_FORM_CONVECTIVE) ? -1.0 : 1.0;
             a.stage = j;
             a.store_Q = (j < 2);
-            a.lambda = 1.0 / (CNRK2_C[j] * h->dt);
+            a.lambda = 1.0 / (CNRK2_C[j] * dt);
             a.Aj = CNRK2_A[j];
             a.weight = CNRK2_B[j] / CNRK2_C[j];
             a.bulk = c.bulk;
             a.c0 = c.c0;
             a.c1 = c.c1;
-            a.bank = h->stage[j].view();
+            a.bank = (banks ? banks[j] : &h->stage[j])->view();
             for (int i = 0; i < 3; ++i) {
                 a.Nphys[i] = fld(h->six, gd.S, i);
                 a.Ncoef[i] = (h->form == CF_FORM_DIVERGENCE) ? Ncoef[i] : nullptr;
@@ -554,6 +558,36 @@ extern "C" int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, dou
     double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
     if (int rc = enqueue_steps(h, U, (double2*)P, nsteps, c, (hipStream_t)stream)) return rc;
     if (t_out) *t_out = t + nsteps * h->dt;
+    return CF_OK;
+}
+
+// Flows.step! adapter semantics (ext/ChannelFlowFlowsExt.jl:36-52): nominal
+// steps of dt from t while they fit in T, then one shortened step of the
+// remaining duration with stage solvers built for it (cached per duration)
+extern "C" int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T,
+                             int constraint, const double* values, double* t_out, void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[4] = {U0, U1, U2, P};
+    if (int rc = check_fields(f, 4)) return rc;
+    if (!(std::isfinite(t) && std::isfinite(T) && T >= t)) return fail(CF_ERR_ARGUMENT, "need finite t <= T");
+    Constraint c;
+    if (int rc = resolve_constraint(h, constraint, values, c)) return rc;
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    const double span = T - t, tol = 1e-10 * h->dt;
+    long long n = (long long)std::floor((span + tol) / h->dt);
+    double rem = span - (double)n * h->dt;
+    if (rem <= tol) rem = 0.0;
+    if (n > (1ll << 30)) return fail(CF_ERR_ARGUMENT, "too many steps");
+    if (int rc = enqueue_steps(h, U, (double2*)P, (int)n, c, st)) return rc;
+    if (rem > 0.0) {
+        DevBank* b[3];
+        for (int j = 0; j < 3; ++j)
+            if (int rc = get_bank(h, h->nu, 1.0 / (CNRK2_C[j] * rem), &b[j])) return rc;
+        if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, nullptr, rem, b)) return rc;
+    }
+    if (t_out) *t_out = T;
     return CF_OK;
 }
 
@@ -846,5 +880,31 @@ extern "C" int cf_power_input(cf_problem* h, const double* U0, const double* U2,
         input -= g[c] * host_bulkmean_real(m, Ny);
     }
     *out = input;
+    return CF_OK;
+}
+
+/* plane statistics of the total velocity at the Lobatto points */
+extern "C" int cf_plane_statistics(cf_problem* h, const double* U0, const double* U1, const double* U2, double* out,
+                                   void* stream) {
+    if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    const void* fu[3] = {U0, U1, U2};
+    if (int rc = check_fields(fu, 3)) return rc;
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    const int Ny = h->gd.Ny, nq = Ny * PLANE_STATS_Q;
+    YArgs a = base_yargs(h);
+    a.U[0] = (double2*)U0;
+    a.U[1] = (double2*)U1;
+    a.U[2] = (double2*)U2;
+    double* part = reinterpret_cast<double*>(h->R);  // nblocks * nq doubles <= 6 S
+    a.partial = part;
+    HIPTRY(launch_ykernel(Y_PLANE_STATS, a, st));
+    double* red = reinterpret_cast<double*>(h->Q);  // nq doubles (Q is free between steps)
+    HIPTRY(launch_reduce_columns(part, (int)y_blocks(h->gd), nq, red, st));
+    std::vector<double> tmp((size_t)nq);
+    HIPTRY(hipMemcpyAsync(tmp.data(), red, nq * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPTRY(hipStreamSynchronize(st));
+    // out: Ny x 9, columns U V W uu vv ww uv uw vw
+    std::memcpy(out, tmp.data(), nq * sizeof(double));
     return CF_OK;
 }

@@ -15,7 +15,7 @@ struct Bank {
     double lambda, nu;
 };
 
-enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_INV, Y_DIVERGENCE, Y_DOT, Y_DISSIPATION };
+enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_INV, Y_DIVERGENCE, Y_DOT, Y_DISSIPATION, Y_PLANE_STATS };
 
 // nonlinear forms (src/nonlinear.jl:113-193); values match CF_FORM_* of the C ABI
 enum { FORM_ROTATING = 0, FORM_CONVECTIVE = 1, FORM_DIVERGENCE = 2 };
@@ -86,6 +86,9 @@ hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st);
 unsigned y_blocks(const GridDims& gd);  // blocks of a y launch (partials of Y_DOT)
 // out[0] = sum of part[0..n) in a fixed order (one block)
 hipError_t launch_reduce(const double* part, int n, double* out, hipStream_t st);
+// out[q] = sum over b < nblocks of part[b * nq + q], q < nq, in a fixed order
+hipError_t launch_reduce_columns(const double* part, int nblocks, int nq, double* out, hipStream_t st);
+constexpr int PLANE_STATS_Q = 9;  // per y: means of u, v, w; covariances uu vv ww uv uw vw
 int y_group_lanes(int Ny);
 size_t y_lds_bytes(int Ny, int ld);
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st);

@@ -754,6 +754,95 @@ __global__ void __launch_bounds__(YNT) ydot_kernel(YArgs a, int curl) {
     }
 }
 
+// Plane statistics of the TOTAL velocity V = U + Ub e_x (examples/mkm590/
+// statistics.jl:26-43) at the Lobatto points.  By Parseval on the padded
+// grid, the x-z plane mean is the (0,0) mode's y-physical value and the
+// plane covariance of a, b is sum over the other modes of
+// w_k Re(a(k,l,y) conj(b(k,l,y))) with w = 1 (k = 0), 2 (k > 0), Nyquist
+// planes excluded.  partial[block * Ny * 9 + y * 9 + q] (PLANE_STATS_Q
+// quantities: means u v w, then uu vv ww uv uw vw).
+template <int L>
+__global__ void __launch_bounds__(YNT) ystats_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    load_tables<L>(smem, a);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Group<L> g(tile.tab);
+    ModeInfo mi = mode_info(a.gd, tile.s);
+    const int Ny = a.gd.Ny, P = a.gd.P;
+    constexpr int M = YNT / L, W = 64 / L, Q = PLANE_STATS_Q;
+    double2 vals[3][4];  // values at y = 4t+e (group layout after the inverse transform)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        CVec U = tile.load(a.U[c], g);
+        if (c == 0 && mi.mean && mi.valid) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int n = 4 * g.t + e;
+                if (n < Ny) U.v[e] = cadd(U.v[e], cmk(a.base[n], 0.0));
+            }
+        }
+        // inverse Chebyshev transform in the wave's lines, result read back in group layout
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            if (n <= P) {
+                double2 x = U.v[e];
+                tile.bufA[tile.mw * tile.ld + n] = (n == 0 || n == P) ? cscale(x, 2.0) : x;
+                if (n >= 1 && n <= P - 1) tile.bufA[tile.mw * tile.ld + 2 * P - n] = x;
+            }
+        }
+        wave_sync();
+        double2* res = lds_fft<true>(tile.bufA, tile.bufB, a.dct, tile.tw, W, tile.ld, -1, tile.lane, 64);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int n = 4 * g.t + e;
+            vals[c][e] = (n <= P) ? cscale(res[tile.mw * tile.ld + n], 0.5) : czero();
+        }
+        wave_sync();
+    }
+    // this mode's contributions at each y of the lane: 9 quantities x 4 points
+    const bool use = mi.valid && !mi.nyq;
+    const double wk = (mi.ix == 0) ? 1.0 : 2.0;
+    const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
+    __syncthreads();  // tables and wave lines are free: reuse LDS as the block's [Ny][Q] accumulator
+    double* red = reinterpret_cast<double*>(smem);
+    for (int idx = threadIdx.x; idx < Ny * Q; idx += YNT) red[idx] = 0.0;
+    __syncthreads();
+    const int mloc = threadIdx.x / L;
+    // the block's modes add in a fixed order (deterministic sums)
+    for (int m = 0; m < M; ++m) {
+        if (mloc == m) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int y = 4 * g.t + e;
+                if (y >= Ny) continue;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) red[y * Q + c] += (use && mi.mean) ? vals[c][e].x : 0.0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const double2 u = vals[pi[k]][e], v = vals[pj[k]][e];
+                    red[y * Q + 3 + k] += (use && !mi.mean) ? wk * (u.x * v.x + u.y * v.y) : 0.0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < Ny * Q; idx += YNT) a.partial[(size_t)blockIdx.x * Ny * Q + idx] = red[idx];
+}
+
+__global__ void __launch_bounds__(256) reduce_columns_kernel(const double* part, int nblocks, int nq, double* out) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nq + q];
+    out[q] = s;
+}
+
+hipError_t launch_reduce_columns(const double* part, int nblocks, int nq, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, part, nblocks, nq, out);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) reduce_kernel(const double* part, int n, double* out) {
     __shared__ double s[256];
     double v = 0.0;
@@ -796,6 +885,7 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     allow_lds(ydct_kernel<L>, lds);
     allow_lds(ydivergence_kernel<L>, lds);
     allow_lds(ydot_kernel<L>, lds);
+    allow_lds(ystats_kernel<L>, lds);
     switch (which) {
         case Y_STAGE: hipLaunchKernelGGL(ystage_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_PROLOGUE: hipLaunchKernelGGL(yprologue_kernel<L>, grid, dim3(YNT), lds, st, a); break;
@@ -807,6 +897,7 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
         case Y_DIVERGENCE: hipLaunchKernelGGL(ydivergence_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_DOT: hipLaunchKernelGGL(ydot_kernel<L>, grid, dim3(YNT), lds, st, a, 0); break;
         case Y_DISSIPATION: hipLaunchKernelGGL(ydot_kernel<L>, grid, dim3(YNT), lds, st, a, 1); break;
+        case Y_PLANE_STATS: hipLaunchKernelGGL(ystats_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -80,6 +80,14 @@ int cf_step(cf_problem* h, double* U0, double* U1, double* U2, double* P, double
 int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int nsteps,
                int constraint, const double* values, double* t_out, void* stream);
 
+/* Integrate from t to T as Flows.jl does with the problem's constant step
+ * (ext/ChannelFlowFlowsExt.jl:36-52): step! with dt while a full step fits,
+ * then one shortened step of the remaining duration (T - t - n dt > 1e-10 dt)
+ * whose stage solvers are factored for that duration and cached on the
+ * handle.  *t_out = T. */
+int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T, int constraint,
+                  const double* values, double* t_out, void* stream);
+
 /* Measurement variant of cf_step (no counterpart in the reference): one step
  * with HIP events recorded on `stream` around every launch; waits for the
  * step, then ms[0..4] = device milliseconds spent in the prologue (y), the
@@ -132,6 +140,14 @@ int cf_dissipation(cf_problem* h, const double* U0, const double* U1, const doub
  * synchronises the stream. */
 int cf_power_input(cf_problem* h, const double* U0, const double* U2, double nu, const double grad[2], double* out,
                    void* stream);
+
+/* Plane statistics of the TOTAL velocity U + Ub e_x at the Lobatto points y_j
+ * (the x-z averages of examples/mkm590/statistics.jl:26-43 sample!, computed
+ * exactly from the spectral coefficients by Parseval on the padded grid).
+ * HOST out[Ny][9]: plane means U, V, W, then plane covariances
+ * uu, vv, ww, uv, uw, vw.  Synchronises the stream; time averaging (the
+ * parallel-variance merge of samples) is left to the caller. */
+int cf_plane_statistics(cf_problem* h, const double* U0, const double* U1, const double* U2, double* out, void* stream);
 
 #ifdef __cplusplus
 }

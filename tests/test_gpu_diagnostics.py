@@ -50,3 +50,45 @@ def test_laminar_closed_forms(gpu, Ny):
         p = cf.power_input(prob, U, nu, pressuregradient=grad)
         say(f"  {name} Ny={Ny}: E {e!r} D {d!r} P {p!r}")
         assert abs(e - E) < 1e-14 and abs(d - D) < 1e-14 * max(1, D / nu) and abs(p - P) < 1e-14
+
+
+def _physical_statistics(go, U, base_values):
+    """Reference algorithm on the padded physical grid (statistics.jl:26-43):
+    total velocity, x-z plane means and covariances."""
+    V = U.copy()
+    V[0, :, 0, 0] += O.chebcoeffs(base_values)
+    u = [O.inverse(go, V[c]) for c in range(3)]  # (Ny, Nzp, Nxp)
+    mean = np.stack([f.mean(axis=(1, 2)) for f in u], axis=1)
+    pairs = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+    cov = np.stack([((u[i] - mean[:, i, None, None]) * (u[j] - mean[:, j, None, None])).mean(axis=(1, 2))
+                    for i, j in pairs], axis=1)
+    return mean, cov
+
+
+@pytest.mark.parametrize("dims", [(8, 17, 8), (7, 9, 9), (16, 33, 12), (10, 97, 12), (6, 129, 4)])
+def test_plane_statistics_match_physical_grid(gpu, dims):
+    import torch
+    cf = gpu
+    say("plane statistics", dims)
+    gc, go, prob = make(cf, dims, profile=lambda y: 1 - y ** 2)
+    rng = np.random.default_rng(31 + sum(dims))
+    U = hermitian_random(go, rng, 3, 0.3)
+    mean, cov = cf.plane_statistics(prob, torch.as_tensor(U, device="cuda"))
+    mref, cref = _physical_statistics(go, U, 1 - go.y ** 2)
+    em = np.abs(mean - mref).max() / np.abs(mref).max()
+    ec = np.abs(cov - cref).max() / np.abs(cref).max()
+    say(f"  mean rel {em:.2e} covariance rel {ec:.2e}")
+    assert em < 1e-13 and ec < 1e-12
+    # time averaging: two samples of different fields merge to the pooled statistics
+    U2 = hermitian_random(go, rng, 3, 0.2)
+    st = cf.ProfileStatistics(go.Ny)
+    st.sample(prob, torch.as_tensor(U, device="cuda"), 0.0).sample(prob, torch.as_tensor(U2, device="cuda"), 1.0)
+    m2, c2 = _physical_statistics(go, U2, 1 - go.y ** 2)
+    pm = (mref + m2) / 2
+    pairs = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+    pc = np.stack([(cref[:, k] + c2[:, k]) / 2 + ((mref[:, i] - pm[:, i]) * (mref[:, j] - pm[:, j]) +
+                                                  (m2[:, i] - pm[:, i]) * (m2[:, j] - pm[:, j])) / 2
+                   for k, (i, j) in enumerate(pairs)], axis=1)
+    gm, gcv = st.profiles()
+    assert np.abs(gm - pm).max() < 1e-13 * np.abs(pm).max()
+    assert np.abs(gcv - pc).max() < 1e-12 * np.abs(pc).max()

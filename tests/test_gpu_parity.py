@@ -266,3 +266,30 @@ def test_step_poiseuille(gpu, dims, bulk):
     gr = ref.last_gradient()
     # the gradient is a functional of the velocity: same relative envelope
     assert np.allclose(g, gr, rtol=max(1e-10, ENVELOPE_FACTOR * env[0]), atol=1e-15)
+
+
+@pytest.mark.parametrize("span_steps", [3.5, 2.0, 0.25])
+def test_flow_shortened_final_step(gpu, span_steps):
+    """Flows adapter semantics (ext/ChannelFlowFlowsExt.jl:36-52): whole steps
+    of dt, then one step of the remainder with its own factors; compared with
+    the oracle stepping the same sequence (a temporary problem for the
+    remainder)."""
+    import torch
+    cf = gpu
+    dims, dt, nu = (8, 17, 8), 0.01, 1 / 400
+    say("flow", span_steps)
+    L = (2 * np.pi / 1.14, 2 * np.pi / 2.5)
+    gc, go, prob = make(cf, dims, L=L, nu=nu, dt=dt)
+    ref = O.Problem(go, lambda y: y, nu, dt)
+    U0, P0 = ref.random_state(cf.gaussian_noise(gc, 0.1, 7))
+    st = cf.State(torch.as_tensor(U0, device="cuda"), torch.as_tensor(P0, device="cuda"))
+    T = span_steps * dt
+    cf.flow(prob)(st, (0.0, T))
+    n = int(np.floor(span_steps + 1e-9))
+    rem = T - n * dt
+    U1, P1 = ref.step(U0, P0, n) if n else (U0, P0)
+    if rem > 1e-10 * dt:
+        U1, P1 = O.Problem(go, lambda y: y, nu, rem).step(U1, P1, 1)
+    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, max(n, 1))), [U0, P0], _split((U1, P1)), trials=2)
+    assert_parity(f"flow {span_steps}", _split((st.velocity.cpu().numpy(), st.stagepressure.cpu().numpy())),
+                  _split((U1, P1)), env)
