@@ -24,7 +24,7 @@ from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State", "zero_state", "random_state",
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
            "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
-           "power_input", "plane_statistics", "ProfileStatistics", "flow"]
+           "power_input", "plane_statistics", "ProfileStatistics", "flow", "device_gaussian_noise"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -370,16 +370,27 @@ def gaussian_noise(grid, epsilon, seed):
     return epsilon * np.random.default_rng(seed).standard_normal((3,) + grid.phys_shape)
 
 
+def device_gaussian_noise(problem, epsilon, seed):
+    """epsilon * N(0,1) on the padded grid for u, v, w generated on the device
+    (Philox4x32-10 + Box-Muller, counter = element; cf_gaussian_noise):
+    float64 tensor (3, Ny, Nzp, Nxp)."""
+    g = problem.grid
+    out = g.zeros_phys(3)
+    _lib.check(_lib.load().cf_gaussian_noise(problem.scheme.h, float(epsilon), int(seed) & (2 ** 64 - 1), _ptr(out),
+                                             _stream()))
+    return out
+
+
 def random_state(problem, epsilon, seed=42):
-    """random_state (initialization.jl:32-51): Gaussian padded field -> FFT ->
-    project! -> pressure."""
+    """random_state (initialization.jl:32-51) entirely on the device: Gaussian
+    padded field -> FFT -> project! -> pressure (cf_random_state)."""
     if epsilon < 0:
         raise ArgumentError("epsilon must be non-negative")
-    torch = _torch()
-    noise = torch.as_tensor(gaussian_noise(problem.grid, epsilon, seed), device="cuda")
-    U = FFT(problem, noise)
-    project_(U, problem)
-    return State(U, pressure(U, problem))
+    g = problem.grid
+    U, P = g.zeros_spec(3), g.zeros_spec()
+    _lib.check(_lib.load().cf_random_state(problem.scheme.h, float(epsilon), int(seed) & (2 ** 64 - 1), _ptr(U[0]),
+                                           _ptr(U[1]), _ptr(U[2]), _ptr(P), _stream()))
+    return State(U, P)
 
 
 def perturbation_state(problem, amplitude, seed=42, decay=4.0):

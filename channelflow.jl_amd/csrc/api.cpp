@@ -81,6 +81,7 @@ struct cf_problem {
     double2* R = nullptr;     // 3 S
     double* d_grad = nullptr;
     double* d_scalar = nullptr;  // reduction result of the diagnostics
+    double* d_noise = nullptr;   // 3 padded physical fields, allocated by the first cf_random_state
     int KT = 1, zld = 0, LPB = 1, xld = 0, yld = 0;
     int xwaves = 4;  // even Nxp: wavefronts per block of the nonlinear x pass
     int zppt = 0;    // z pass in-place points per thread (0: two buffers)
@@ -907,4 +908,34 @@ extern "C" int cf_plane_statistics(cf_problem* h, const double* U0, const double
     // out: Ny x 9, columns U V W uu vv ww uv uw vw
     std::memcpy(out, tmp.data(), nq * sizeof(double));
     return CF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// random_state on the device (src/initialization.jl:32-51)
+// ---------------------------------------------------------------------------
+extern "C" int cf_gaussian_noise(cf_problem* h, double epsilon, unsigned long long seed, double* out, void* stream) {
+    if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    if (!(std::isfinite(epsilon) && epsilon >= 0)) return fail(CF_ERR_ARGUMENT, "epsilon must be finite and non-negative");
+    DeviceGuard dg(h->device);
+    const long long n = (long long)h->gd.Nxp * h->gd.Nzp * h->gd.Ny;
+    HIPTRY(launch_gaussian(out, n, epsilon, seed, (hipStream_t)stream));
+    return CF_OK;
+}
+
+extern "C" int cf_random_state(cf_problem* h, double epsilon, unsigned long long seed, double* U0, double* U1, double* U2,
+                               double* P, void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[4] = {U0, U1, U2, P};
+    if (int rc = check_fields(f, 4)) return rc;
+    if (!(std::isfinite(epsilon) && epsilon >= 0)) return fail(CF_ERR_ARGUMENT, "epsilon must be finite and non-negative");
+    DeviceGuard dg(h->device);
+    const long long n = (long long)h->gd.Nxp * h->gd.Nzp * h->gd.Ny;
+    if (!h->d_noise)
+        if (int rc = dalloc(h, &h->d_noise, 3 * (size_t)n)) return rc;
+    if (int rc = cf_gaussian_noise(h, epsilon, seed, h->d_noise, stream)) return rc;
+    double* U[3] = {U0, U1, U2};
+    for (int c = 0; c < 3; ++c)
+        if (int rc = cf_forward(h, h->d_noise + (size_t)c * n, U[c], stream)) return rc;
+    if (int rc = cf_project(h, U0, U1, U2, stream)) return rc;
+    return cf_pressure(h, U0, U1, U2, P, stream);
 }

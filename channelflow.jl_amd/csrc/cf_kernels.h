@@ -88,6 +88,8 @@ unsigned y_blocks(const GridDims& gd);  // blocks of a y launch (partials of Y_D
 hipError_t launch_reduce(const double* part, int n, double* out, hipStream_t st);
 // out[q] = sum over b < nblocks of part[b * nq + q], q < nq, in a fixed order
 hipError_t launch_reduce_columns(const double* part, int nblocks, int nq, double* out, hipStream_t st);
+// epsilon * N(0,1) samples into out3[3][n] (Philox4x32-10 + Box-Muller, counter = element)
+hipError_t launch_gaussian(double* out3, long long n, double eps, unsigned long long seed, hipStream_t st);
 constexpr int PLANE_STATS_Q = 9;  // per y: means of u, v, w; covariances uu vv ww uv uw vw
 int y_group_lanes(int Ny);
 size_t y_lds_bytes(int Ny, int ld);

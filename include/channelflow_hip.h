@@ -149,6 +149,19 @@ int cf_power_input(cf_problem* h, const double* U0, const double* U2, double nu,
  * parallel-variance merge of samples) is left to the caller. */
 int cf_plane_statistics(cf_problem* h, const double* U0, const double* U1, const double* U2, double* out, void* stream);
 
+/* Gaussian noise epsilon * N(0,1) for u, v, w on the padded physical grid
+ * (initialization.jl:37-42): out (DEVICE) holds 3 fields of Nxp*Nzp*Ny
+ * doubles in (x, z, y) order.  Stream: Philox4x32-10 keyed by the 64-bit seed,
+ * counter = (element pair, component); Box-Muller.  Julia's Xoshiro stream is
+ * not reproduced. */
+int cf_gaussian_noise(cf_problem* h, double epsilon, unsigned long long seed, double* out, void* stream);
+
+/* random_state(problem, epsilon; seed) (initialization.jl:32-51) on the
+ * device: noise (cf_gaussian_noise) -> FFT -> project! -> pressure, into
+ * U0..U2 and P. */
+int cf_random_state(cf_problem* h, double epsilon, unsigned long long seed, double* U0, double* U1, double* U2, double* P,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif
