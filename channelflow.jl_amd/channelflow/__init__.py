@@ -19,12 +19,14 @@ import math
 import numpy as np
 
 from . import _lib
+from . import io  # noqa: F401  (portable state files)
 from ._lib import ArgumentError, DimensionMismatch, DeviceError  # noqa: F401
 
 __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State", "zero_state", "random_state",
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
            "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
-           "power_input", "plane_statistics", "ProfileStatistics", "flow", "device_gaussian_noise"]
+           "power_input", "plane_statistics", "ProfileStatistics", "flow", "device_gaussian_noise",
+           "savefield", "loadfield"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -188,12 +190,14 @@ class NonLinearTerm:
 def _constraint(pressuregradient, bulkvelocity):
     if pressuregradient is not None and bulkvelocity is not None:
         raise ArgumentError("specify either pressuregradient or bulkvelocity")
-    if bulkvelocity is not None:
-        vals = tuple(float(v) for v in bulkvelocity)
-        kind = 1
-    else:
-        vals = tuple(float(v) for v in (pressuregradient if pressuregradient is not None else (0.0, 0.0)))
-        kind = 0
+    kind = 1 if bulkvelocity is not None else 0
+    given = bulkvelocity if kind else (pressuregradient if pressuregradient is not None else (0.0, 0.0))
+    try:
+        vals = tuple(float(v) for v in given)
+    except TypeError:
+        raise ArgumentError("mean-flow constraint must be an (x, z) pair") from None
+    if len(vals) != 2:
+        raise ArgumentError("mean-flow constraint must be an (x, z) pair")
     if not all(math.isfinite(v) for v in vals):
         raise ArgumentError("mean-flow constraint values must be finite")
     return kind, vals
@@ -519,6 +523,27 @@ class ProfileStatistics:
     def profiles(self):
         """(mean (Ny, 3), covariances (Ny, 6)) of the samples so far."""
         return self.mean.copy(), self.moment / max(self.count, 1)
+
+
+def savefield(path, grid, field=None, *, state=None, time=None, step=None):
+    """savefield(path, field | state) (src/io.jl:21-30) in the portable layout
+    documented in channelflow.io; device tensors are copied to the host."""
+    if state is not None:
+        return io.savefield(path, grid, velocity=state.velocity, pressure=state.stagepressure, time=time, step=step)
+    return io.savefield(path, grid, field, time=time, step=step)
+
+
+def loadfield(path, grid=None, device="cuda"):
+    """loadfield(path) (src/io.jl:40-54): a State (velocity, stage pressure) or
+    a field tensor on `device`.  With `grid`, its resolution and domain must
+    match the file's."""
+    torch = _torch()
+    f = io.loadfield(path)
+    if grid is not None and (f.Nx, f.Ny, f.Nz, f.Lx, f.Lz) != (grid.Nx, grid.Ny, grid.Nz, grid.Lx, grid.Lz):
+        raise ArgumentError("file grid does not match the problem grid")
+    if f.is_state:
+        return State(torch.as_tensor(f.velocity(), device=device), torch.as_tensor(f.pressure(), device=device))
+    return torch.as_tensor(f.field(), device=device)
 
 
 def kinetic_energy(problem, U):

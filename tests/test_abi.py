@@ -64,5 +64,8 @@ def test_python_mirror_validates_constraints():
         cf._constraint((0, 0), (0, 0))
     with pytest.raises(cf.ArgumentError):
         cf._constraint(None, (float("nan"), 0))
+    for bad in (2 / 3, (1.0,), (1.0, 0.0, 0.0)):
+        with pytest.raises(cf.ArgumentError):
+            cf._constraint(None, bad)
     assert cf._constraint(None, None) == (0, (0.0, 0.0))
     assert cf._constraint(None, (1.0, 0.0)) == (1, (1.0, 0.0))
