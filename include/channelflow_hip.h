@@ -50,7 +50,7 @@ typedef struct cf_config {
     double nu, dt;           /* CNRK2(grid, baseflow, nu, dt), cnrk2.jl:58-83 */
     const double* baseflow;  /* HOST: Ny base-velocity values at the Lobatto
                                 points y_j = cos(pi j/(Ny-1)) (problem.jl:105) */
-    int form;                /* CF_FORM_* (only CF_FORM_ROTATING on device) */
+    int form;                /* CF_FORM_*: nonlinear form of the problem */
     int constraint;          /* CF_CONSTRAINT_*: the problem default */
     double constraint_values[2];
     int device;              /* HIP device ordinal */
@@ -104,8 +104,9 @@ int cf_forward(cf_problem* h, const double* u, double* U, void* stream);
 /* InverseFFT! (src/ffts.jl:151-166): resolved spectrum U -> padded physical u */
 int cf_inverse(cf_problem* h, const double* U, double* u, void* stream);
 
-/* NonLinearTerm (rotational form) N = FFT(V x curl V), V = U + Ub e_x
- * (src/nonlinear.jl:173-193); overwrites N0..N2 */
+/* NonLinearTerm of the problem's form, V = U + Ub e_x: rotating N = FFT(V x curl V)
+ * (src/nonlinear.jl:173-193), convective N = -FFT((V.grad)V) (113-140) or
+ * divergence N = -div FFT(V V) (146-167); overwrites N0..N2 */
 int cf_nonlinear(cf_problem* h, const double* U0, const double* U1, const double* U2, double* N0, double* N1,
                  double* N2, void* stream);
 
