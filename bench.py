@@ -58,6 +58,92 @@ def algorithmic_bytes(Nx, Ny, Nz):
     return per_launch, step_canonical, S, Sz
 
 
+MIXED = dict(N=(256, 193, 256), L=(4 * math.pi, 4 * math.pi / 3), nu=1 / 2800, dt=0.0025, amp=0.1, bulk=(1.0, 0.0),
+             desc="plane Poiseuille 256x193x256, Lx=4pi Lz=4pi/3 nu=1/2800 dt=0.0025, bulk velocity (1, 0), "
+                  "amplitude 0.1; FP32 transforms and nonlinear term, FP64 solves")
+
+
+def mixed_bytes(Nx, Ny, Nz):
+    """Algorithmic bytes of the mixed step: FP64 state traffic 16*36 S, FP32
+    y-physical and z-padded traffic 8*(54 S + 54 Sz)."""
+    Nxh, Nzp = Nx // 2 + 1, (3 * Nz + 1) // 2
+    S, Sz = Nxh * Nz * Ny, Nxh * Nzp * Ny
+    return 16 * 36 * S + 8 * (54 * S + 54 * Sz), 16 * (90 * S + 54 * Sz)
+
+
+def wall_shear(g, U, nu, base):
+    """nu (u'(-1) - u'(+1)) / 2 of the total mean streamwise velocity
+    (examples/mkm590/run.jl:236): Chebyshev sums over the (0,0) column."""
+    import numpy as np
+    a = U[0, :, 0, 0].real.cpu().numpy() + base
+    n = np.arange(g.Ny, dtype=float)
+    dp = float(np.sum(n ** 2 * a))
+    dm = float(np.sum((-1.0) ** (n + 1) * n ** 2 * a))
+    return nu * (dm - dp) / 2
+
+
+def bench_mixed(cf, steps):
+    """FP32-mixed step against the plain FP64 step on the same seeded state:
+    steps/s of each, and the relative error of energy, dissipation and wall
+    shear after `steps` steps."""
+    import numpy as np
+    import torch
+
+    w = MIXED
+    g = cf.Grid(*w["N"], *w["L"])
+    prof = lambda y: 1.5 * (1 - y ** 2)  # noqa: E731
+    res = {}
+    states = {}
+    s0 = None
+    for prec in ("fp64", "mixed"):
+        p = cf.ChannelFlowProblem(g, prof, w["nu"], w["dt"], bulkvelocity=w["bulk"], precision=prec)
+        if s0 is None:
+            s0 = cf.perturbation_state(p, w["amp"], seed=7)
+        st = s0.copy()
+        cf.step_(p.scheme, p.nlterm, st.velocity, st.stagepressure, 0.0, **p.constraint)  # warm-up step
+        st = s0.copy()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        cf.step_(p.scheme, p.nlterm, st.velocity, st.stagepressure, 0.0, nsteps=steps, **p.constraint)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        base = base_coefficients(p)
+        res[prec] = {"ms_per_step": round(ms, 4), "steps_per_s": round(1000.0 / ms, 2),
+                     "E": cf.kinetic_energy(p, st.velocity), "D": cf.dissipation_rate(p, st.velocity),
+                     "wall_shear": wall_shear(g, st.velocity, w["nu"], base)}
+        states[prec] = st
+        del p
+    mb, fb = mixed_bytes(*w["N"])
+    f64, mx = res["fp64"], res["mixed"]
+    rel = lambda k: abs(mx[k] - f64[k]) / abs(f64[k])  # noqa: E731
+    dU = float((states["mixed"].velocity - states["fp64"].velocity).abs().max())
+    return {
+        "workload": w["desc"], "steps": steps,
+        "mixed_steps_per_s": mx["steps_per_s"], "fp64_steps_per_s": f64["steps_per_s"],
+        "speedup_vs_fp64": round(mx["steps_per_s"] / f64["steps_per_s"], 3),
+        "mixed_hbm_frac": round(mb * mx["steps_per_s"] / 1e9 / HBM_PEAK_GBS, 4),
+        "fp64_hbm_frac": round(fb * f64["steps_per_s"] / 1e9 / HBM_PEAK_GBS, 4),
+        "rel_err_energy": rel("E"), "rel_err_dissipation": rel("D"), "rel_err_wall_shear": rel("wall_shear"),
+        "max_abs_dU": dU,
+    }
+
+
+def base_coefficients(problem):
+    """Chebyshev coefficients of the problem's base profile (host DCT-I of its
+    Lobatto-point values, chebcoeffs normalisation)."""
+    import numpy as np
+    v = np.asarray(problem.baseflow, dtype=float)
+    P = len(v) - 1
+    j = np.arange(P + 1)
+    c = np.array([(v * np.cos(np.pi * n * j / P) * np.where((j == 0) | (j == P), 0.5, 1.0)).sum() for n in range(P + 1)])
+    c *= 2.0 / P
+    c[0] /= 2
+    c[P] /= 2
+    return c
+
+
 def make_problem(cf, w):
     Nx, Ny, Nz = w["N"]
     g = cf.Grid(Nx, Ny, Nz, *w["L"])
@@ -90,6 +176,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--workload", default="poiseuille128", choices=sorted(WORKLOADS))
+    ap.add_argument("--mixed-steps", type=int, default=500,
+                    help="steps of the FP32-mixed 256x193x256 comparison (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -166,6 +254,15 @@ def main():
                "sample": f"{args.cpu_steps} oracle steps of the same workload after 1 warm-up step "
                          f"({secs:.1f} s, {cores} OpenMP threads)"}
 
+    forms = {}
+    if args.mixed_steps > 0:
+        del prob, state, U, P
+        torch.cuda.empty_cache()
+        try:
+            forms["mixed"] = bench_mixed(cf, args.mixed_steps)
+        except Exception as e:  # the headline line must still print
+            forms["mixed"] = {"error": f"{type(e).__name__}: {e}"}
+
     out = {
         "metric": "CNRK2 steps/s & %HBM roofline, 128×97×128 FP64; mixed/batched/closure vs plain",
         "value": round(steps_per_s, 3),
@@ -188,6 +285,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(out), flush=True)
     return 0

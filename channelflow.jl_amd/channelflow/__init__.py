@@ -137,8 +137,10 @@ class CNRK2:
     """Device CNRK2 scheme: owns the libcfhip problem handle (factored
     stage solvers, plans, workspaces)."""
 
-    def __init__(self, grid, baseflow_values, nu, dt, form, constraint, device=0):
+    def __init__(self, grid, baseflow_values, nu, dt, form, constraint, device=0, precision="fp64"):
         require_device()
+        if precision not in _lib.PRECISIONS:
+            raise ArgumentError("precision must be 'fp64' or 'mixed'")
         L = _lib.load()
         cfg = _lib.cf_config()
         cfg.Nx, cfg.Ny, cfg.Nz = grid.Nx, grid.Ny, grid.Nz
@@ -153,10 +155,12 @@ class CNRK2:
         cfg.constraint = kind
         cfg.constraint_values[0], cfg.constraint_values[1] = vals
         cfg.device = int(device)
+        cfg.precision = _lib.PRECISIONS[precision]
         h = C.c_void_p()
         _lib.check(L.cf_problem_create(C.byref(cfg), C.byref(h)))
         self.h = h
         self.grid, self.nu, self.dt = grid, float(nu), float(dt)
+        self.precision = precision
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -205,17 +209,18 @@ def _constraint(pressuregradient, bulkvelocity):
 
 class ChannelFlowProblem:
     """ChannelFlowProblem(grid, profile, nu, dt; form, forcing, pressuregradient|bulkvelocity)
-    (src/problem.jl:69-107)."""
+    (src/problem.jl:69-107).  precision="mixed" runs the time step with FP32
+    transforms and nonlinear term and FP64 solves (CF_PRECISION_MIXED)."""
 
     def __init__(self, grid, profile, nu, dt, form=RotatingForm, forcing=None, pressuregradient=None,
-                 bulkvelocity=None, device=0):
+                 bulkvelocity=None, device=0, precision="fp64"):
         if forcing is not None:
             raise NotImplementedError("custom forcing is not available on the device path")
         kind, vals = _constraint(pressuregradient, bulkvelocity)
         self.grid = grid
         self.baseflow = np.asarray(profile(grid.y) if callable(profile) else profile, dtype=np.float64)
         self.baseflow = np.broadcast_to(self.baseflow, (grid.Ny,)).copy()
-        self.scheme = CNRK2(grid, self.baseflow, nu, dt, form, (kind, vals), device)
+        self.scheme = CNRK2(grid, self.baseflow, nu, dt, form, (kind, vals), device, precision)
         self.nlterm = NonLinearTerm(self.scheme)
         self.forcing = None
         self.constraint = {"bulkvelocity": vals} if kind == 1 else {"pressuregradient": vals}

@@ -27,7 +27,11 @@ class cf_config(C.Structure):
         ("form", C.c_int), ("constraint", C.c_int),
         ("constraint_values", C.c_double * 2),
         ("device", C.c_int),
+        ("precision", C.c_int),
     ]
+
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
 
 
 _lib = None

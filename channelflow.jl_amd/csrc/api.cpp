@@ -86,6 +86,9 @@ struct cf_problem {
     int xwaves = 4;  // even Nxp: wavefronts per block of the nonlinear x pass
     int zppt = 0;    // z pass in-place points per thread (0: two buffers)
     int xppt = 0;    // even-Nxp x pass in-place points per lane (0: two buffers)
+    // mixed-precision step: FP32 y-physical fields and periodic passes
+    int mixed = 0;
+    int KT32 = 1, zppt32 = 0, xppt32 = 0, xwaves32 = 0;
 };
 
 struct DeviceGuard {
@@ -192,6 +195,10 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         return fail(CF_ERR_ARGUMENT, "mean-flow constraint values must be finite");
     if (cfg->form != CF_FORM_ROTATING && cfg->form != CF_FORM_CONVECTIVE && cfg->form != CF_FORM_DIVERGENCE)
         return fail(CF_ERR_ARGUMENT, "unknown nonlinear form");
+    if (cfg->precision != CF_PRECISION_FP64 && cfg->precision != CF_PRECISION_MIXED)
+        return fail(CF_ERR_ARGUMENT, "unknown precision");
+    if (cfg->precision == CF_PRECISION_MIXED && (cfg->form == CF_FORM_DIVERGENCE || ((3 * cfg->Nx + 1) / 2) % 2 != 0))
+        return fail(CF_ERR_UNSUPPORTED, "mixed precision needs an even padded Nx and the rotating or convective form");
     if (y_group_lanes(cfg->Ny) == 0) return fail(CF_ERR_UNSUPPORTED, "Ny above 255 is not supported");
     // device indexing: mode counts and padded line counts in 32 bits
     if ((long long)(cfg->Nx / 2 + 1) * cfg->Nz * cfg->Ny >= (1ll << 31) ||
@@ -299,6 +306,23 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         xa.ppt = h->xppt;
         h->xwaves = x_max_waves(xa);
         if (h->xwaves < 1) return bail(fail(CF_ERR_UNSUPPORTED, "streamwise line exceeds LDS"));
+        if (cfg->precision == CF_PRECISION_MIXED) {
+            xa.mixed = 1;
+            xa.ppt = 0;
+            h->xppt32 = x_inplace_ppt(xa);
+            xa.ppt = h->xppt32;
+            h->xwaves32 = x_max_waves(xa);
+        }
+    }
+    if (cfg->precision == CF_PRECISION_MIXED) {
+        // FP32 z tiles: twice the k per ~32 KB buffer
+        h->KT32 = (int)std::max<size_t>(1, std::min<size_t>(32, (32 * 1024) / ((size_t)h->zld * sizeof(float2))));
+        h->KT32 = std::min(h->KT32, gd.Nxh);
+        while (h->KT32 > 1 && !z_inplace_ppt(h->pz, h->KT32)) --h->KT32;
+        h->zppt32 = z_inplace_ppt(h->pz, h->KT32);
+        if (!h->zppt32 || !h->xppt32 || h->xwaves32 < 1)
+            return bail(fail(CF_ERR_UNSUPPORTED, "mixed precision needs FFT lengths with radices 2, 3, 4, 5 and 8"));
+        h->mixed = 1;
     }
     // workspaces
     if ((rc = dalloc(h, &h->six, 6 * (size_t)gd.S))) return bail(rc);
@@ -360,17 +384,18 @@ static YArgs base_yargs(cf_problem* h) {
 static double2* fld(double2* base, long long n, int i) { return base + (size_t)n * i; }
 
 static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* const* out, hipStream_t st,
-                 unsigned dz_mask = 0) {
+                 unsigned dz_mask = 0, bool mixed = false) {
     ZArgs z;
     std::memset(&z, 0, sizeof(z));
     z.gd = h->gd;
     z.plan = h->pz;
-    z.KT = h->KT;
+    z.KT = mixed ? h->KT32 : h->KT;
     z.ld = h->zld;
     z.nfields = nf;
-    z.ppt = h->zppt;
+    z.ppt = mixed ? h->zppt32 : h->zppt;
+    z.mixed = mixed ? 1 : 0;
     z.dz_mask = dz_mask;
-    z.mul_kt = recip_mul((unsigned)h->KT);
+    z.mul_kt = recip_mul((unsigned)z.KT);
     for (int i = 0; i < nf; ++i) {
         z.in[i] = in[i];
         z.out[i] = out[i];
@@ -379,7 +404,7 @@ static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* co
     return 0;
 }
 
-static XArgs base_xargs(cf_problem* h) {
+static XArgs base_xargs(cf_problem* h, bool mixed = false) {
     XArgs x;
     std::memset(&x, 0, sizeof(x));
     x.gd = h->gd;
@@ -387,8 +412,9 @@ static XArgs base_xargs(cf_problem* h) {
     x.twx = h->d_twx;
     x.LPB = h->LPB;
     x.ld = h->xld;
-    x.waves = h->xwaves;
-    x.ppt = h->xppt;
+    x.waves = mixed ? h->xwaves32 : h->xwaves;
+    x.ppt = mixed ? h->xppt32 : h->xppt;
+    x.mixed = mixed ? 1 : 0;
     x.mul_m = recip_mul((unsigned)std::max(1, h->gd.Nxp / 2));
     x.mul_nxh = recip_mul((unsigned)h->gd.Nxh);
     x.mul_nx = recip_mul((unsigned)(std::max(1, h->gd.Nxp / 2) + 1));
@@ -400,23 +426,30 @@ static XArgs base_xargs(cf_problem* h) {
 // the six flux products V_i V_j in six[0..5] (divergence)
 struct Timer;
 static int tmark(Timer* tm, int c, hipStream_t st);
-static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st, Timer* tm = nullptr) {
+// y-physical field i of the six buffer in the step's storage (mixed: FP32
+// complex fields packed contiguously, S elements apart)
+static double2* six_field(cf_problem* h, int i, bool mixed) {
+    return mixed ? (double2*)((float2*)h->six + (size_t)h->gd.S * i) : fld(h->six, h->gd.S, i);
+}
+
+// mixed: FP32 complex fields (z buffer fields Sz elements apart)
+static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st, Timer* tm = nullptr, bool mixed = false) {
     const GridDims& gd = h->gd;
     double2* six[6];
     double2* zb[9];
-    for (int i = 0; i < 6; ++i) six[i] = fld(h->six, gd.S, i);
+    for (int i = 0; i < 6; ++i) six[i] = six_field(h, i, mixed);
     const int nzb = h->form == CF_FORM_CONVECTIVE ? 9 : 6;
-    for (int i = 0; i < nzb; ++i) zb[i] = fld(h->zbuf, gd.Sz, i);
-    XArgs x = base_xargs(h);
+    for (int i = 0; i < nzb; ++i) zb[i] = mixed ? (double2*)((float2*)h->zbuf + (size_t)gd.Sz * i) : fld(h->zbuf, gd.Sz, i);
+    XArgs x = base_xargs(h, mixed);
     if (h->form == CF_FORM_ROTATING) {
-        if (int rc = zpass(h, +1, 6, six, zb, st)) return rc;
+        if (int rc = zpass(h, +1, 6, six, zb, st, 0, mixed)) return rc;
         x.mode = X_ROTATIONAL;
         for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
         for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a wavefront reads its line first
     } else if (h->form == CF_FORM_CONVECTIVE) {
         // V, dV/dy and (from V again) dV/dz
         double2* zin[9] = {six[0], six[1], six[2], six[3], six[4], six[5], six[0], six[1], six[2]};
-        if (int rc = zpass(h, +1, 9, zin, zb, st, 0x1C0u)) return rc;
+        if (int rc = zpass(h, +1, 9, zin, zb, st, 0x1C0u, mixed)) return rc;
         x.mode = X_CONVECTIVE;
         for (int i = 0; i < 9; ++i) x.in[i] = zb[i];
         for (int i = 0; i < 3; ++i) x.out[i] = zb[i];
@@ -430,7 +463,7 @@ static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st, Timer* tm = nul
     HIPTRY(launch_xpass(x, st));
     if (int rc = tmark(tm, 2, st)) return rc;
     const int nout = h->form == CF_FORM_DIVERGENCE ? 6 : 3;
-    if (int rc = zpass(h, -1, nout, zb, six, st)) return rc;
+    if (int rc = zpass(h, -1, nout, zb, six, st, 0, mixed)) return rc;
     if (int rc = tmark(tm, 3, st)) return rc;
     return 0;
 }
@@ -445,10 +478,11 @@ static int enqueue_nonlinear_coef(cf_problem* h, double2* const Ncoef[3], hipStr
     return 0;
 }
 
-static int enqueue_prologue(cf_problem* h, double2* const U[3], hipStream_t st) {
+static int enqueue_prologue(cf_problem* h, double2* const U[3], hipStream_t st, bool mixed = false) {
     YArgs a = base_yargs(h);
+    a.mixed = mixed ? 1 : 0;
     for (int i = 0; i < 3; ++i) a.U[i] = U[i];
-    for (int i = 0; i < 6; ++i) a.out6[i] = fld(h->six, h->gd.S, i);
+    for (int i = 0; i < 6; ++i) a.out6[i] = six_field(h, i, mixed);
     HIPTRY(launch_ykernel(Y_PROLOGUE, a, st));
     return 0;
 }
@@ -508,17 +542,19 @@ static int tmark(Timer* tm, int c, hipStream_t st) { return tm ? tm->mark(c, st)
 static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
                          Timer* tm = nullptr, double dt = 0.0, DevBank* const* banks = nullptr) {
     const GridDims& gd = h->gd;
+    const bool mixed = h->mixed != 0;
     if (!(dt > 0.0)) dt = h->dt;
     for (int step = 0; step < nsteps; ++step) {
         if (int rc = tmark(tm, -1, st)) return rc;
-        if (int rc = enqueue_prologue(h, U, st)) return rc;
+        if (int rc = enqueue_prologue(h, U, st, mixed)) return rc;
         if (int rc = tmark(tm, 0, st)) return rc;
         for (int j = 0; j < 3; ++j) {
-            if (int rc = enqueue_nonlinear_phys(h, st, tm)) return rc;
+            if (int rc = enqueue_nonlinear_phys(h, st, tm, mixed)) return rc;
             double2* Ncoef[3] = {fld(h->R, gd.S, 0), fld(h->R, gd.S, 1), fld(h->R, gd.S, 2)};
             if (h->form == CF_FORM_DIVERGENCE)
                 if (int rc = enqueue_nonlinear_coef(h, Ncoef, st)) return rc;
             YArgs a = base_yargs(h);
+            a.mixed = mixed ? 1 : 0;
             a.nl_sign = (h->form == CF_FORM_CONVECTIVE) ? -1.0 : 1.0;
             a.stage = j;
             a.store_Q = (j < 2);
@@ -530,7 +566,7 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
             a.c1 = c.c1;
             a.bank = (banks ? banks[j] : &h->stage[j])->view();
             for (int i = 0; i < 3; ++i) {
-                a.Nphys[i] = fld(h->six, gd.S, i);
+                a.Nphys[i] = six_field(h, i, mixed);
                 a.Ncoef[i] = (h->form == CF_FORM_DIVERGENCE) ? Ncoef[i] : nullptr;
                 a.U[i] = U[i];
                 a.Q[i] = fld(h->Q, gd.S, i);
@@ -538,7 +574,7 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
             a.P = P;
             a.grad_out = h->d_grad;
             if (j < 2)
-                for (int i = 0; i < 6; ++i) a.out6[i] = fld(h->six, gd.S, i);
+                for (int i = 0; i < 6; ++i) a.out6[i] = six_field(h, i, mixed);
             HIPTRY(launch_ykernel(Y_STAGE, a, st));
             if (int rc = tmark(tm, 4, st)) return rc;
         }

@@ -1,6 +1,6 @@
 // Device-side building blocks for the CNRK2 step on gfx950 (CDNA4, wave64).
 //
-//  * complex FP64 arithmetic on double2
+//  * complex FP64 arithmetic on double2 (and FP32 on float2 for the mixed step)
 //  * lane-group cross-lane primitives (DPP / permlane) and scans used by the
 //    wall-normal (Chebyshev) recurrences: one mode is held by a group of L
 //    lanes, lane t owning coefficients n = 4t .. 4t+3
@@ -26,6 +26,37 @@ __device__ __forceinline__ double2 cmuli(double2 a) { return cmk(-a.y, a.x); }  
 __device__ __forceinline__ double2 cmulni(double2 a) { return cmk(a.y, -a.x); }   // -i*a
 __device__ __forceinline__ double2 cfma(double s, double2 a, double2 b) { return cmk(fma(s, a.x, b.x), fma(s, a.y, b.y)); }
 __device__ __forceinline__ double2 czero() { return cmk(0.0, 0.0); }
+
+// FP32 complex (the y-physical / periodic-pass storage of the mixed step)
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cmuli(float2 a) { return make_float2(-a.y, a.x); }
+__device__ __forceinline__ float2 cmulni(float2 a) { return make_float2(a.y, -a.x); }
+__device__ __forceinline__ float2 cfma(float s, float2 a, float2 b) { return make_float2(fmaf(s, a.x, b.x), fmaf(s, a.y, b.y)); }
+
+// complex element type C (double2 or float2) -> its real type; construction
+template <typename C> struct cplx_traits;
+template <> struct cplx_traits<double2> { using real = double; };
+template <> struct cplx_traits<float2> { using real = float; };
+template <typename C> using creal_t = typename cplx_traits<C>::real;
+template <typename C>
+__device__ __forceinline__ C cplx(creal_t<C> r, creal_t<C> i) {
+    C z;
+    z.x = r;
+    z.y = i;
+    return z;
+}
+// storage conversions (round to nearest when narrowing)
+__device__ __forceinline__ double2 to_c64(double2 a) { return a; }
+__device__ __forceinline__ double2 to_c64(float2 a) { return make_double2((double)a.x, (double)a.y); }
+template <typename C> __device__ __forceinline__ C from_c64(double2 a);
+template <> __device__ __forceinline__ double2 from_c64<double2>(double2 a) { return a; }
+template <> __device__ __forceinline__ float2 from_c64<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
 
 // ---------------------------------------------------------------------------
 // cross-lane moves.  A wave64 is four DPP rows of 16 lanes.  Within a row:
@@ -496,58 +527,59 @@ __device__ __forceinline__ CVec helm_solve(const Group<L>& g, int P, double t1, 
 // result ends in *a or *b (returned).  sign = -1 forward, +1 backward.
 // tw[t] = exp(-2 pi i t / n), t < n (global memory, read-only).
 // ---------------------------------------------------------------------------
-template <int R>
-__device__ __forceinline__ void dft_small(double2* v, int sign) {
+template <int R, typename C>
+__device__ __forceinline__ void dft_small(C* v, int sign) {
+    using T = creal_t<C>;
     if constexpr (R == 2) {
-        double2 a = v[0], b = v[1];
+        C a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else if constexpr (R == 3) {
-        const double c = -0.5, s = 0.86602540378443864676;  // cos(2pi/3), sin(2pi/3)
-        double2 a = v[0], b = v[1], d = v[2];
-        double2 t = cadd(b, d), u = csub(b, d);
-        double2 m = cfma(c, t, a);
-        double2 su = cscale(u, sign * s);  // forward: v1 = m - i s u
-        double2 isu = cmuli(su);
+        const T c = T(-0.5), s = T(0.86602540378443864676);  // cos(2pi/3), sin(2pi/3)
+        C a = v[0], b = v[1], d = v[2];
+        C t = cadd(b, d), u = csub(b, d);
+        C m = cfma(c, t, a);
+        C su = cscale(u, sign < 0 ? -s : s);  // forward: v1 = m - i s u
+        C isu = cmuli(su);
         v[0] = cadd(a, t);
         v[1] = cadd(m, isu);
         v[2] = csub(m, isu);
     } else if constexpr (R == 4) {
-        double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        double2 t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
-        double2 jt3 = (sign < 0) ? cmulni(t3) : cmuli(t3);
+        C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        C t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+        C jt3 = (sign < 0) ? cmulni(t3) : cmuli(t3);
         v[0] = cadd(t0, t2);
         v[1] = cadd(t1, jt3);
         v[2] = csub(t0, t2);
         v[3] = csub(t1, jt3);
     } else if constexpr (R == 5) {
-        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-        double2 a = v[0];
-        double2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
-        double2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
-        double2 m1 = cadd(a, cadd(cscale(b1, c1), cscale(b2, c2)));
-        double2 m2 = cadd(a, cadd(cscale(b1, c2), cscale(b2, c1)));
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+        C a = v[0];
+        C b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+        C d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+        C m1 = cadd(a, cadd(cscale(b1, c1), cscale(b2, c2)));
+        C m2 = cadd(a, cadd(cscale(b1, c2), cscale(b2, c1)));
         // forward: -i*(s1 d1 + s2 d2), -i*(s2 d1 - s1 d2)
-        double2 n1 = cadd(cscale(d1, s1), cscale(d2, s2));
-        double2 n2 = csub(cscale(d1, s2), cscale(d2, s1));
-        double2 j1 = (sign < 0) ? cmulni(n1) : cmuli(n1);
-        double2 j2 = (sign < 0) ? cmulni(n2) : cmuli(n2);
+        C n1 = cadd(cscale(d1, s1), cscale(d2, s2));
+        C n2 = csub(cscale(d1, s2), cscale(d2, s1));
+        C j1 = (sign < 0) ? cmulni(n1) : cmuli(n1);
+        C j2 = (sign < 0) ? cmulni(n2) : cmuli(n2);
         v[0] = cadd(a, cadd(b1, b2));
         v[1] = cadd(m1, j1);
         v[4] = csub(m1, j1);
         v[2] = cadd(m2, j2);
         v[3] = csub(m2, j2);
     } else if constexpr (R == 8) {
-        const double h = 0.70710678118654752440;
+        const T h = T(0.70710678118654752440);
         // radix-8 = 2 x radix-4 with twiddles
-        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
         dft_small<4>(e, sign);
         dft_small<4>(o, sign);
         // twiddles W8^k, k=0..3 (forward W8 = exp(-i pi/4))
-        double2 w1 = cmk(h, sign < 0 ? -h : h);
-        double2 w2 = cmk(0.0, sign < 0 ? -1.0 : 1.0);
-        double2 w3 = cmk(-h, sign < 0 ? -h : h);
+        C w1 = cplx<C>(h, sign < 0 ? -h : h);
+        C w2 = cplx<C>(T(0), sign < 0 ? T(-1) : T(1));
+        C w3 = cplx<C>(-h, sign < 0 ? -h : h);
         o[1] = cmul(o[1], w1);
         o[2] = cmul(o[2], w2);
         o[3] = cmul(o[3], w3);
@@ -559,10 +591,10 @@ __device__ __forceinline__ void dft_small(double2* v, int sign) {
     }
 }
 
-template <int R>
-__device__ __forceinline__ void stockham_pass_fixed(const double2* src, double2* dst, int n, int nlines, int ld, int Ns,
+template <int R, typename C>
+__device__ __forceinline__ void stockham_pass_fixed(const C* src, C* dst, int n, int nlines, int ld, int Ns,
                                                     unsigned long long mul_m, unsigned long long mul_ns,
-                                                    const double2* tw, int sign, int tid, int nthr) {
+                                                    const C* tw, int sign, int tid, int nthr) {
     const int m = n / R;
     const int twstep = n / (Ns * R);
     const int total = nlines * m;
@@ -571,14 +603,14 @@ __device__ __forceinline__ void stockham_pass_fixed(const double2* src, double2*
         int j = idx - line * m;
         int jq = (int)fast_div((unsigned)j, mul_ns);
         int jm = j - jq * Ns;
-        const double2* s = src + (size_t)line * ld;
-        double2* d = dst + (size_t)line * ld;
-        double2 v[R];
+        const C* s = src + (size_t)line * ld;
+        C* d = dst + (size_t)line * ld;
+        C v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            double2 x = s[j + r * m];
+            C x = s[j + r * m];
             if (r > 0) {
-                double2 w = tw[r * jm * twstep];  // r*jm*twstep < n
+                C w = tw[r * jm * twstep];  // r*jm*twstep < n
                 if (sign > 0) w = cconj(w);
                 x = cmul(x, w);
             }
@@ -592,26 +624,27 @@ __device__ __forceinline__ void stockham_pass_fixed(const double2* src, double2*
 }
 
 // generic odd radix (rare sizes): O(R^2) per butterfly, reads LDS directly
-__device__ __noinline__ void stockham_pass_generic(const double2* src, double2* dst, int n, int R, int nlines, int ld,
-                                                      int Ns, const double2* tw, int sign, int tid, int nthr) {
+template <typename C>
+__device__ __noinline__ void stockham_pass_generic(const C* src, C* dst, int n, int R, int nlines, int ld, int Ns,
+                                                   const C* tw, int sign, int tid, int nthr) {
     const int m = n / R;
     const int twstep = n / (Ns * R);
     const int total = nlines * m;
     for (int idx = tid; idx < total; idx += nthr) {
         int line = idx / m;
         int j = idx - line * m;
-        const double2* s = src + (size_t)line * ld;
-        double2* d = dst + (size_t)line * ld;
+        const C* s = src + (size_t)line * ld;
+        C* d = dst + (size_t)line * ld;
         int jm = j % Ns;
         int idxD = (j / Ns) * Ns * R + jm;
         for (int q = 0; q < R; ++q) {
-            double2 acc = czero();
+            C acc = cplx<C>(0, 0);
             for (int r = 0; r < R; ++r) {
-                double2 x = s[j + r * m];
+                C x = s[j + r * m];
                 // twiddle (r * jm / (Ns R)) and DFT_R phase (r q / R), combined:
                 // exp(-2 pi i r (jm + q Ns) / (Ns R))
                 int t = (int)(((long long)r * (jm + (long long)q * Ns) * twstep) % n);
-                double2 w = tw[t];
+                C w = tw[t];
                 if (sign > 0) w = cconj(w);
                 acc = cadd(acc, cmul(x, w));
             }
@@ -634,12 +667,12 @@ __device__ __forceinline__ void wave_sync() {
 // (passes end in __syncthreads); WAVE = true: the lines are private to one
 // wavefront, tid/nthr are its lane/64 and passes end in wave_sync().
 // tw: twiddle table exp(-2 pi i t/n) (global or LDS).
-template <bool WAVE = false>
-__device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, const double2* tw, int nlines, int ld,
-                                            int sign, int tid, int nthr) {
+template <bool WAVE = false, typename C>
+__device__ __forceinline__ C* lds_fft(C* a, C* b, const FFTPlan& plan, const C* tw, int nlines, int ld, int sign, int tid,
+                                      int nthr) {
     int Ns = 1;
-    double2* src = a;
-    double2* dst = b;
+    C* src = a;
+    C* dst = b;
     for (int f = 0; f < plan.nf; ++f) {
         int R = plan.radix[f];
         const unsigned long long mm = plan.mul_m[f], mn = plan.mul_ns[f];
@@ -654,7 +687,7 @@ __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPla
         if constexpr (WAVE) wave_sync();
         else __syncthreads();
         Ns *= R;
-        double2* t = src;
+        C* t = src;
         src = dst;
         dst = t;
     }
@@ -665,15 +698,15 @@ __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPla
 // bounds the points per thread (nlines * n <= PPT * nthr), so a thread holds
 // at most ceil(PPT / R) butterflies of radix R.  The plan must contain only
 // radices 2, 3, 4, 5 and 8 (fft_inplace_fits).
-template <int R, int PPT, bool WAVE>
-__device__ __forceinline__ void stockham_pass_inplace(double2* buf, int n, int nlines, int ld, int Ns,
-                                                      unsigned long long mul_m, unsigned long long mul_ns,
-                                                      const double2* tw, int sign, int tid, int nthr) {
+template <int R, int PPT, bool WAVE, typename C>
+__device__ __forceinline__ void stockham_pass_inplace(C* buf, int n, int nlines, int ld, int Ns,
+                                                      unsigned long long mul_m, unsigned long long mul_ns, const C* tw,
+                                                      int sign, int tid, int nthr) {
     constexpr int NB = (PPT + R - 1) / R;
     const int m = n / R;
     const int twstep = n / (Ns * R);
     const int total = nlines * m;
-    double2 v[NB][R];
+    C v[NB][R];
     int dst[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -684,12 +717,12 @@ __device__ __forceinline__ void stockham_pass_inplace(double2* buf, int n, int n
             int j = idx - line * m;
             int jq = (int)fast_div((unsigned)j, mul_ns);
             int jm = j - jq * Ns;
-            const double2* s = buf + (size_t)line * ld;
+            const C* s = buf + (size_t)line * ld;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                double2 x = s[j + r * m];
+                C x = s[j + r * m];
                 if (r > 0) {
-                    double2 w = tw[r * jm * twstep];
+                    C w = tw[r * jm * twstep];
                     if (sign > 0) w = cconj(w);
                     x = cmul(x, w);
                 }
@@ -711,9 +744,9 @@ __device__ __forceinline__ void stockham_pass_inplace(double2* buf, int n, int n
     else __syncthreads();
 }
 
-template <int PPT, bool WAVE>
-__device__ __forceinline__ void lds_fft_inplace(double2* buf, const FFTPlan& plan, const double2* tw, int nlines, int ld,
-                                                int sign, int tid, int nthr) {
+template <int PPT, bool WAVE, typename C>
+__device__ __forceinline__ void lds_fft_inplace(C* buf, const FFTPlan& plan, const C* tw, int nlines, int ld, int sign,
+                                                int tid, int nthr) {
     int Ns = 1;
     for (int f = 0; f < plan.nf; ++f) {
         const int R = plan.radix[f];
@@ -739,7 +772,7 @@ __host__ __device__ inline bool fft_inplace_fits(const FFTPlan& plan, int nlines
 
 __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
                                             int nthr) {
-    return lds_fft<false>(a, b, plan, plan.tw, nlines, ld, sign, tid, nthr);
+    return lds_fft<false, double2>(a, b, plan, plan.tw, nlines, ld, sign, tid, nthr);
 }
 
 }  // namespace cf

@@ -41,6 +41,7 @@ struct YArgs {
     int ntab;                // entries per array (= 4L of the launch)
     Bank bank;
     int form;                 // FORM_*: which y-physical fields the epilogue emits
+    int mixed;                // Y_STAGE / Y_PROLOGUE: Nphys and out6 hold FP32 complex (float2)
     double nl_sign;           // sign applied to the transformed nonlinear term
     const double2* Nphys[6];  // y-physical nonlinear term (Y_DIVERGENCE: the six products)
     const double2* Ncoef[3];  // nonlinear term already in coefficients (skip the DCT) or null
@@ -62,6 +63,7 @@ struct ZArgs {
     int ppt;           // > 0: in-place FFT in one LDS buffer with this many points per thread; 0: two buffers
     unsigned dz_mask;  // bit f: inverse pass multiplies field f by i*beta*l (z derivative)
     unsigned long long mul_kt;  // recip_mul(KT): index splits without integer division
+    int mixed;                  // 1: fields are FP32 complex (float2) in the same buffers; in-place plans only
     const double2* in[9];
     double2* out[9];
 };
@@ -76,6 +78,7 @@ struct XArgs {
     int waves;             // even Nxp: wavefronts (lines) per block, 1..4
     int ppt;               // even Nxp: in-place FFT points per lane (one LDS buffer), 0: two buffers
     unsigned long long mul_m, mul_nxh, mul_nx;  // recip_mul of Nxp/2, Nxh and Nxp/2+1 (index splits)
+    int mixed;             // 1: in/out are FP32 complex (float2); even Nxp, in-place wave path only
     const double2* in[9];  // (k, zp, y) complex, Nxh per line
     double2* out[6];
     const double* rin[3];  // (x, zp, y) real

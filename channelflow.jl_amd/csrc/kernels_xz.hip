@@ -31,13 +31,17 @@ __device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned nblocks) {
 
 // z inverse (dir=+1): (k,l,y) -> (k,zp,y); z forward (dir=-1): (k,zp,y) -> (k,l,y)
 // 1-D grid of ntiles * Ny * nfields blocks, k tile fastest in logical order.
-template <int DIR, int PPT>
+// C = double2: FP64 fields; C = float2: FP32 fields of the mixed step (in-place
+// plans only, twiddles rounded from the FP64 table).
+template <int DIR, int PPT, typename C>
 __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
-    extern __shared__ double2 smem[];
+    using T = creal_t<C>;
+    extern __shared__ double2 smem_raw[];
+    C* smem = reinterpret_cast<C*>(smem_raw);
     const GridDims& gd = a.gd;
     const int KT = a.KT, ld = a.ld;
-    double2* bufA = smem;
-    double2* bufB = smem + KT * ld;
+    C* bufA = smem;
+    C* bufB = smem + KT * ld;
     const unsigned ntiles = (unsigned)((gd.Nxh + KT - 1) / KT);
     const unsigned lb = xcd_logical(blockIdx.x, gridDim.x);
     const unsigned tile = lb % ntiles, plane = lb / ntiles;
@@ -47,47 +51,50 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
     const int Nz = gd.Nz, Nzp = gd.Nzp, Nxh = gd.Nxh;
     const int npos = (Nz >> 1) + 1, nneg = Nz - npos;
     const bool xeven = (gd.Nx % 2 == 0), zeven = (Nz % 2 == 0);
+    const C zero = cplx<C>(T(0), T(0));
     if constexpr (PPT > 0)  // in-place mode: the second buffer holds the twiddle table
-        for (int i = tid; i < Nzp; i += ZNT) bufB[i] = a.plan.tw[i];
+        for (int i = tid; i < Nzp; i += ZNT) bufB[i] = from_c64<C>(a.plan.tw[i]);
     if (DIR > 0) {
-        const double2* in = a.in[f] + (size_t)Nxh * Nz * y;
+        const C* in = reinterpret_cast<const C*>(a.in[f]) + (size_t)Nxh * Nz * y;
         const bool dz = (a.dz_mask >> f) & 1u;  // emit d/dz: multiply by i beta l
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
             int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
             int k = k0 + kk;
-            double2 v = czero();
+            C v = zero;
             if (k < Nxh && !(xeven && k == gd.Nx / 2)) {
                 int l = -1;
                 if (z < npos) l = (zeven && z == Nz / 2) ? -1 : z;
                 else if (z >= Nzp - nneg) l = npos + (z - (Nzp - nneg));
                 if (l >= 0) {
                     v = in[k + (size_t)Nxh * l];
-                    if (dz) v = cmuli(cscale(v, gd.beta * (l <= Nz / 2 ? l : l - Nz)));
+                    if (dz) v = cmuli(cscale(v, T(gd.beta * (l <= Nz / 2 ? l : l - Nz))));
                 }
             }
             bufA[kk * ld + z] = v;
         }
     } else {
-        const double2* in = a.in[f] + (size_t)Nxh * Nzp * y;
+        const C* in = reinterpret_cast<const C*>(a.in[f]) + (size_t)Nxh * Nzp * y;
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
             int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
             int k = k0 + kk;
-            bufA[kk * ld + z] = (k < Nxh) ? in[k + (size_t)Nxh * z] : czero();
+            C v = zero;
+            if (k < Nxh) v = in[k + (size_t)Nxh * z];
+            bufA[kk * ld + z] = v;
         }
     }
     __syncthreads();
-    double2* res = bufA;
-    if constexpr (PPT > 0) lds_fft_inplace<PPT, false>(bufA, a.plan, bufB, KT, ld, DIR, tid, ZNT);  // bufB: twiddles
+    C* res = bufA;
+    if constexpr (PPT > 0) lds_fft_inplace<PPT, false>(bufA, a.plan, (const C*)bufB, KT, ld, DIR, tid, ZNT);  // bufB: twiddles
     else res = lds_fft(bufA, bufB, a.plan, KT, ld, DIR, tid, ZNT);
     if (DIR > 0) {
-        double2* out = a.out[f] + (size_t)Nxh * Nzp * y;
+        C* out = reinterpret_cast<C*>(a.out[f]) + (size_t)Nxh * Nzp * y;
         for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
             int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
             int k = k0 + kk;
             if (k < Nxh) out[k + (size_t)Nxh * z] = res[kk * ld + z];
         }
     } else {
-        double2* out = a.out[f] + (size_t)Nxh * Nz * y;
+        C* out = reinterpret_cast<C*>(a.out[f]) + (size_t)Nxh * Nz * y;
         for (int idx = tid; idx < KT * Nz; idx += ZNT) {
             int l = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - l * KT;
             int k = k0 + kk;
@@ -215,30 +222,33 @@ __host__ __device__ constexpr int x_region_slots(int mode, int ppt, int ld) { re
 
 // retained coefficient X_k of field f on line gl for the half-length inverse
 // transform (padding, Nyquist plane and Im of DC / of k = M removed)
-__device__ __forceinline__ double2 x_coef(const XArgs& a, int f, long long gl, int k, int M) {
-    double2 v = czero();
-    if (k < a.gd.Nxh && !((a.gd.Nx % 2 == 0) && k == a.gd.Nx / 2)) v = a.in[f][(size_t)gl * a.gd.Nxh + k];
-    if (k == 0 || k == M) v.y = 0.0;
+template <typename C>
+__device__ __forceinline__ C x_coef(const XArgs& a, int f, long long gl, int k, int M) {
+    C v = cplx<C>(0, 0);
+    if (k < a.gd.Nxh && !((a.gd.Nx % 2 == 0) && k == a.gd.Nx / 2))
+        v = reinterpret_cast<const C*>(a.in[f])[(size_t)gl * a.gd.Nxh + k];
+    if (k == 0 || k == M) v.y = 0;
     return v;
 }
 
-template <int MODE, int PPT>
-__device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
+template <int MODE, int PPT, typename C>
+__device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem) {
+    using T = creal_t<C>;
     constexpr int NIN = x_fields_in(MODE), NFFT = x_fft_lines_in(MODE), NOUT = x_fields_out(MODE);
     constexpr int NB = x_buffer_lines(MODE);
     const int Nxp = a.gd.Nxp, Nxh = a.gd.Nxh, M = Nxp / 2, ld = a.ld;
     const long long nlines = (long long)a.gd.Nzp * a.gd.Ny;
     const bool xeven = (a.gd.Nx % 2 == 0);
     // twiddles: FFT table (M) and pack/unpack table exp(-2 pi i k / Nxp), k <= M
-    double2* tw = smem;
-    double2* twx = smem + M;
-    for (int i = threadIdx.x; i < M; i += blockDim.x) tw[i] = a.plan.tw[i];
-    for (int i = threadIdx.x; i <= M; i += blockDim.x) twx[i] = a.twx[i];
+    C* tw = smem;
+    C* twx = smem + M;
+    for (int i = threadIdx.x; i < M; i += blockDim.x) tw[i] = from_c64<C>(a.plan.tw[i]);
+    for (int i = threadIdx.x; i <= M; i += blockDim.x) twx[i] = from_c64<C>(a.twx[i]);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long gl = (long long)blockIdx.x * a.waves + wave;
     if (gl >= nlines) return;  // whole wavefront leaves; no block barrier follows
-    double2* bufA = smem + 2 * M + 1 + (size_t)wave * x_region_slots(MODE, PPT, ld);
+    C* bufA = smem + 2 * M + 1 + (size_t)wave * x_region_slots(MODE, PPT, ld);
     if constexpr (PPT > 0) {
         // pack straight from HBM into the single buffer; FFT line q reads
         // field src(q), times i alpha k for the convective x derivatives
@@ -250,66 +260,67 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
                 dx = (q >= 3 && q < 6);
                 f = q < 3 ? q : q - 3;
             }
-            double2 Xk = x_coef(a, f, gl, k, M);
-            double2 Xm = x_coef(a, f, gl, M - k, M);
+            C Xk = x_coef<C>(a, f, gl, k, M);
+            C Xm = x_coef<C>(a, f, gl, M - k, M);
             if (dx) {
-                Xk = cmuli(cscale(Xk, a.gd.alpha * k));
-                Xm = cmuli(cscale(Xm, a.gd.alpha * (M - k)));
+                Xk = cmuli(cscale(Xk, T(a.gd.alpha * k)));
+                Xm = cmuli(cscale(Xm, T(a.gd.alpha * (M - k))));
             }
-            double2 Xc = cconj(Xm);
-            double2 w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
+            C Xc = cconj(Xm);
+            C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
             bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
         }
         wave_sync();
-        lds_fft_inplace<PPT, true>(bufA, a.plan, tw, NFFT, ld, +1, lane, 64);
+        lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, NFFT, ld, +1, lane, 64);
         // products at x = 2j (real parts) and 2j+1 (imaginary parts), in place:
         // each lane reads all lines at its j before writing them
         for (int j = lane; j < M; j += 64) {
             if constexpr (MODE == X_ROTATIONAL) {
-                double2 u0 = bufA[0 * ld + j], u1 = bufA[1 * ld + j], u2 = bufA[2 * ld + j];
-                double2 w0 = bufA[3 * ld + j], w1 = bufA[4 * ld + j], w2 = bufA[5 * ld + j];
-                bufA[0 * ld + j] = cmk(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
-                bufA[1 * ld + j] = cmk(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
-                bufA[2 * ld + j] = cmk(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+                C u0 = bufA[0 * ld + j], u1 = bufA[1 * ld + j], u2 = bufA[2 * ld + j];
+                C w0 = bufA[3 * ld + j], w1 = bufA[4 * ld + j], w2 = bufA[5 * ld + j];
+                bufA[0 * ld + j] = cplx<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+                bufA[1 * ld + j] = cplx<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+                bufA[2 * ld + j] = cplx<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
             } else if constexpr (MODE == X_CONVECTIVE) {
-                double2 u = bufA[0 * ld + j], v = bufA[1 * ld + j], w = bufA[2 * ld + j];
-                double2 n[3];
+                C u = bufA[0 * ld + j], v = bufA[1 * ld + j], w = bufA[2 * ld + j];
+                C n[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    double2 gx = bufA[(3 + i) * ld + j], gy = bufA[(6 + i) * ld + j], gz = bufA[(9 + i) * ld + j];
-                    n[i] = cmk(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
+                    C gx = bufA[(3 + i) * ld + j], gy = bufA[(6 + i) * ld + j], gz = bufA[(9 + i) * ld + j];
+                    n[i] = cplx<C>(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
                 }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) bufA[i * ld + j] = n[i];
             } else {
-                double2 u = bufA[0 * ld + j], v = bufA[1 * ld + j], w = bufA[2 * ld + j];
-                bufA[0 * ld + j] = cmk(u.x * u.x, u.y * u.y);
-                bufA[1 * ld + j] = cmk(u.x * v.x, u.y * v.y);
-                bufA[2 * ld + j] = cmk(u.x * w.x, u.y * w.y);
-                bufA[3 * ld + j] = cmk(v.x * v.x, v.y * v.y);
-                bufA[4 * ld + j] = cmk(v.x * w.x, v.y * w.y);
-                bufA[5 * ld + j] = cmk(w.x * w.x, w.y * w.y);
+                C u = bufA[0 * ld + j], v = bufA[1 * ld + j], w = bufA[2 * ld + j];
+                bufA[0 * ld + j] = cplx<C>(u.x * u.x, u.y * u.y);
+                bufA[1 * ld + j] = cplx<C>(u.x * v.x, u.y * v.y);
+                bufA[2 * ld + j] = cplx<C>(u.x * w.x, u.y * w.y);
+                bufA[3 * ld + j] = cplx<C>(v.x * v.x, v.y * v.y);
+                bufA[4 * ld + j] = cplx<C>(v.x * w.x, v.y * w.y);
+                bufA[5 * ld + j] = cplx<C>(w.x * w.x, w.y * w.y);
             }
         }
         wave_sync();
-        lds_fft_inplace<PPT, true>(bufA, a.plan, tw, NOUT, ld, -1, lane, 64);
+        lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, NOUT, ld, -1, lane, 64);
         for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
             int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
-            double2 Zk = bufA[c * ld + k];
-            double2 Zc = cconj(bufA[c * ld + ((M - k) % M)]);
-            double2 w = twx[k];  // exp(-2 pi i k / Nxp)
-            a.out[c][(size_t)gl * Nxh + k] = csub(cscale(cadd(Zk, Zc), 0.5), cscale(cmuli(cmul(w, csub(Zk, Zc))), 0.5));
+            C Zk = bufA[c * ld + k];
+            C Zc = cconj(bufA[c * ld + ((M - k) % M)]);
+            C w = twx[k];  // exp(-2 pi i k / Nxp)
+            reinterpret_cast<C*>(a.out[c])[(size_t)gl * Nxh + k] =
+                csub(cscale(cadd(Zk, Zc), T(0.5)), cscale(cmuli(cmul(w, csub(Zk, Zc))), T(0.5)));
         }
         return;
     }
-    double2* bufB = bufA + NB * ld;
+    C* bufB = bufA + NB * ld;
     const int nX = M + 1;
     // stage X_k, k = 0..M (padding, Nyquist plane and Im of DC/Nyquist removed)
     for (int idx = lane; idx < NIN * nX; idx += 64) {
         int f = (int)fast_div((unsigned)idx, a.mul_nx), k = idx - f * nX;
-        double2 v = czero();
-        if (k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = a.in[f][(size_t)gl * Nxh + k];
-        if (k == 0 || k == M) v.y = 0.0;
+        C v = cplx<C>(0, 0);
+        if (k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = reinterpret_cast<const C*>(a.in[f])[(size_t)gl * Nxh + k];
+        if (k == 0 || k == M) v.y = 0;
         bufB[f * ld + k] = v;
     }
     wave_sync();
@@ -323,60 +334,61 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, double2* smem) {
             dx = (q >= 3 && q < 6);
             f = q < 3 ? q : (q < 6 ? q - 3 : q - 3);
         }
-        double2 Xk = bufB[f * ld + k];
-        double2 Xm = bufB[f * ld + (M - k)];
+        C Xk = bufB[f * ld + k];
+        C Xm = bufB[f * ld + (M - k)];
         if (dx) {
-            Xk = cmuli(cscale(Xk, a.gd.alpha * k));
-            Xm = cmuli(cscale(Xm, a.gd.alpha * (M - k)));
+            Xk = cmuli(cscale(Xk, T(a.gd.alpha * k)));
+            Xm = cmuli(cscale(Xm, T(a.gd.alpha * (M - k))));
         }
-        double2 Xc = cconj(Xm);
-        double2 w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
+        C Xc = cconj(Xm);
+        C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
         bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
     }
     wave_sync();
-    double2* res = lds_fft<true>(bufA, bufB, a.plan, tw, NFFT, ld, +1, lane, 64);
-    double2* nxt = (res == bufA) ? bufB : bufA;
+    C* res = lds_fft<true>(bufA, bufB, a.plan, (const C*)tw, NFFT, ld, +1, lane, 64);
+    C* nxt = (res == bufA) ? bufB : bufA;
     // pointwise products at x = 2j (real parts) and x = 2j+1 (imaginary parts)
     for (int j = lane; j < M; j += 64) {
         if constexpr (MODE == X_ROTATIONAL) {
-            double2 u0 = res[0 * ld + j], u1 = res[1 * ld + j], u2 = res[2 * ld + j];
-            double2 w0 = res[3 * ld + j], w1 = res[4 * ld + j], w2 = res[5 * ld + j];
-            nxt[0 * ld + j] = cmk(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
-            nxt[1 * ld + j] = cmk(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
-            nxt[2 * ld + j] = cmk(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+            C u0 = res[0 * ld + j], u1 = res[1 * ld + j], u2 = res[2 * ld + j];
+            C w0 = res[3 * ld + j], w1 = res[4 * ld + j], w2 = res[5 * ld + j];
+            nxt[0 * ld + j] = cplx<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+            nxt[1 * ld + j] = cplx<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+            nxt[2 * ld + j] = cplx<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
         } else if constexpr (MODE == X_CONVECTIVE) {
-            double2 u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
+            C u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                double2 gx = res[(3 + i) * ld + j], gy = res[(6 + i) * ld + j], gz = res[(9 + i) * ld + j];
-                nxt[i * ld + j] = cmk(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
+                C gx = res[(3 + i) * ld + j], gy = res[(6 + i) * ld + j], gz = res[(9 + i) * ld + j];
+                nxt[i * ld + j] = cplx<C>(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
             }
         } else {
-            double2 u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
-            nxt[0 * ld + j] = cmk(u.x * u.x, u.y * u.y);
-            nxt[1 * ld + j] = cmk(u.x * v.x, u.y * v.y);
-            nxt[2 * ld + j] = cmk(u.x * w.x, u.y * w.y);
-            nxt[3 * ld + j] = cmk(v.x * v.x, v.y * v.y);
-            nxt[4 * ld + j] = cmk(v.x * w.x, v.y * w.y);
-            nxt[5 * ld + j] = cmk(w.x * w.x, w.y * w.y);
+            C u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
+            nxt[0 * ld + j] = cplx<C>(u.x * u.x, u.y * u.y);
+            nxt[1 * ld + j] = cplx<C>(u.x * v.x, u.y * v.y);
+            nxt[2 * ld + j] = cplx<C>(u.x * w.x, u.y * w.y);
+            nxt[3 * ld + j] = cplx<C>(v.x * v.x, v.y * v.y);
+            nxt[4 * ld + j] = cplx<C>(v.x * w.x, v.y * w.y);
+            nxt[5 * ld + j] = cplx<C>(w.x * w.x, w.y * w.y);
         }
     }
     wave_sync();
-    double2* r2 = lds_fft<true>(nxt, res, a.plan, tw, NOUT, ld, -1, lane, 64);
+    C* r2 = lds_fft<true>(nxt, res, a.plan, (const C*)tw, NOUT, ld, -1, lane, 64);
     for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
         int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
-        double2 Zk = r2[c * ld + k];
-        double2 Zc = cconj(r2[c * ld + ((M - k) % M)]);
-        double2 w = twx[k];  // exp(-2 pi i k / Nxp)
-        a.out[c][(size_t)gl * Nxh + k] = csub(cscale(cadd(Zk, Zc), 0.5), cscale(cmuli(cmul(w, csub(Zk, Zc))), 0.5));
+        C Zk = r2[c * ld + k];
+        C Zc = cconj(r2[c * ld + ((M - k) % M)]);
+        C w = twx[k];  // exp(-2 pi i k / Nxp)
+        reinterpret_cast<C*>(a.out[c])[(size_t)gl * Nxh + k] =
+            csub(cscale(cadd(Zk, Zc), T(0.5)), cscale(cmuli(cmul(w, csub(Zk, Zc))), T(0.5)));
     }
 }
 
 // one kernel per (form, in-place width) so each gets its own register budget
-template <int MODE, int PPT>
+template <int MODE, int PPT, typename C>
 __global__ void __launch_bounds__(XNT) xwave_kernel(XArgs a) {
     extern __shared__ double2 smem[];
-    xpass_wave<MODE, PPT>(a, smem);
+    xpass_wave<MODE, PPT>(a, reinterpret_cast<C*>(smem));
 }
 
 __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
@@ -449,7 +461,8 @@ __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
 
 size_t z_lds_bytes(const ZArgs& a) {
     // in place: one buffer plus the twiddle table (Nzp entries); else two buffers
-    return (a.ppt > 0 ? (size_t)a.KT * a.ld + a.gd.Nzp : 2 * (size_t)a.KT * a.ld) * sizeof(double2);
+    const size_t elem = a.mixed ? sizeof(float2) : sizeof(double2);
+    return (a.ppt > 0 ? (size_t)a.KT * a.ld + a.gd.Nzp : 2 * (size_t)a.KT * a.ld) * elem;
 }
 
 // in-place instantiations (points per thread); the host picks the smallest that fits
@@ -461,15 +474,21 @@ int z_inplace_ppt(const FFTPlan& plan, int KT) {
 
 template <int DIR>
 static void launch_z(const ZArgs& a, dim3 grid, size_t lds, hipStream_t st) {
+    if (a.mixed) {
+        if (a.ppt == 8) hipLaunchKernelGGL((zpass_kernel<DIR, 8, float2>), grid, dim3(ZNT), lds, st, a);
+        else hipLaunchKernelGGL((zpass_kernel<DIR, 16, float2>), grid, dim3(ZNT), lds, st, a);
+        return;
+    }
     switch (a.ppt) {
-        case 8: hipLaunchKernelGGL((zpass_kernel<DIR, 8>), grid, dim3(ZNT), lds, st, a); break;
-        case 16: hipLaunchKernelGGL((zpass_kernel<DIR, 16>), grid, dim3(ZNT), lds, st, a); break;
-        default: hipLaunchKernelGGL((zpass_kernel<DIR, 0>), grid, dim3(ZNT), lds, st, a); break;
+        case 8: hipLaunchKernelGGL((zpass_kernel<DIR, 8, double2>), grid, dim3(ZNT), lds, st, a); break;
+        case 16: hipLaunchKernelGGL((zpass_kernel<DIR, 16, double2>), grid, dim3(ZNT), lds, st, a); break;
+        default: hipLaunchKernelGGL((zpass_kernel<DIR, 0, double2>), grid, dim3(ZNT), lds, st, a); break;
     }
 }
 
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st) {
     if (a.ppt != 0 && a.ppt != 8 && a.ppt != 16) return hipErrorInvalidValue;
+    if (a.mixed && a.ppt == 0) return hipErrorInvalidValue;  // FP32 fields: in-place plans only
     dim3 grid((unsigned)((a.gd.Nxh + a.KT - 1) / a.KT) * (unsigned)a.gd.Ny * (unsigned)a.nfields);
     size_t lds = z_lds_bytes(a);
     if (dir > 0) launch_z<+1>(a, grid, lds, st);
@@ -486,7 +505,8 @@ int x_phys_lines(int mode) {
 }
 
 static size_t x_wave_lds(const XArgs& a, int waves) {
-    return ((size_t)a.gd.Nxp + 1 + (size_t)waves * x_region_slots(a.mode, a.ppt, a.ld)) * sizeof(double2);
+    const size_t elem = a.mixed ? sizeof(float2) : sizeof(double2);
+    return ((size_t)a.gd.Nxp + 1 + (size_t)waves * x_region_slots(a.mode, a.ppt, a.ld)) * elem;
 }
 
 int x_inplace_ppt(const XArgs& a) {
@@ -513,6 +533,7 @@ hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
     const bool wave = x_wave_path(a);
     if (wave && (a.waves < 1 || a.waves > XNT / 64)) return hipErrorInvalidValue;
     if (wave && a.ppt != 0 && a.ppt != 9 && a.ppt != 18) return hipErrorInvalidValue;
+    if (a.mixed && (!wave || a.ppt == 0)) return hipErrorInvalidValue;  // FP32 fields: in-place wave path only
     const int per_block = wave ? a.waves : a.LPB;
     dim3 grid((unsigned)((nlines + per_block - 1) / per_block));
     size_t lds = x_lds_bytes(a);
@@ -522,23 +543,31 @@ hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
         return hipGetLastError();
     }
     const dim3 block(64 * a.waves);
-#define CF_XWAVE(MODE, PPT)                                                                                      \
+#define CF_XWAVE(MODE, PPT, C)                                                                                   \
     {                                                                                                            \
         if (lds > 64 * 1024)                                                                                     \
-            (void)hipFuncSetAttribute((const void*)xwave_kernel<MODE, PPT>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                      (int)lds);                                                                 \
-        hipLaunchKernelGGL((xwave_kernel<MODE, PPT>), grid, block, lds, st, a);                                  \
+            (void)hipFuncSetAttribute((const void*)xwave_kernel<MODE, PPT, C>,                                   \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                     \
+        hipLaunchKernelGGL((xwave_kernel<MODE, PPT, C>), grid, block, lds, st, a);                               \
         return hipGetLastError();                                                                                \
     }
-#define CF_XMODE(MODE)                        \
-    switch (a.ppt) {                          \
-        case 9: CF_XWAVE(MODE, 9)             \
-        case 18: CF_XWAVE(MODE, 18)           \
-        default: CF_XWAVE(MODE, 0)            \
+// one statement (braced): every path returns the launch status
+#define CF_XMODE(MODE)                                    \
+    {                                                     \
+        if (a.mixed) {                                    \
+            if (a.ppt == 9) CF_XWAVE(MODE, 9, float2)     \
+            CF_XWAVE(MODE, 18, float2)                    \
+        }                                                 \
+        switch (a.ppt) {                                  \
+            case 9: CF_XWAVE(MODE, 9, double2)            \
+            case 18: CF_XWAVE(MODE, 18, double2)          \
+            default: CF_XWAVE(MODE, 0, double2)           \
+        }                                                 \
     }
     if (a.mode == X_ROTATIONAL) CF_XMODE(X_ROTATIONAL)
     if (a.mode == X_CONVECTIVE) CF_XMODE(X_CONVECTIVE)
-    CF_XMODE(X_DIVERGENCE)
+    if (a.mode == X_DIVERGENCE) CF_XMODE(X_DIVERGENCE)
+    return hipErrorInvalidValue;
 #undef CF_XMODE
 #undef CF_XWAVE
 }

@@ -99,13 +99,16 @@ struct Tile {
     }
     // forward Chebyshev transform of y-physical rows (values at y_j, j=0..P):
     // REDFT00, times norm, endpoint 1/2, Nyquist modes zeroed (normalize_forward!).
-    __device__ __forceinline__ CVec dct_forward(const double2* __restrict__ src, const FFTPlan& plan, double norm,
+    // SC: storage of the rows (double2, or float2 widened on load); the
+    // transform itself runs in FP64.
+    template <typename SC>
+    __device__ __forceinline__ CVec dct_forward(const SC* __restrict__ src, const FFTPlan& plan, double norm,
                                                 const Group<L>& g, bool zero_mode) {
         const int P = gd.P;
         for (int idx = lane; idx < (P + 1) * W; idx += 64) {
             int n = idx / W, m = idx - n * W;
             long long sm = sw0 + m;
-            double2 v = (sm < gd.B) ? src[sm + gd.B * n] : czero();
+            double2 v = (sm < gd.B) ? to_c64(src[sm + gd.B * n]) : czero();
             bufA[m * ld + n] = v;
             if (n >= 1 && n <= P - 1) bufA[m * ld + 2 * P - n] = v;
         }
@@ -123,7 +126,9 @@ struct Tile {
     }
     // inverse: coefficient registers -> values at y_j (endpoints doubled,
     // REDFT00, x1/2: InverseFFT!'s chebyshev + copy_to_padded! scale).
-    __device__ __forceinline__ void dct_inverse(double2* __restrict__ dst, const FFTPlan& plan, const Group<L>& g,
+    // Computed in FP64; SC = float2 rounds the stored values.
+    template <typename SC>
+    __device__ __forceinline__ void dct_inverse(SC* __restrict__ dst, const FFTPlan& plan, const Group<L>& g,
                                                 const CVec& a) {
         const int P = gd.P;
 #pragma unroll
@@ -140,7 +145,7 @@ struct Tile {
         for (int idx = lane; idx < (P + 1) * W; idx += 64) {
             int n = idx / W, m = idx - n * W;
             long long sm = sw0 + m;
-            if (sm < gd.B) dst[sm + gd.B * n] = cscale(res[m * ld + n], 0.5);
+            if (sm < gd.B) dst[sm + gd.B * n] = from_c64<SC>(cscale(res[m * ld + n], 0.5));
         }
         wave_sync();
     }
@@ -208,9 +213,12 @@ __device__ __forceinline__ void load_fac(const double* __restrict__ f, long long
 //   convective V, dV/dy            (nonlinear.jl:113-140; x and z derivatives
 //                                   are formed by the periodic passes)
 //   divergence V                   (nonlinear.jl:146-167)
-template <int L>
+template <int L, typename SC = double2>
 __device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
                             const CVec U[3]) {
+    SC* out6[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) out6[i] = reinterpret_cast<SC*>(a.out6[i]);
     CVec V[3] = {U[0], U[1], U[2]};
     if (mi.mean && mi.valid) {
 #pragma unroll
@@ -219,29 +227,29 @@ __device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const
             if (n < a.gd.Ny) V[0].v[e] = cadd(V[0].v[e], cmk(a.base[n], 0.0));
         }
     }
-    tile.dct_inverse(a.out6[0], a.dct, g, V[0]);
-    tile.dct_inverse(a.out6[1], a.dct, g, V[1]);
-    tile.dct_inverse(a.out6[2], a.dct, g, V[2]);
+    tile.dct_inverse(out6[0], a.dct, g, V[0]);
+    tile.dct_inverse(out6[1], a.dct, g, V[1]);
+    tile.dct_inverse(out6[2], a.dct, g, V[2]);
     if (a.form == FORM_DIVERGENCE) return;
     if (a.form == FORM_CONVECTIVE) {
-        tile.dct_inverse(a.out6[3], a.dct, g, cheb_deriv(g, V[0]));
-        tile.dct_inverse(a.out6[4], a.dct, g, cheb_deriv(g, V[1]));
-        tile.dct_inverse(a.out6[5], a.dct, g, cheb_deriv(g, V[2]));
+        tile.dct_inverse(out6[3], a.dct, g, cheb_deriv(g, V[0]));
+        tile.dct_inverse(out6[4], a.dct, g, cheb_deriv(g, V[1]));
+        tile.dct_inverse(out6[5], a.dct, g, cheb_deriv(g, V[2]));
         return;
     }
     CVec d3 = cheb_deriv(g, V[2]);
     CVec W;
 #pragma unroll
     for (int e = 0; e < 4; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V[1].v[e], mi.kz)));  // dy V3 - i l V2
-    tile.dct_inverse(a.out6[3], a.dct, g, W);
+    tile.dct_inverse(out6[3], a.dct, g, W);
 #pragma unroll
     for (int e = 0; e < 4; ++e)  // i (l V1 - k V3)
         W.v[e] = cmuli(csub(cscale(V[0].v[e], mi.kz), cscale(V[2].v[e], mi.kx)));
-    tile.dct_inverse(a.out6[4], a.dct, g, W);
+    tile.dct_inverse(out6[4], a.dct, g, W);
     CVec d1 = cheb_deriv(g, V[0]);
 #pragma unroll
     for (int e = 0; e < 4; ++e) W.v[e] = csub(cmuli(cscale(V[1].v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
-    tile.dct_inverse(a.out6[5], a.dct, g, W);
+    tile.dct_inverse(out6[5], a.dct, g, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -404,9 +412,10 @@ __device__ __forceinline__ CVec laplacian(const Group<L>& g, const CVec& a, doub
 // Full CNRK2 stage: forward DCT of the nonlinear term, Q update, RHS,
 // Stokes solve, state write, inverse transforms for the next stage.
 // ---------------------------------------------------------------------------
-template <int L>
-// two waves per SIMD (<= 256 registers per lane) at the price of a small
-// spill: one wave per SIMD leaves the wave-serial scans and LDS FFTs unhidden
+// SC: storage of the y-physical fields (double2; float2 in the mixed step).
+template <int L, typename SC>
+// two waves per SIMD (<= 256 registers per lane): one wave per SIMD leaves
+// the wave-serial scans and LDS FFTs unhidden
 __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
@@ -427,7 +436,7 @@ __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) y
         if (a.Ncoef[c]) {
             Qc = tile.load(a.Ncoef[c], g);
         } else {
-            Qc = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
+            Qc = tile.dct_forward(reinterpret_cast<const SC*>(a.Nphys[c]), a.dct, a.norm, g, mi.nyq);
 #pragma unroll
             for (int e = 0; e < 4; ++e) Qc.v[e] = cscale(Qc.v[e], a.nl_sign);
         }
@@ -501,11 +510,11 @@ __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) y
     tile.store(a.U[1], g, Un[1]);
     tile.store(a.U[2], g, Un[2]);
     tile.store(a.P, g, Pn);
-    if (a.out6[0]) inverse_six(tile, a, g, mi, Un);
+    if (a.out6[0]) inverse_six<L, SC>(tile, a, g, mi, Un);
 }
 
 // prologue: state U -> six y-physical fields (velocity and vorticity)
-template <int L>
+template <int L, typename SC>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
@@ -515,7 +524,7 @@ __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     CVec U[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
-    inverse_six(tile, a, g, mi, U);
+    inverse_six<L, SC>(tile, a, g, mi, U);
 }
 
 // Stokes solve! of a bank: R (3 spectral) -> U (3), P (stokes.jl:81-108)
@@ -877,8 +886,10 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     constexpr int M = YNT / L;
     dim3 grid((unsigned)((a.gd.B + M - 1) / M));
     size_t lds = y_lds_bytes(a.gd.Ny, a.ld);
-    allow_lds(ystage_kernel<L>, lds);
-    allow_lds(yprologue_kernel<L>, lds);
+    allow_lds(ystage_kernel<L, double2>, lds);
+    allow_lds(yprologue_kernel<L, double2>, lds);
+    allow_lds(ystage_kernel<L, float2>, lds);
+    allow_lds(yprologue_kernel<L, float2>, lds);
     allow_lds(ysolve_kernel<L>, lds);
     allow_lds(yneglap_kernel<L>, lds);
     allow_lds(ypressure_kernel<L>, lds);
@@ -887,8 +898,14 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     allow_lds(ydot_kernel<L>, lds);
     allow_lds(ystats_kernel<L>, lds);
     switch (which) {
-        case Y_STAGE: hipLaunchKernelGGL(ystage_kernel<L>, grid, dim3(YNT), lds, st, a); break;
-        case Y_PROLOGUE: hipLaunchKernelGGL(yprologue_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_STAGE:
+            if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, float2>), grid, dim3(YNT), lds, st, a);
+            else hipLaunchKernelGGL((ystage_kernel<L, double2>), grid, dim3(YNT), lds, st, a);
+            break;
+        case Y_PROLOGUE:
+            if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, float2>), grid, dim3(YNT), lds, st, a);
+            else hipLaunchKernelGGL((yprologue_kernel<L, double2>), grid, dim3(YNT), lds, st, a);
+            break;
         case Y_SOLVE: hipLaunchKernelGGL(ysolve_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_NEGLAP: hipLaunchKernelGGL(yneglap_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_PRESSURE: hipLaunchKernelGGL(ypressure_kernel<L>, grid, dim3(YNT), lds, st, a); break;

@@ -38,6 +38,14 @@ extern "C" {
 #define CF_FORM_CONVECTIVE 1
 #define CF_FORM_DIVERGENCE 2
 
+/* arithmetic of the step (cf_config.precision).  MIXED: the y-physical
+ * fields, the x/z FFTs and the nonlinear products are FP32; the state U, P,
+ * the accumulator Q, the Chebyshev transforms' arithmetic, the Helmholtz-tau
+ * and influence-matrix solves and the mean mode stay FP64.  Only the time
+ * step (cf_step, cf_advance, cf_advance_to) uses it; every other call is FP64. */
+#define CF_PRECISION_FP64 0
+#define CF_PRECISION_MIXED 1
+
 /* mean-flow constraint (src/problem.jl:81-87) */
 #define CF_CONSTRAINT_PRESSURE_GRADIENT 0
 #define CF_CONSTRAINT_BULK_VELOCITY 1
@@ -54,6 +62,8 @@ typedef struct cf_config {
     int constraint;          /* CF_CONSTRAINT_*: the problem default */
     double constraint_values[2];
     int device;              /* HIP device ordinal */
+    int precision;           /* CF_PRECISION_*; MIXED needs even Nxp and a
+                                rotating or convective form */
 } cf_config;
 
 /* ChannelFlowProblem(grid, profile, nu, dt; form, pressuregradient|bulkvelocity)

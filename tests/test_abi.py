@@ -42,6 +42,7 @@ def test_create_rejects_bad_configs_without_device():
         cfg.form = kw.get("form", 0)
         cfg.constraint = kw.get("constraint", 0)
         cfg.constraint_values[0], cfg.constraint_values[1] = kw.get("vals", (0.0, 0.0))
+        cfg.precision = kw.get("precision", 0)
         h = C.c_void_p()
         rc = L.cf_problem_create(C.byref(cfg), C.byref(h))
         return rc, b
@@ -56,6 +57,9 @@ def test_create_rejects_bad_configs_without_device():
     assert make(form=3)[0] == _lib.CF_ERR_ARGUMENT          # forms: rotating, convective, divergence
     assert b"form" in L.cf_last_error()
     assert make(Ny=257)[0] == _lib.CF_ERR_UNSUPPORTED      # a mode's column must fit one wavefront
+    assert make(precision=2)[0] == _lib.CF_ERR_ARGUMENT
+    assert make(precision=1, form=2)[0] == _lib.CF_ERR_UNSUPPORTED  # mixed: no divergence form
+    assert make(precision=1, Nx=7)[0] == _lib.CF_ERR_UNSUPPORTED    # mixed: odd Nxp (11) rejected
 
 
 def test_python_mirror_validates_constraints():
