@@ -130,6 +130,70 @@ def bench_mixed(cf, steps):
     }
 
 
+ENSEMBLE = dict(n=64, grids=[(16, 17, 16), (24, 33, 24), (32, 33, 32), (48, 49, 48)], Re=(300.0, 500.0),
+                L=(2 * math.pi / 1.14, 2 * math.pi / 2.5), dt=0.01, amp=0.1,
+                desc="64 plane Couette instances, grids drawn (seed 2024) from 16x17x16, 24x33x24, 32x33x32, "
+                     "48x49x48, Re uniform in [300, 500], Lx=2pi/1.14 Lz=2pi/2.5 dt=0.01, amplitude 0.1 "
+                     "(seed 1000+i); one graph-captured ensemble step, autotuned tiles")
+
+
+def bench_ensemble(cf, steps, plain_steps=20):
+    """Batched ensemble (one graph launch per ensemble step) against the plain
+    path (each problem advanced by its own step_ calls): instance-steps/s."""
+    import numpy as np
+    import torch
+
+    w = ENSEMBLE
+    rng = np.random.default_rng(2024)
+    probs, states, total_bytes, drawn = [], [], 0, []
+    for i in range(w["n"]):
+        dims = w["grids"][int(rng.integers(len(w["grids"])))]
+        Re = float(rng.uniform(*w["Re"]))
+        g = cf.Grid(*dims, *w["L"])
+        p = cf.CouetteFlow(g, 1.0 / Re, w["dt"])
+        probs.append(p)
+        states.append(cf.random_state(p, w["amp"], seed=1000 + i))
+        total_bytes += algorithmic_bytes(*dims)[1]
+        drawn.append(dims)
+    plain_states = [s.copy() for s in states]
+    t0 = time.perf_counter()
+    ens = cf.Ensemble(probs, states, tune=True)
+    setup_s = time.perf_counter() - t0
+    launches, _ = ens.info()
+    ens.advance(10)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    ens.advance(steps)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    ok = all(bool(torch.isfinite(torch.view_as_real(s.velocity)).all()) for s in states)
+    # plain path: every instance stepped by its own step_ calls, one after another
+    for p, s in zip(probs, plain_states):
+        cf.step_(p.scheme, p.nlterm, s.velocity, s.stagepressure, 0.0, nsteps=2, **p.constraint)
+    torch.cuda.synchronize()
+    a.record()
+    for p, s in zip(probs, plain_states):
+        cf.step_(p.scheme, p.nlterm, s.velocity, s.stagepressure, 0.0, nsteps=plain_steps, **p.constraint)
+    b.record()
+    torch.cuda.synchronize()
+    ms_plain = a.elapsed_time(b) / plain_steps
+    counts = {f"{d[0]}x{d[1]}x{d[2]}": drawn.count(d) for d in w["grids"]}
+    del ens
+    return {
+        "workload": w["desc"], "instances": w["n"], "grid_counts": counts, "steps": steps,
+        "instance_steps_per_s": round(w["n"] * 1000.0 / ms, 1),
+        "ms_per_ensemble_step": round(ms, 4),
+        "plain_instance_steps_per_s": round(w["n"] * 1000.0 / ms_plain, 1),
+        "speedup_vs_plain": round(ms_plain / ms, 2),
+        "launches_per_ensemble_step": launches, "plain_launches_per_ensemble_step": 13 * w["n"],
+        "ensemble_step_algorithmic_bytes": total_bytes,
+        "hbm_frac": round(total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "setup_s_with_tuning": round(setup_s, 2), "finite": ok,
+    }
+
+
 def base_coefficients(problem):
     """Chebyshev coefficients of the problem's base profile (host DCT-I of its
     Lobatto-point values, chebcoeffs normalisation)."""
@@ -178,6 +242,8 @@ def main():
     ap.add_argument("--workload", default="poiseuille128", choices=sorted(WORKLOADS))
     ap.add_argument("--mixed-steps", type=int, default=500,
                     help="steps of the FP32-mixed 256x193x256 comparison (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=1000,
+                    help="steps of the 64-instance batched ensemble (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -255,13 +321,19 @@ def main():
                          f"({secs:.1f} s, {cores} OpenMP threads)"}
 
     forms = {}
+    del prob, state, U, P
     if args.mixed_steps > 0:
-        del prob, state, U, P
         torch.cuda.empty_cache()
         try:
             forms["mixed"] = bench_mixed(cf, args.mixed_steps)
         except Exception as e:  # the headline line must still print
             forms["mixed"] = {"error": f"{type(e).__name__}: {e}"}
+    if args.ensemble_steps > 0:
+        torch.cuda.empty_cache()
+        try:
+            forms["batched"] = bench_ensemble(cf, args.ensemble_steps)
+        except Exception as e:
+            forms["batched"] = {"error": f"{type(e).__name__}: {e}"}
 
     out = {
         "metric": "CNRK2 steps/s & %HBM roofline, 128×97×128 FP64; mixed/batched/closure vs plain",

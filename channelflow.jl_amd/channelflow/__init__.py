@@ -26,7 +26,7 @@ __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
            "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
            "power_input", "plane_statistics", "ProfileStatistics", "flow", "device_gaussian_noise",
-           "savefield", "loadfield"]
+           "savefield", "loadfield", "Ensemble"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -549,6 +549,52 @@ def loadfield(path, grid=None, device="cuda"):
     if f.is_state:
         return State(torch.as_tensor(f.velocity(), device=device), torch.as_tensor(f.pressure(), device=device))
     return torch.as_tensor(f.field(), device=device)
+
+
+class Ensemble:
+    """n independent problems advanced together (cf_ensemble_*): each kernel
+    class of the step is one launch per group of instances sharing its tile
+    parameters, and one ensemble step is a captured HIP graph.  Every
+    instance evolves exactly as step_ would advance it.  The states stay
+    bound to the ensemble (their device buffers are baked into the graph)."""
+
+    def __init__(self, problems, states, tune=True):
+        if len(problems) != len(states) or not problems:
+            raise ArgumentError("need one state per problem")
+        L = _lib.load()
+        for p, s in zip(problems, states):
+            g = p.grid
+            _check_vec(g, s.velocity)
+            _check_spec(g, s.stagepressure)
+        n = len(problems)
+        hs = (C.c_void_p * n)(*[p.scheme.h.value for p in problems])
+        ptrs = []
+        for s in states:
+            ptrs += [_ptr(s.velocity[0]), _ptr(s.velocity[1]), _ptr(s.velocity[2]), _ptr(s.stagepressure)]
+        fs = (C.c_void_p * (4 * n))(*ptrs)
+        h = C.c_void_p()
+        _torch().cuda.synchronize()
+        _lib.check(L.cf_ensemble_create(hs, fs, n, 1 if tune else 0, C.byref(h)))
+        self.h = h
+        self.problems, self.states = list(problems), list(states)
+
+    def advance(self, nsteps, stream=None):
+        _lib.check(_lib.load().cf_ensemble_advance(self.h, int(nsteps), _stream(stream)))
+
+    def info(self):
+        """(launches per ensemble step, [(z tile width, x wavefronts per block)] per instance)."""
+        n = len(self.problems)
+        launches = C.c_int()
+        tiles = (C.c_int * (2 * n))()
+        _lib.check(_lib.load().cf_ensemble_info(self.h, C.byref(launches), tiles))
+        return launches.value, [(tiles[2 * i], tiles[2 * i + 1]) for i in range(n)]
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None
+        if h and lib is not None:
+            lib.cf_ensemble_destroy(h)
+            self.h = None
 
 
 def kinetic_energy(problem, U):

@@ -40,7 +40,8 @@ _lib = None
 SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_problem_sizes", "cf_step", "cf_advance",
            "cf_step_timed",
            "cf_last_gradient", "cf_forward", "cf_inverse", "cf_nonlinear", "cf_stokes_solve", "cf_project",
-           "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input", "cf_plane_statistics", "cf_advance_to", "cf_gaussian_noise", "cf_random_state"]
+           "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input", "cf_plane_statistics", "cf_advance_to", "cf_gaussian_noise", "cf_random_state",
+           "cf_ensemble_create", "cf_ensemble_advance", "cf_ensemble_info", "cf_ensemble_destroy"]
 
 
 def load(path=LIB_PATH):
@@ -73,6 +74,10 @@ def load(path=LIB_PATH):
     L.cf_advance_to.argtypes = [vp, vp, vp, vp, vp, d, d, i, dp, dp, vp]
     L.cf_gaussian_noise.argtypes = [vp, d, C.c_ulonglong, vp, vp]
     L.cf_random_state.argtypes = [vp, d, C.c_ulonglong, vp, vp, vp, vp, vp]
+    L.cf_ensemble_create.argtypes = [C.POINTER(vp), C.POINTER(vp), i, i, C.POINTER(vp)]
+    L.cf_ensemble_advance.argtypes = [vp, i, vp]
+    L.cf_ensemble_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.cf_ensemble_destroy.argtypes = [vp]
     for name in SYMBOLS:
         getattr(L, name).restype = C.c_int if name != "cf_last_error" else C.c_char_p
     _lib = L

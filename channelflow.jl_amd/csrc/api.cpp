@@ -14,6 +14,8 @@
 #include <cstring>
 #include <string>
 #include <utility>
+#include <algorithm>
+#include <array>
 #include <list>
 #include <vector>
 
@@ -383,8 +385,7 @@ static YArgs base_yargs(cf_problem* h) {
 
 static double2* fld(double2* base, long long n, int i) { return base + (size_t)n * i; }
 
-static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* const* out, hipStream_t st,
-                 unsigned dz_mask = 0, bool mixed = false) {
+static ZArgs make_zargs(cf_problem* h, int nf, double2* const* in, double2* const* out, unsigned dz_mask, bool mixed) {
     ZArgs z;
     std::memset(&z, 0, sizeof(z));
     z.gd = h->gd;
@@ -400,7 +401,12 @@ static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* co
         z.in[i] = in[i];
         z.out[i] = out[i];
     }
-    HIPTRY(launch_zpass(dir, z, st));
+    return z;
+}
+
+static int zpass(cf_problem* h, int dir, int nf, double2* const* in, double2* const* out, hipStream_t st,
+                 unsigned dz_mask = 0, bool mixed = false) {
+    HIPTRY(launch_zpass(dir, make_zargs(h, nf, in, out, dz_mask, mixed), st));
     return 0;
 }
 
@@ -432,38 +438,54 @@ static double2* six_field(cf_problem* h, int i, bool mixed) {
     return mixed ? (double2*)((float2*)h->six + (size_t)h->gd.S * i) : fld(h->six, h->gd.S, i);
 }
 
-// mixed: FP32 complex fields (z buffer fields Sz elements apart)
-static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st, Timer* tm = nullptr, bool mixed = false) {
+// The three launches of the nonlinear term's periodic work: z inverse of the
+// form's y-physical fields, x pass (products), z forward of the products.
+// mixed: FP32 complex fields (z buffer fields Sz elements apart).
+struct NonlinearArgs {
+    ZArgs zin, zout;
+    XArgs x;
+};
+
+static NonlinearArgs nonlinear_args(cf_problem* h, bool mixed) {
     const GridDims& gd = h->gd;
     double2* six[6];
     double2* zb[9];
     for (int i = 0; i < 6; ++i) six[i] = six_field(h, i, mixed);
     const int nzb = h->form == CF_FORM_CONVECTIVE ? 9 : 6;
     for (int i = 0; i < nzb; ++i) zb[i] = mixed ? (double2*)((float2*)h->zbuf + (size_t)gd.Sz * i) : fld(h->zbuf, gd.Sz, i);
-    XArgs x = base_xargs(h, mixed);
+    NonlinearArgs na;
+    na.x = base_xargs(h, mixed);
+    XArgs& x = na.x;
     if (h->form == CF_FORM_ROTATING) {
-        if (int rc = zpass(h, +1, 6, six, zb, st, 0, mixed)) return rc;
+        na.zin = make_zargs(h, 6, six, zb, 0, mixed);
         x.mode = X_ROTATIONAL;
         for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
         for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a wavefront reads its line first
     } else if (h->form == CF_FORM_CONVECTIVE) {
         // V, dV/dy and (from V again) dV/dz
         double2* zin[9] = {six[0], six[1], six[2], six[3], six[4], six[5], six[0], six[1], six[2]};
-        if (int rc = zpass(h, +1, 9, zin, zb, st, 0x1C0u, mixed)) return rc;
+        na.zin = make_zargs(h, 9, zin, zb, 0x1C0u, mixed);
         x.mode = X_CONVECTIVE;
         for (int i = 0; i < 9; ++i) x.in[i] = zb[i];
         for (int i = 0; i < 3; ++i) x.out[i] = zb[i];
     } else {
-        if (int rc = zpass(h, +1, 3, six, zb, st)) return rc;
+        na.zin = make_zargs(h, 3, six, zb, 0, mixed);
         x.mode = X_DIVERGENCE;
         for (int i = 0; i < 3; ++i) x.in[i] = zb[i];
         for (int i = 0; i < 6; ++i) x.out[i] = zb[i];
     }
-    if (int rc = tmark(tm, 1, st)) return rc;
-    HIPTRY(launch_xpass(x, st));
-    if (int rc = tmark(tm, 2, st)) return rc;
     const int nout = h->form == CF_FORM_DIVERGENCE ? 6 : 3;
-    if (int rc = zpass(h, -1, nout, zb, six, st, 0, mixed)) return rc;
+    na.zout = make_zargs(h, nout, zb, six, 0, mixed);
+    return na;
+}
+
+static int enqueue_nonlinear_phys(cf_problem* h, hipStream_t st, Timer* tm = nullptr, bool mixed = false) {
+    const NonlinearArgs na = nonlinear_args(h, mixed);
+    HIPTRY(launch_zpass(+1, na.zin, st));
+    if (int rc = tmark(tm, 1, st)) return rc;
+    HIPTRY(launch_xpass(na.x, st));
+    if (int rc = tmark(tm, 2, st)) return rc;
+    HIPTRY(launch_zpass(-1, na.zout, st));
     if (int rc = tmark(tm, 3, st)) return rc;
     return 0;
 }
@@ -478,12 +500,16 @@ static int enqueue_nonlinear_coef(cf_problem* h, double2* const Ncoef[3], hipStr
     return 0;
 }
 
-static int enqueue_prologue(cf_problem* h, double2* const U[3], hipStream_t st, bool mixed = false) {
+static YArgs prologue_args(cf_problem* h, double2* const U[3], bool mixed) {
     YArgs a = base_yargs(h);
     a.mixed = mixed ? 1 : 0;
     for (int i = 0; i < 3; ++i) a.U[i] = U[i];
     for (int i = 0; i < 6; ++i) a.out6[i] = six_field(h, i, mixed);
-    HIPTRY(launch_ykernel(Y_PROLOGUE, a, st));
+    return a;
+}
+
+static int enqueue_prologue(cf_problem* h, double2* const U[3], hipStream_t st, bool mixed = false) {
+    HIPTRY(launch_ykernel(Y_PROLOGUE, prologue_args(h, U, mixed), st));
     return 0;
 }
 
@@ -537,6 +563,36 @@ struct Timer {
 };
 static int tmark(Timer* tm, int c, hipStream_t st) { return tm ? tm->mark(c, st) : 0; }
 
+// Arguments of CNRK2 stage j (cnrk2.jl:168-204) for step length dt with
+// solver bank `bank`; Ncoef: the divergence form's coefficient-space term.
+static YArgs stage_args(cf_problem* h, double2* const U[3], double2* P, int j, const Constraint& c, double dt,
+                        const DevBank* bank, bool mixed, double2* const Ncoef[3]) {
+    const GridDims& gd = h->gd;
+    YArgs a = base_yargs(h);
+    a.mixed = mixed ? 1 : 0;
+    a.nl_sign = (h->form == CF_FORM_CONVECTIVE) ? -1.0 : 1.0;
+    a.stage = j;
+    a.store_Q = (j < 2);
+    a.lambda = 1.0 / (CNRK2_C[j] * dt);
+    a.Aj = CNRK2_A[j];
+    a.weight = CNRK2_B[j] / CNRK2_C[j];
+    a.bulk = c.bulk;
+    a.c0 = c.c0;
+    a.c1 = c.c1;
+    a.bank = bank->view();
+    for (int i = 0; i < 3; ++i) {
+        a.Nphys[i] = six_field(h, i, mixed);
+        a.Ncoef[i] = (h->form == CF_FORM_DIVERGENCE) ? Ncoef[i] : nullptr;
+        a.U[i] = U[i];
+        a.Q[i] = fld(h->Q, gd.S, i);
+    }
+    a.P = P;
+    a.grad_out = h->d_grad;
+    if (j < 2)
+        for (int i = 0; i < 6; ++i) a.out6[i] = six_field(h, i, mixed);
+    return a;
+}
+
 // nsteps CNRK2 steps of length dt with the stage solver banks `banks`
 // (default: the problem's dt and its three stage banks)
 static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
@@ -553,28 +609,7 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
             double2* Ncoef[3] = {fld(h->R, gd.S, 0), fld(h->R, gd.S, 1), fld(h->R, gd.S, 2)};
             if (h->form == CF_FORM_DIVERGENCE)
                 if (int rc = enqueue_nonlinear_coef(h, Ncoef, st)) return rc;
-            YArgs a = base_yargs(h);
-            a.mixed = mixed ? 1 : 0;
-            a.nl_sign = (h->form == CF_FORM_CONVECTIVE) ? -1.0 : 1.0;
-            a.stage = j;
-            a.store_Q = (j < 2);
-            a.lambda = 1.0 / (CNRK2_C[j] * dt);
-            a.Aj = CNRK2_A[j];
-            a.weight = CNRK2_B[j] / CNRK2_C[j];
-            a.bulk = c.bulk;
-            a.c0 = c.c0;
-            a.c1 = c.c1;
-            a.bank = (banks ? banks[j] : &h->stage[j])->view();
-            for (int i = 0; i < 3; ++i) {
-                a.Nphys[i] = six_field(h, i, mixed);
-                a.Ncoef[i] = (h->form == CF_FORM_DIVERGENCE) ? Ncoef[i] : nullptr;
-                a.U[i] = U[i];
-                a.Q[i] = fld(h->Q, gd.S, i);
-            }
-            a.P = P;
-            a.grad_out = h->d_grad;
-            if (j < 2)
-                for (int i = 0; i < 6; ++i) a.out6[i] = six_field(h, i, mixed);
+            const YArgs a = stage_args(h, U, P, j, c, dt, banks ? banks[j] : &h->stage[j], mixed, Ncoef);
             HIPTRY(launch_ykernel(Y_STAGE, a, st));
             if (int rc = tmark(tm, 4, st)) return rc;
         }
@@ -974,4 +1009,347 @@ extern "C" int cf_random_state(cf_problem* h, double epsilon, unsigned long long
         if (int rc = cf_forward(h, h->d_noise + (size_t)c * n, U[c], stream)) return rc;
     if (int rc = cf_project(h, U0, U1, U2, stream)) return rc;
     return cf_pressure(h, U0, U1, U2, P, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: many independent problems advanced together.  Each kernel class
+// of the step is one launch per group of instances that share its template
+// parameters (y: group lanes L; z: in-place width; x: form, in-place width and
+// wavefronts per block); a block finds its instance from a prefix table of
+// block counts and reads that instance's argument block from device memory.
+// One ensemble step is captured once into a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+namespace {
+
+enum { B_YPRO = 0, B_ZINV, B_X, B_ZFWD, B_YSTAGE };
+
+struct BatchLaunch {
+    int cls;        // B_*
+    int tp, waves;  // template parameter (L or in-place width), x wavefronts per block
+    int mode;       // x form
+    const void* tab;
+    const unsigned* start;
+    int n;
+    unsigned nblocks;
+    size_t lds;
+};
+
+// per-instance tile choices (the tunables): z tile width and x wavefronts per block
+struct Tiles {
+    int KT, zppt, xwaves;
+};
+
+}  // namespace
+
+struct cf_ensemble {
+    int device = 0;
+    std::vector<cf_problem*> probs;
+    std::vector<std::array<double2*, 4>> fields;  // U0, U1, U2, P per instance
+    std::vector<Tiles> tiles;
+    std::vector<void*> allocs;
+    std::vector<BatchLaunch> step;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void ens_free(cf_ensemble* e) {
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->cap) (void)hipStreamDestroy(e->cap);
+    for (void* p : e->allocs) (void)hipFree(p);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    e->cap = nullptr;
+    e->allocs.clear();
+}
+
+static hipError_t run_batch(const BatchLaunch& b, hipStream_t st) {
+    switch (b.cls) {
+        case B_YPRO: return launch_ykernel_batch(Y_PROLOGUE, b.tp, (const YArgs*)b.tab, b.start, b.n, b.nblocks, b.lds, st);
+        case B_YSTAGE: return launch_ykernel_batch(Y_STAGE, b.tp, (const YArgs*)b.tab, b.start, b.n, b.nblocks, b.lds, st);
+        case B_ZINV: return launch_zpass_batch(+1, b.tp, (const ZArgs*)b.tab, b.start, b.n, b.nblocks, b.lds, st);
+        case B_ZFWD: return launch_zpass_batch(-1, b.tp, (const ZArgs*)b.tab, b.start, b.n, b.nblocks, b.lds, st);
+        case B_X: return launch_xpass_batch(b.mode, b.tp, b.waves, (const XArgs*)b.tab, b.start, b.n, b.nblocks, b.lds, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Upload one group's argument table and block prefix; append its launch.
+template <typename A>
+static int ens_group(cf_ensemble* e, int cls, int tp, int waves, int mode, const std::vector<A>& args,
+                     const std::vector<unsigned>& blocks, const std::vector<size_t>& lds, std::vector<BatchLaunch>& out) {
+    const int n = (int)args.size();
+    std::vector<unsigned> start(n + 1, 0);
+    size_t need = 0;
+    for (int i = 0; i < n; ++i) {
+        start[i + 1] = start[i] + blocks[i];
+        need = std::max(need, lds[i]);
+    }
+    if ((unsigned long long)start[n] >= (1ull << 31)) return fail(CF_ERR_UNSUPPORTED, "ensemble launch too large");
+    void* t = nullptr;
+    void* s = nullptr;
+    HIPTRY(hipMalloc(&t, sizeof(A) * n));
+    e->allocs.push_back(t);
+    HIPTRY(hipMalloc(&s, sizeof(unsigned) * (n + 1)));
+    e->allocs.push_back(s);
+    HIPTRY(hipMemcpy(t, args.data(), sizeof(A) * n, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(s, start.data(), sizeof(unsigned) * (n + 1), hipMemcpyHostToDevice));
+    out.push_back(BatchLaunch{cls, tp, waves, mode, t, (const unsigned*)s, n, start[n], need});
+    return 0;
+}
+
+// Group instances by key and emit one launch per group, in first-seen order.
+template <typename A, typename KeyFn, typename InfoFn>
+static int ens_class(cf_ensemble* e, int cls, const std::vector<A>& all, KeyFn key, InfoFn info,
+                     std::vector<BatchLaunch>& out) {
+    std::vector<std::array<int, 3>> keys;
+    for (const A& a : all) {
+        std::array<int, 3> k = key(a);
+        bool seen = false;
+        for (auto& q : keys) seen = seen || (q == k);
+        if (!seen) keys.push_back(k);
+    }
+    for (auto& k : keys) {
+        std::vector<A> args;
+        std::vector<unsigned> blocks;
+        std::vector<size_t> lds;
+        for (const A& a : all)
+            if (key(a) == k) {
+                unsigned b;
+                size_t l;
+                info(a, b, l);
+                args.push_back(a);
+                blocks.push_back(b);
+                lds.push_back(l);
+            }
+        if (int rc = ens_group(e, cls, k[0], k[1], k[2], args, blocks, lds, out)) return rc;
+    }
+    return 0;
+}
+
+// Launch tables of one ensemble step for the current tile choices.
+static int ens_build(cf_ensemble* e, std::vector<BatchLaunch>& step) {
+    const int n = (int)e->probs.size();
+    std::vector<YArgs> pro(n);
+    std::vector<ZArgs> zin(n), zout(n);
+    std::vector<XArgs> xs(n);
+    std::vector<std::array<YArgs, 3>> stages(n);
+    for (int i = 0; i < n; ++i) {
+        cf_problem* h = e->probs[i];
+        double2* U[3] = {e->fields[i][0], e->fields[i][1], e->fields[i][2]};
+        Constraint c;
+        if (int rc = resolve_constraint(h, -1, nullptr, c)) return rc;
+        pro[i] = prologue_args(h, U, false);
+        NonlinearArgs na = nonlinear_args(h, false);
+        const Tiles& t = e->tiles[i];
+        for (ZArgs* z : {&na.zin, &na.zout}) {
+            z->KT = t.KT;
+            z->ppt = t.zppt;
+            z->mul_kt = recip_mul((unsigned)t.KT);
+        }
+        na.x.waves = t.xwaves;
+        zin[i] = na.zin;
+        zout[i] = na.zout;
+        xs[i] = na.x;
+        double2* Nc[3] = {nullptr, nullptr, nullptr};
+        for (int j = 0; j < 3; ++j) stages[i][j] = stage_args(h, U, e->fields[i][3], j, c, h->dt, &h->stage[j], false, Nc);
+    }
+    auto ykey = [](const YArgs& a) { return std::array<int, 3>{y_group_lanes(a.gd.Ny), 0, 0}; };
+    auto yinfo = [](const YArgs& a, unsigned& b, size_t& l) {
+        b = y_blocks(a.gd);
+        l = y_lds_bytes(a.gd.Ny, a.ld);
+    };
+    auto zkey = [](const ZArgs& a) { return std::array<int, 3>{a.ppt, 0, 0}; };
+    auto zinfo = [](const ZArgs& a, unsigned& b, size_t& l) {
+        b = z_blocks(a);
+        l = z_lds_bytes(a);
+    };
+    auto xkey = [](const XArgs& a) { return std::array<int, 3>{a.ppt, a.waves, a.mode}; };
+    auto xinfo = [](const XArgs& a, unsigned& b, size_t& l) {
+        b = x_blocks(a);
+        l = x_lds_bytes(a);
+    };
+    step.clear();
+    if (int rc = ens_class(e, B_YPRO, pro, ykey, yinfo, step)) return rc;
+    for (int j = 0; j < 3; ++j) {
+        std::vector<YArgs> st(n);
+        for (int i = 0; i < n; ++i) st[i] = stages[i][j];
+        if (int rc = ens_class(e, B_ZINV, zin, zkey, zinfo, step)) return rc;
+        if (int rc = ens_class(e, B_X, xs, xkey, xinfo, step)) return rc;
+        if (int rc = ens_class(e, B_ZFWD, zout, zkey, zinfo, step)) return rc;
+        if (int rc = ens_class(e, B_YSTAGE, st, ykey, yinfo, step)) return rc;
+    }
+    return 0;
+}
+
+// Autotuner: per distinct grid, time the batched z inverse pass for each
+// in-place tile width KT (and the x pass for each wavefront count) on the
+// instances' own workspaces (results do not depend on the tiling: every
+// line is transformed by the same butterflies), keep the fastest.
+static int ens_time_class(cf_ensemble* e, int cls, float* ms) {
+    std::vector<BatchLaunch> step;
+    if (int rc = ens_build(e, step)) return rc;
+    hipEvent_t a, b;
+    HIPTRY(hipEventCreate(&a));
+    HIPTRY(hipEventCreate(&b));
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        HIPTRY(hipEventRecord(a, e->cap));
+        for (const BatchLaunch& L : step)
+            if (L.cls == cls) HIPTRY(run_batch(L, e->cap));
+        HIPTRY(hipEventRecord(b, e->cap));
+        HIPTRY(hipEventSynchronize(b));
+        float t = 0;
+        HIPTRY(hipEventElapsedTime(&t, a, b));
+        best = std::min(best, t);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = best;
+    return 0;
+}
+
+static int ens_tune(cf_ensemble* e) {
+    const int n = (int)e->probs.size();
+    std::vector<std::array<int, 3>> grids;
+    for (cf_problem* h : e->probs) {
+        std::array<int, 3> g{h->gd.Nx, h->gd.Ny, h->gd.Nz};
+        bool seen = false;
+        for (auto& q : grids) seen = seen || (q == g);
+        if (!seen) grids.push_back(g);
+    }
+    for (auto& g : grids) {
+        std::vector<int> members;
+        for (int i = 0; i < n; ++i)
+            if (e->probs[i]->gd.Nx == g[0] && e->probs[i]->gd.Ny == g[1] && e->probs[i]->gd.Nz == g[2]) members.push_back(i);
+        cf_problem* h = e->probs[members[0]];
+        // z tile width: in-place widths only (one LDS buffer), 1..16 k per tile
+        Tiles keep = e->tiles[members[0]];
+        float best = 1e30f;
+        for (int kt = 1; kt <= std::min(16, h->gd.Nxh); kt *= 2) {
+            const int ppt = z_inplace_ppt(h->pz, kt);
+            if (!ppt) continue;
+            for (int i : members) {
+                e->tiles[i].KT = kt;
+                e->tiles[i].zppt = ppt;
+            }
+            float ms;
+            if (int rc = ens_time_class(e, B_ZINV, &ms)) return rc;
+            if (ms < best) {
+                best = ms;
+                keep.KT = kt;
+                keep.zppt = ppt;
+            }
+        }
+        for (int i : members) {
+            e->tiles[i].KT = keep.KT;
+            e->tiles[i].zppt = keep.zppt;
+        }
+        // x wavefronts per block (LDS permitting)
+        best = 1e30f;
+        int wkeep = e->tiles[members[0]].xwaves;
+        for (int w = 1; w <= h->xwaves; w *= 2) {
+            for (int i : members) e->tiles[i].xwaves = w;
+            float ms;
+            if (int rc = ens_time_class(e, B_X, &ms)) return rc;
+            if (ms < best) {
+                best = ms;
+                wkeep = w;
+            }
+        }
+        for (int i : members) e->tiles[i].xwaves = wkeep;
+    }
+    // the tables built while timing are replaced by the final build
+    ens_free(e);
+    return 0;
+}
+
+extern "C" int cf_ensemble_create(cf_problem* const* problems, double* const* fields, int n, int tune, cf_ensemble** out) {
+    if (!problems || !fields || !out || n < 1) return fail(CF_ERR_ARGUMENT, "need n >= 1 problems and their fields");
+    *out = nullptr;
+    std::vector<const void*> all;
+    for (int i = 0; i < n; ++i) {
+        cf_problem* h = problems[i];
+        if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+        for (int j = 0; j < i; ++j)
+            if (problems[j] == h) return fail(CF_ERR_ARGUMENT, "each instance needs its own problem (workspaces)");
+        if (h->device != problems[0]->device) return fail(CF_ERR_ARGUMENT, "ensemble problems must share a device");
+        if (h->mixed) return fail(CF_ERR_UNSUPPORTED, "ensembles run the FP64 step");
+        if (h->form == CF_FORM_DIVERGENCE || h->gd.Nxp % 2 != 0)
+            return fail(CF_ERR_UNSUPPORTED, "ensembles need the rotating or convective form and an even padded Nx");
+        for (int f = 0; f < 4; ++f) all.push_back(fields[4 * i + f]);
+    }
+    if (int rc = check_fields(all.data(), (int)all.size())) return rc;
+    DeviceGuard dg(problems[0]->device);
+    cf_ensemble* e = new cf_ensemble();
+    e->device = problems[0]->device;
+    auto bail = [&](int code) {
+        if (e->cap) {
+            hipGraph_t g = nullptr;
+            hipStreamCaptureStatus cs;
+            if (hipStreamIsCapturing(e->cap, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive)
+                (void)hipStreamEndCapture(e->cap, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        ens_free(e);
+        delete e;
+        return code;
+    };
+    for (int i = 0; i < n; ++i) {
+        cf_problem* h = problems[i];
+        e->probs.push_back(h);
+        e->fields.push_back({(double2*)fields[4 * i], (double2*)fields[4 * i + 1], (double2*)fields[4 * i + 2],
+                             (double2*)fields[4 * i + 3]});
+        e->tiles.push_back(Tiles{h->KT, h->zppt, h->xwaves});
+        if (!h->zppt) e->tiles.back().KT = h->KT;  // two-buffer z plans keep the problem's width
+    }
+    if (hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CF_ERR_DEVICE, "hipStreamCreate"));
+    if (tune) {
+        if (int rc = ens_tune(e)) return bail(rc);
+        if (hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking) != hipSuccess)
+            return bail(fail(CF_ERR_DEVICE, "hipStreamCreate"));
+    }
+    if (int rc = ens_build(e, e->step)) return bail(rc);
+    if (hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed) != hipSuccess)
+        return bail(fail(CF_ERR_DEVICE, "hipStreamBeginCapture"));
+    for (const BatchLaunch& L : e->step) {
+        hipError_t err = run_batch(L, e->cap);
+        if (err != hipSuccess) return bail(fail(CF_ERR_DEVICE, std::string("ensemble launch: ") + hipGetErrorString(err)));
+    }
+    if (hipStreamEndCapture(e->cap, &e->graph) != hipSuccess) return bail(fail(CF_ERR_DEVICE, "hipStreamEndCapture"));
+    if (hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0) != hipSuccess)
+        return bail(fail(CF_ERR_DEVICE, "hipGraphInstantiate"));
+    *out = e;
+    return CF_OK;
+}
+
+extern "C" int cf_ensemble_advance(cf_ensemble* e, int nsteps, void* stream) {
+    if (!e) return fail(CF_ERR_ARGUMENT, "null ensemble");
+    if (nsteps < 0) return fail(CF_ERR_ARGUMENT, "nsteps must be nonnegative");
+    DeviceGuard dg(e->device);
+    for (int s = 0; s < nsteps; ++s) HIPTRY(hipGraphLaunch(e->exec, (hipStream_t)stream));
+    return CF_OK;
+}
+
+extern "C" int cf_ensemble_info(const cf_ensemble* e, int* launches, int* tiles) {
+    if (!e) return fail(CF_ERR_ARGUMENT, "null ensemble");
+    if (launches) *launches = (int)e->step.size();
+    if (tiles)
+        for (size_t i = 0; i < e->tiles.size(); ++i) {
+            tiles[2 * i] = e->tiles[i].KT;
+            tiles[2 * i + 1] = e->tiles[i].xwaves;
+        }
+    return CF_OK;
+}
+
+extern "C" int cf_ensemble_destroy(cf_ensemble* e) {
+    if (!e) return CF_OK;
+    DeviceGuard dg(e->device);
+    if (e->cap) (void)hipStreamSynchronize(e->cap);
+    (void)hipDeviceSynchronize();
+    ens_free(e);
+    delete e;
+    return CF_OK;
 }

@@ -653,6 +653,18 @@ __device__ __noinline__ void stockham_pass_generic(const C* src, C* dst, int n, 
     }
 }
 
+// Instance of a batched (ensemble) launch owning block b: the largest i < n
+// with start[i] <= b (start is ascending, start[0] = 0).  Uniform per block.
+__device__ __forceinline__ int batch_instance(const unsigned* __restrict__ start, int n, unsigned b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Ordering of LDS accesses among the lanes of one wavefront: LDS operations
 // of a wave are performed in issue order, so a compiler-level barrier with
 // wavefront-scope fences is all a wave-private exchange needs.

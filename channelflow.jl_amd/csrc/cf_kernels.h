@@ -105,4 +105,17 @@ int x_phys_lines(int mode);  // physical lines per (zp, y) line of the odd-Nxp x
 int x_max_waves(const XArgs& a);  // even Nxp: wavefronts per block that fit the LDS (0: none)
 int x_inplace_ppt(const XArgs& a);  // even Nxp: in-place points per lane for the mode (0: two buffers)
 
+// Batched ensemble launches (FP64): instance i of n owns blocks
+// [start[i], start[i+1]) of an nblocks grid and reads its arguments from
+// tab[i] (device memory).  All instances of one launch share the template
+// parameters named in the call; lds is the largest instance's need.
+hipError_t launch_ykernel_batch(int which, int L, const YArgs* tab, const unsigned* start, int n, unsigned nblocks,
+                                size_t lds, hipStream_t st);  // Y_STAGE or Y_PROLOGUE
+hipError_t launch_zpass_batch(int dir, int ppt, const ZArgs* tab, const unsigned* start, int n, unsigned nblocks,
+                              size_t lds, hipStream_t st);
+hipError_t launch_xpass_batch(int mode, int ppt, int waves, const XArgs* tab, const unsigned* start, int n,
+                              unsigned nblocks, size_t lds, hipStream_t st);  // rotating / convective, even Nxp
+unsigned z_blocks(const ZArgs& a);  // grid of one instance's z pass
+unsigned x_blocks(const XArgs& a);  // grid of one instance's even-Nxp x pass
+
 }  // namespace cf

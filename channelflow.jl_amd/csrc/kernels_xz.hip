@@ -33,8 +33,9 @@ __device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned nblocks) {
 // 1-D grid of ntiles * Ny * nfields blocks, k tile fastest in logical order.
 // C = double2: FP64 fields; C = float2: FP32 fields of the mixed step (in-place
 // plans only, twiddles rounded from the FP64 table).
+// lb: the block's logical index within its instance (after the XCD remap)
 template <int DIR, int PPT, typename C>
-__global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
+__device__ __forceinline__ void zpass_body(const ZArgs& a, unsigned lb) {
     using T = creal_t<C>;
     extern __shared__ double2 smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
@@ -43,7 +44,6 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
     C* bufA = smem;
     C* bufB = smem + KT * ld;
     const unsigned ntiles = (unsigned)((gd.Nxh + KT - 1) / KT);
-    const unsigned lb = xcd_logical(blockIdx.x, gridDim.x);
     const unsigned tile = lb % ntiles, plane = lb / ntiles;
     const int y = (int)(plane % (unsigned)gd.Ny), f = (int)(plane / (unsigned)gd.Ny);
     const int k0 = (int)tile * KT;
@@ -102,6 +102,21 @@ __global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
             if (k < Nxh) out[k + (size_t)Nxh * l] = res[kk * ld + z];
         }
     }
+}
+
+template <int DIR, int PPT, typename C>
+__global__ void __launch_bounds__(ZNT) zpass_kernel(ZArgs a) {
+    zpass_body<DIR, PPT, C>(a, xcd_logical(blockIdx.x, gridDim.x));
+}
+
+// Batched ensemble launch: logical blocks (XCD-remapped over the whole grid)
+// [start[i], start[i+1]) belong to instance i with arguments tab[i].
+template <int DIR, int PPT>
+__global__ void __launch_bounds__(ZNT) zpass_batch_kernel(const ZArgs* __restrict__ tab,
+                                                          const unsigned* __restrict__ start, int n) {
+    const unsigned lb = xcd_logical(blockIdx.x, gridDim.x);
+    const int i = batch_instance(start, n, lb);
+    zpass_body<DIR, PPT, double2>(tab[i], lb - start[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,7 +247,7 @@ __device__ __forceinline__ C x_coef(const XArgs& a, int f, long long gl, int k, 
 }
 
 template <int MODE, int PPT, typename C>
-__device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem) {
+__device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned block) {
     using T = creal_t<C>;
     constexpr int NIN = x_fields_in(MODE), NFFT = x_fft_lines_in(MODE), NOUT = x_fields_out(MODE);
     constexpr int NB = x_buffer_lines(MODE);
@@ -246,7 +261,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem) {
     for (int i = threadIdx.x; i <= M; i += blockDim.x) twx[i] = from_c64<C>(a.twx[i]);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long long gl = (long long)blockIdx.x * a.waves + wave;
+    const long long gl = (long long)block * a.waves + wave;
     if (gl >= nlines) return;  // whole wavefront leaves; no block barrier follows
     C* bufA = smem + 2 * M + 1 + (size_t)wave * x_region_slots(MODE, PPT, ld);
     if constexpr (PPT > 0) {
@@ -388,7 +403,17 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem) {
 template <int MODE, int PPT, typename C>
 __global__ void __launch_bounds__(XNT) xwave_kernel(XArgs a) {
     extern __shared__ double2 smem[];
-    xpass_wave<MODE, PPT>(a, reinterpret_cast<C*>(smem));
+    xpass_wave<MODE, PPT>(a, reinterpret_cast<C*>(smem), blockIdx.x);
+}
+
+// Batched ensemble launch (FP64); every instance of the launch has the same
+// wavefronts per block (the block size) and in-place width.
+template <int MODE, int PPT>
+__global__ void __launch_bounds__(XNT) xwave_batch_kernel(const XArgs* __restrict__ tab,
+                                                          const unsigned* __restrict__ start, int n) {
+    extern __shared__ double2 smem[];
+    const int i = batch_instance(start, n, blockIdx.x);
+    xpass_wave<MODE, PPT>(tab[i], smem, blockIdx.x - start[i]);
 }
 
 __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
@@ -496,6 +521,33 @@ hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+unsigned z_blocks(const ZArgs& a) {
+    return (unsigned)((a.gd.Nxh + a.KT - 1) / a.KT) * (unsigned)a.gd.Ny * (unsigned)a.nfields;
+}
+
+hipError_t launch_zpass_batch(int dir, int ppt, const ZArgs* tab, const unsigned* start, int n, unsigned nblocks,
+                              size_t lds, hipStream_t st) {
+#define CF_ZB(D, P)                                                                                            \
+    {                                                                                                          \
+        if (lds > 64 * 1024)                                                                                   \
+            (void)hipFuncSetAttribute((const void*)zpass_batch_kernel<D, P>,                                   \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                   \
+        hipLaunchKernelGGL((zpass_batch_kernel<D, P>), dim3(nblocks), dim3(ZNT), lds, st, tab, start, n);      \
+        return hipGetLastError();                                                                              \
+    }
+    if (dir > 0) {
+        if (ppt == 8) CF_ZB(+1, 8)
+        if (ppt == 16) CF_ZB(+1, 16)
+        if (ppt == 0) CF_ZB(+1, 0)
+    } else {
+        if (ppt == 8) CF_ZB(-1, 8)
+        if (ppt == 16) CF_ZB(-1, 16)
+        if (ppt == 0) CF_ZB(-1, 0)
+    }
+#undef CF_ZB
+    return hipErrorInvalidValue;
+}
+
 static bool x_wave_path(const XArgs& a) {
     return a.gd.Nxp % 2 == 0 && (a.mode == X_ROTATIONAL || a.mode == X_CONVECTIVE || a.mode == X_DIVERGENCE);
 }
@@ -526,6 +578,36 @@ size_t x_lds_bytes(const XArgs& a) {
     if (x_wave_path(a)) return x_wave_lds(a, a.waves);
     int nphys = x_phys_lines(a.mode);
     return 2 * (size_t)a.LPB * a.ld * sizeof(double2) + (size_t)nphys * a.LPB * a.gd.Nxp * sizeof(double);
+}
+
+unsigned x_blocks(const XArgs& a) {
+    const long long nlines = (long long)a.gd.Nzp * a.gd.Ny;
+    return (unsigned)((nlines + a.waves - 1) / a.waves);
+}
+
+hipError_t launch_xpass_batch(int mode, int ppt, int waves, const XArgs* tab, const unsigned* start, int n,
+                              unsigned nblocks, size_t lds, hipStream_t st) {
+    if (waves < 1 || waves > XNT / 64) return hipErrorInvalidValue;
+    const dim3 block(64 * waves);
+#define CF_XB(M, P)                                                                                            \
+    {                                                                                                          \
+        if (lds > 64 * 1024)                                                                                   \
+            (void)hipFuncSetAttribute((const void*)xwave_batch_kernel<M, P>,                                   \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                   \
+        hipLaunchKernelGGL((xwave_batch_kernel<M, P>), dim3(nblocks), block, lds, st, tab, start, n);          \
+        return hipGetLastError();                                                                              \
+    }
+#define CF_XBM(M)                        \
+    {                                    \
+        if (ppt == 9) CF_XB(M, 9)        \
+        if (ppt == 18) CF_XB(M, 18)      \
+        if (ppt == 0) CF_XB(M, 0)        \
+    }
+    if (mode == X_ROTATIONAL) CF_XBM(X_ROTATIONAL)
+    if (mode == X_CONVECTIVE) CF_XBM(X_CONVECTIVE)
+#undef CF_XBM
+#undef CF_XB
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_xpass(const XArgs& a, hipStream_t st) {

@@ -65,12 +65,14 @@ struct Tile {
     // twiddle table of the DCT plan (ntw entries) and the per-degree table
     // (y_table_slots<L>), see load_tables
     const double* tab;
-    __device__ __forceinline__ Tile(const GridDims& g_, double2* smem, int ntw, int ld_, int region)
+    // block: this block's index within its launch (or within its instance
+    // of a batched ensemble launch)
+    __device__ __forceinline__ Tile(const GridDims& g_, double2* smem, int ntw, int ld_, int region, unsigned block)
         : gd(g_), ld(ld_) {
         const int tid = threadIdx.x, wave = tid >> 6;
         lane = tid & 63;
         mw = lane / L;
-        sw0 = (long long)blockIdx.x * M + wave * W;
+        sw0 = (long long)block * M + wave * W;
         s = sw0 + mw;
         tw = smem;
         tab = reinterpret_cast<const double*>(smem + ntw);
@@ -414,12 +416,10 @@ __device__ __forceinline__ CVec laplacian(const Group<L>& g, const CVec& a, doub
 // ---------------------------------------------------------------------------
 // SC: storage of the y-physical fields (double2; float2 in the mixed step).
 template <int L, typename SC>
-// two waves per SIMD (<= 256 registers per lane): one wave per SIMD leaves
-// the wave-serial scans and LDS FFTs unhidden
-__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
+__device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny;
@@ -513,12 +513,19 @@ __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) y
     if (a.out6[0]) inverse_six<L, SC>(tile, a, g, mi, Un);
 }
 
+template <int L, typename SC>
+// two waves per SIMD (<= 256 registers per lane): one wave per SIMD leaves
+// the wave-serial scans and LDS FFTs unhidden
+__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
+    ystage_body<L, SC>(a, blockIdx.x);
+}
+
 // prologue: state U -> six y-physical fields (velocity and vorticity)
 template <int L, typename SC>
-__global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
+__device__ __forceinline__ void yprologue_body(const YArgs& a, unsigned block) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     CVec U[3];
@@ -527,12 +534,34 @@ __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     inverse_six<L, SC>(tile, a, g, mi, U);
 }
 
+template <int L, typename SC>
+__global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
+    yprologue_body<L, SC>(a, blockIdx.x);
+}
+
+// Batched ensemble launches: instance i owns blocks [start[i], start[i+1])
+// of the grid and its own argument block tab[i] (grid, factors, fields).
+// All instances of one launch share the group size L.
+template <int L>
+__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2)))
+ystage_batch_kernel(const YArgs* __restrict__ tab, const unsigned* __restrict__ start, int n) {
+    const int i = batch_instance(start, n, blockIdx.x);
+    ystage_body<L, double2>(tab[i], blockIdx.x - start[i]);
+}
+
+template <int L>
+__global__ void __launch_bounds__(YNT) yprologue_batch_kernel(const YArgs* __restrict__ tab,
+                                                              const unsigned* __restrict__ start, int n) {
+    const int i = batch_instance(start, n, blockIdx.x);
+    yprologue_body<L, double2>(tab[i], blockIdx.x - start[i]);
+}
+
 // Stokes solve! of a bank: R (3 spectral) -> U (3), P (stokes.jl:81-108)
 template <int L>
 __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     CVec R[3];
@@ -566,7 +595,7 @@ template <int L>
 __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < 3; ++c) {
@@ -584,7 +613,7 @@ template <int L>
 __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int P = a.gd.P, Ny = a.gd.Ny;
@@ -643,7 +672,7 @@ template <int L>
 __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < a.nfields; ++c) {
@@ -666,7 +695,7 @@ template <int L>
 __global__ void __launch_bounds__(YNT) ydivergence_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     CVec N[3];
@@ -705,7 +734,7 @@ template <int L>
 __global__ void __launch_bounds__(YNT) ydot_kernel(YArgs a, int curl) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny;
@@ -774,7 +803,7 @@ template <int L>
 __global__ void __launch_bounds__(YNT) ystats_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld));
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny, P = a.gd.P;
@@ -950,6 +979,33 @@ size_t y_lds_bytes(int Ny, int ld) {
         default: return 0;
     }
     return (2 * (size_t)(Ny - 1) + (size_t)CHEB_TAB_ARRAYS * 4 * L / 2 + (YNT / 64) * (size_t)region) * sizeof(double2);
+}
+
+template <int L>
+static hipError_t launch_y_batch(int which, const YArgs* tab, const unsigned* start, int n, unsigned nblocks, size_t lds,
+                                 hipStream_t st) {
+    if (which == Y_STAGE) {
+        allow_lds(ystage_batch_kernel<L>, lds);
+        hipLaunchKernelGGL(ystage_batch_kernel<L>, dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
+    } else if (which == Y_PROLOGUE) {
+        allow_lds(yprologue_batch_kernel<L>, lds);
+        hipLaunchKernelGGL(yprologue_batch_kernel<L>, dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ykernel_batch(int which, int L, const YArgs* tab, const unsigned* start, int n, unsigned nblocks,
+                                size_t lds, hipStream_t st) {
+    switch (L) {
+        case 4: return launch_y_batch<4>(which, tab, start, n, nblocks, lds, st);
+        case 8: return launch_y_batch<8>(which, tab, start, n, nblocks, lds, st);
+        case 16: return launch_y_batch<16>(which, tab, start, n, nblocks, lds, st);
+        case 32: return launch_y_batch<32>(which, tab, start, n, nblocks, lds, st);
+        case 64: return launch_y_batch<64>(which, tab, start, n, nblocks, lds, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {

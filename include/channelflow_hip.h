@@ -173,6 +173,28 @@ int cf_gaussian_noise(cf_problem* h, double epsilon, unsigned long long seed, do
 int cf_random_state(cf_problem* h, double epsilon, unsigned long long seed, double* U0, double* U1, double* U2, double* P,
                     void* stream);
 
+/* Ensembles.  The reference advances one problem per step! call and runs
+ * concurrent trajectories as independent problems (cnrk2.jl:147,
+ * docs/src/gpu.md:42); an ensemble advances n such problems together, each
+ * exactly as cf_advance would (bit-identical results), with every kernel
+ * class of the step issued as one launch per group of instances that share
+ * its tile parameters, and the whole ensemble step captured once into a HIP
+ * graph.  problems: n distinct handles (FP64 precision, rotating or
+ * convective form, even padded Nx) on one device; fields: 4n DEVICE
+ * pointers, (U0, U1, U2, P) of instance i at 4i..4i+3, fixed for the life of
+ * the ensemble.  tune != 0 picks each grid's z tile width and x wavefronts per
+ * block by timing the candidates on the device (workspaces only; results do
+ * not depend on the tiling).  Each instance uses its own problem's
+ * constraint.  Synchronises at creation. */
+typedef struct cf_ensemble cf_ensemble;
+int cf_ensemble_create(cf_problem* const* problems, double* const* fields, int n, int tune, cf_ensemble** out);
+/* nsteps ensemble steps (graph launches) on `stream` */
+int cf_ensemble_advance(cf_ensemble* e, int nsteps, void* stream);
+/* launches per ensemble step; tiles (may be NULL): 2n ints, z tile width and
+ * x wavefronts per block of each instance */
+int cf_ensemble_info(const cf_ensemble* e, int* launches, int* tiles);
+int cf_ensemble_destroy(cf_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif
