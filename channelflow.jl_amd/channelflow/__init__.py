@@ -214,15 +214,15 @@ class ChannelFlowProblem:
 
     def __init__(self, grid, profile, nu, dt, form=RotatingForm, forcing=None, pressuregradient=None,
                  bulkvelocity=None, device=0, precision="fp64"):
-        if forcing is not None:
-            raise NotImplementedError("custom forcing is not available on the device path")
+        if forcing is not None and not callable(forcing):
+            raise ArgumentError("forcing must be callable as forcing(t, U, F)")
         kind, vals = _constraint(pressuregradient, bulkvelocity)
         self.grid = grid
         self.baseflow = np.asarray(profile(grid.y) if callable(profile) else profile, dtype=np.float64)
         self.baseflow = np.broadcast_to(self.baseflow, (grid.Ny,)).copy()
         self.scheme = CNRK2(grid, self.baseflow, nu, dt, form, (kind, vals), device, precision)
         self.nlterm = NonLinearTerm(self.scheme)
-        self.forcing = None
+        self.forcing = forcing
         self.constraint = {"bulkvelocity": vals} if kind == 1 else {"pressuregradient": vals}
         self.form = form
 
@@ -258,10 +258,47 @@ def _check_vec(g, U):
         _check_spec(g, U[c])
 
 
+class _Forcing:
+    """forcing(tstage, U, F) (cnrk2.jl:173-175) installed on the problem
+    handle for one call: the library calls back at every stage; the callback
+    zeroes F and lets `forcing` add its spectral acceleration into F with
+    device operations enqueued on the step's stream (the callback runs while
+    the step is being enqueued, so it must not read U on the host)."""
+
+    def __init__(self, scheme, U, forcing):
+        self.scheme, self.U, self.forcing = scheme, U, forcing
+        self.F = scheme.grid.zeros_spec(3)
+        self.error = None
+
+        def cb(user, tstage, u0, u1, u2, f0, f1, f2, stream):
+            try:
+                self.F.zero_()
+                self.forcing(tstage, self.U, self.F)
+                return 0
+            except BaseException as e:  # reported after the call returns
+                self.error = e
+                return 1
+
+        self.fn = _lib.FORCING_FN(cb)
+
+    def __enter__(self):
+        F = self.F
+        _lib.check(_lib.load().cf_set_forcing(self.scheme.h, self.fn, None, _ptr(F[0]), _ptr(F[1]), _ptr(F[2])))
+        return self
+
+    def __exit__(self, *exc):
+        _lib.load().cf_set_forcing(self.scheme.h, _lib.FORCING_FN(), None, None, None, None)
+        if self.error is not None:  # the callback's own exception, not the library's status
+            raise self.error
+        return False
+
+
 def step_(scheme, nlterm, U, P, t, forcing=None, pressuregradient=None, bulkvelocity=None, nsteps=1, stream=None):
-    """Advance (U, P) in place by scheme.dt (nsteps times); return the new time."""
-    if forcing is not None:
-        raise NotImplementedError("custom forcing is not available on the device path")
+    """Advance (U, P) in place by scheme.dt (nsteps times); return the new time.
+    forcing(tstage, U, F) adds a spectral acceleration into F at the stage
+    times t + (0, 1/3, 3/4) dt (cnrk2.jl:149-207)."""
+    if forcing is not None and not callable(forcing):
+        raise ArgumentError("forcing must be callable as forcing(t, U, F)")
     if not isinstance(nlterm, NonLinearTerm) or nlterm.scheme is not scheme:
         raise ArgumentError("nlterm must be the problem's nonlinear term")
     kind, vals = _constraint(pressuregradient, bulkvelocity)
@@ -270,8 +307,16 @@ def step_(scheme, nlterm, U, P, t, forcing=None, pressuregradient=None, bulkvelo
     _check_spec(g, P)
     arr = (C.c_double * 2)(*vals)
     tout = C.c_double()
-    _lib.check(_lib.load().cf_advance(scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), float(t), int(nsteps),
-                                      kind, arr, C.byref(tout), _stream(stream)))
+
+    def run():
+        _lib.check(_lib.load().cf_advance(scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), float(t),
+                                          int(nsteps), kind, arr, C.byref(tout), _stream(stream)))
+
+    if forcing is None:
+        run()
+    else:
+        with _Forcing(scheme, U, forcing):
+            run()
     return tout.value
 
 
@@ -293,8 +338,16 @@ class Flow:
         kind, vals = _constraint(p.constraint.get("pressuregradient"), p.constraint.get("bulkvelocity"))
         arr = (C.c_double * 2)(*vals)
         tout = C.c_double()
-        _lib.check(_lib.load().cf_advance_to(p.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), t0, T, kind, arr,
-                                             C.byref(tout), _stream(stream)))
+
+        def run():
+            _lib.check(_lib.load().cf_advance_to(p.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), t0, T, kind,
+                                                 arr, C.byref(tout), _stream(stream)))
+
+        if p.forcing is None:  # forcing=channel.forcing (ext/ChannelFlowFlowsExt.jl:48-49)
+            run()
+        else:
+            with _Forcing(p.scheme, U, p.forcing):
+                run()
         return state
 
 

@@ -41,7 +41,11 @@ SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_probl
            "cf_step_timed",
            "cf_last_gradient", "cf_forward", "cf_inverse", "cf_nonlinear", "cf_stokes_solve", "cf_project",
            "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input", "cf_plane_statistics", "cf_advance_to", "cf_gaussian_noise", "cf_random_state",
-           "cf_ensemble_create", "cf_ensemble_advance", "cf_ensemble_info", "cf_ensemble_destroy"]
+           "cf_ensemble_create", "cf_ensemble_advance", "cf_ensemble_info", "cf_ensemble_destroy", "cf_set_forcing"]
+
+# int (*cf_forcing_fn)(void* user, double tstage, const double* U0, U1, U2, double* F0, F1, F2, void* stream)
+FORCING_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                         C.c_void_p, C.c_void_p, C.c_void_p)
 
 
 def load(path=LIB_PATH):
@@ -78,6 +82,7 @@ def load(path=LIB_PATH):
     L.cf_ensemble_advance.argtypes = [vp, i, vp]
     L.cf_ensemble_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.cf_ensemble_destroy.argtypes = [vp]
+    L.cf_set_forcing.argtypes = [vp, FORCING_FN, vp, vp, vp, vp]
     for name in SYMBOLS:
         getattr(L, name).restype = C.c_int if name != "cf_last_error" else C.c_char_p
     _lib = L

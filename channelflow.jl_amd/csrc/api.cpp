@@ -88,6 +88,10 @@ struct cf_problem {
     int xwaves = 4;  // even Nxp: wavefronts per block of the nonlinear x pass
     int zppt = 0;    // z pass in-place points per thread (0: two buffers)
     int xppt = 0;    // even-Nxp x pass in-place points per lane (0: two buffers)
+    // forcing(tstage, U, F) hook: host callback and the caller's F fields
+    cf_forcing_fn force_fn = nullptr;
+    void* force_user = nullptr;
+    double2* F[3] = {nullptr, nullptr, nullptr};
     // mixed-precision step: FP32 y-physical fields and periodic passes
     int mixed = 0;
     int KT32 = 1, zppt32 = 0, xppt32 = 0, xwaves32 = 0;
@@ -566,7 +570,7 @@ static int tmark(Timer* tm, int c, hipStream_t st) { return tm ? tm->mark(c, st)
 // Arguments of CNRK2 stage j (cnrk2.jl:168-204) for step length dt with
 // solver bank `bank`; Ncoef: the divergence form's coefficient-space term.
 static YArgs stage_args(cf_problem* h, double2* const U[3], double2* P, int j, const Constraint& c, double dt,
-                        const DevBank* bank, bool mixed, double2* const Ncoef[3]) {
+                        const DevBank* bank, bool mixed, double2* const Ncoef[3], bool forced = false) {
     const GridDims& gd = h->gd;
     YArgs a = base_yargs(h);
     a.mixed = mixed ? 1 : 0;
@@ -583,6 +587,7 @@ static YArgs stage_args(cf_problem* h, double2* const U[3], double2* P, int j, c
     for (int i = 0; i < 3; ++i) {
         a.Nphys[i] = six_field(h, i, mixed);
         a.Ncoef[i] = (h->form == CF_FORM_DIVERGENCE) ? Ncoef[i] : nullptr;
+        a.Fcoef[i] = forced ? h->F[i] : nullptr;
         a.U[i] = U[i];
         a.Q[i] = fld(h->Q, gd.S, i);
     }
@@ -593,12 +598,15 @@ static YArgs stage_args(cf_problem* h, double2* const U[3], double2* P, int j, c
     return a;
 }
 
+static const double STAGE_T[3] = {0.0, 1.0 / 3.0, 3.0 / 4.0};  // stage times, cnrk2.jl:170
+
 // nsteps CNRK2 steps of length dt with the stage solver banks `banks`
 // (default: the problem's dt and its three stage banks)
 static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
-                         Timer* tm = nullptr, double dt = 0.0, DevBank* const* banks = nullptr) {
+                         Timer* tm = nullptr, double dt = 0.0, DevBank* const* banks = nullptr, double t0 = 0.0) {
     const GridDims& gd = h->gd;
     const bool mixed = h->mixed != 0;
+    const bool forced = h->force_fn != nullptr;
     if (!(dt > 0.0)) dt = h->dt;
     for (int step = 0; step < nsteps; ++step) {
         if (int rc = tmark(tm, -1, st)) return rc;
@@ -609,7 +617,13 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
             double2* Ncoef[3] = {fld(h->R, gd.S, 0), fld(h->R, gd.S, 1), fld(h->R, gd.S, 2)};
             if (h->form == CF_FORM_DIVERGENCE)
                 if (int rc = enqueue_nonlinear_coef(h, Ncoef, st)) return rc;
-            const YArgs a = stage_args(h, U, P, j, c, dt, banks ? banks[j] : &h->stage[j], mixed, Ncoef);
+            if (forced) {
+                const double ts = t0 + (double)step * dt + STAGE_T[j] * dt;
+                if (h->force_fn(h->force_user, ts, (const double*)U[0], (const double*)U[1], (const double*)U[2],
+                                (double*)h->F[0], (double*)h->F[1], (double*)h->F[2], st) != 0)
+                    return fail(CF_ERR_ARGUMENT, "forcing callback failed");
+            }
+            const YArgs a = stage_args(h, U, P, j, c, dt, banks ? banks[j] : &h->stage[j], mixed, Ncoef, forced);
             HIPTRY(launch_ykernel(Y_STAGE, a, st));
             if (int rc = tmark(tm, 4, st)) return rc;
         }
@@ -628,7 +642,7 @@ extern "C" int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, dou
     if (int rc = resolve_constraint(h, constraint, values, c)) return rc;
     DeviceGuard dg(h->device);
     double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
-    if (int rc = enqueue_steps(h, U, (double2*)P, nsteps, c, (hipStream_t)stream)) return rc;
+    if (int rc = enqueue_steps(h, U, (double2*)P, nsteps, c, (hipStream_t)stream, nullptr, 0.0, nullptr, t)) return rc;
     if (t_out) *t_out = t + nsteps * h->dt;
     return CF_OK;
 }
@@ -652,14 +666,28 @@ extern "C" int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, 
     double rem = span - (double)n * h->dt;
     if (rem <= tol) rem = 0.0;
     if (n > (1ll << 30)) return fail(CF_ERR_ARGUMENT, "too many steps");
-    if (int rc = enqueue_steps(h, U, (double2*)P, (int)n, c, st)) return rc;
+    if (int rc = enqueue_steps(h, U, (double2*)P, (int)n, c, st, nullptr, 0.0, nullptr, t)) return rc;
     if (rem > 0.0) {
         DevBank* b[3];
         for (int j = 0; j < 3; ++j)
             if (int rc = get_bank(h, h->nu, 1.0 / (CNRK2_C[j] * rem), &b[j])) return rc;
-        if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, nullptr, rem, b)) return rc;
+        if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, nullptr, rem, b, t + (double)n * h->dt)) return rc;
     }
     if (t_out) *t_out = T;
+    return CF_OK;
+}
+
+extern "C" int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, double* F0, double* F1, double* F2) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    if (fn) {
+        const void* f[3] = {F0, F1, F2};
+        if (int rc = check_fields(f, 3)) return rc;
+    }
+    h->force_fn = fn;
+    h->force_user = fn ? user : nullptr;
+    h->F[0] = fn ? (double2*)F0 : nullptr;
+    h->F[1] = fn ? (double2*)F1 : nullptr;
+    h->F[2] = fn ? (double2*)F2 : nullptr;
     return CF_OK;
 }
 
@@ -679,7 +707,7 @@ extern "C" int cf_step_timed(cf_problem* h, double* U0, double* U1, double* U2, 
     hipStream_t st = (hipStream_t)stream;
     double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
     Timer tm;
-    if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, &tm)) return rc;
+    if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, &tm, 0.0, nullptr, t)) return rc;
     HIPTRY(hipEventSynchronize(tm.ev.back()));
     for (int i = 0; i < Timer::NCLASS; ++i) ms[i] = 0.0;
     for (size_t i = 1; i < tm.ev.size(); ++i) {
@@ -1276,6 +1304,7 @@ extern "C" int cf_ensemble_create(cf_problem* const* problems, double* const* fi
             if (problems[j] == h) return fail(CF_ERR_ARGUMENT, "each instance needs its own problem (workspaces)");
         if (h->device != problems[0]->device) return fail(CF_ERR_ARGUMENT, "ensemble problems must share a device");
         if (h->mixed) return fail(CF_ERR_UNSUPPORTED, "ensembles run the FP64 step");
+        if (h->force_fn) return fail(CF_ERR_UNSUPPORTED, "a forced problem cannot join an ensemble (host callback)");
         if (h->form == CF_FORM_DIVERGENCE || h->gd.Nxp % 2 != 0)
             return fail(CF_ERR_UNSUPPORTED, "ensembles need the rotating or convective form and an even padded Nx");
         for (int f = 0; f < 4; ++f) all.push_back(fields[4 * i + f]);

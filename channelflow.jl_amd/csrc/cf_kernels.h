@@ -45,6 +45,7 @@ struct YArgs {
     double nl_sign;           // sign applied to the transformed nonlinear term
     const double2* Nphys[6];  // y-physical nonlinear term (Y_DIVERGENCE: the six products)
     const double2* Ncoef[3];  // nonlinear term already in coefficients (skip the DCT) or null
+    const double2* Fcoef[3];  // forcing acceleration in coefficients, added to N (or null)
     double2* U[3];
     double2* P;
     double2* Q[3];

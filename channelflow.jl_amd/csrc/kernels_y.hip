@@ -440,6 +440,11 @@ __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) Qc.v[e] = cscale(Qc.v[e], a.nl_sign);
         }
+        if (a.Fcoef[c]) {  // forcing(tstage, U, N): N += F (cnrk2.jl:175)
+            const CVec F = tile.load(a.Fcoef[c], g);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Qc.v[e] = cadd(Qc.v[e], F.v[e]);
+        }
         double bN = 0.0;
         if (a.bulk && meanmode && c != 1) bN = bulk_mean(g, Qc).x;
         if (a.stage != 0) {

@@ -105,6 +105,22 @@ int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, 
 int cf_step_timed(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int constraint,
                   const double* values, double* t_out, double ms[5], void* stream);
 
+/* forcing(tstage, U, F) (cnrk2.jl:173-175, forcing.jl:3-11).  During each
+ * stage of cf_step / cf_advance / cf_advance_to / cf_step_timed, after the
+ * nonlinear term is enqueued, the library calls fn(user, tstage, U0, U1, U2,
+ * F0, F1, F2, stream) on the host with tstage = t + (0, 1/3, 3/4) dt.  fn
+ * must ENQUEUE on `stream` device work that overwrites F0..F2 (spectral
+ * fields owned by the caller, given here) with the forcing acceleration; it
+ * must not read U on the host.  The library then adds F into the nonlinear
+ * term N before the bulk-gradient evaluation and the Q update, as the
+ * reference's forcing adds into N.  fn returns 0 on success; a nonzero
+ * return stops the step with CF_ERR_ARGUMENT (earlier stages may already
+ * be enqueued).  fn = NULL restores NoForcing.  Problems with a forcing
+ * cannot join an ensemble. */
+typedef int (*cf_forcing_fn)(void* user, double tstage, const double* U0, const double* U1, const double* U2,
+                             double* F0, double* F1, double* F2, void* stream);
+int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, double* F0, double* F1, double* F2);
+
 /* Uniform pressure gradient (dPdx, dPdz) returned by the last Stokes solve of
  * the last step (stokes.jl:101-107).  Synchronises the stream; HOST output. */
 int cf_last_gradient(cf_problem* h, double out[2], void* stream);

@@ -947,6 +947,11 @@ struct Problem {
     std::unique_ptr<NonLinear> nl;
     std::vector<cplx> Q[3], N[3], R[3];
     double last_grad[2] = {0, 0};
+    // forcing(tstage, U, N) (cnrk2.jl:173-175, forcing.jl:3-11) of the form
+    // F(t) = F0 + sin(omega t) F1, added into N at the stage times; t advances by dt per step
+    bool forced = false;
+    std::vector<cplx> F0[3], F1[3];
+    double omega = 0.0, t = 0.0;
     Problem(const Grid& g_, const double* basevals, double nu_, double dt_, int form, int kind_, double c0_, double c1_)
         : g(g_), nu(nu_), dt(dt_), kind(kind_), c0(c0_), c1(c1_) {
         base.resize(g.Ny); curv.resize(g.Ny);
@@ -981,9 +986,15 @@ struct Problem {
         double grads[2] = {kind == 0 ? c0 : 0.0, kind == 0 ? c1 : 0.0};
         cplx* Np[3] = {N[0].data(), N[1].data(), N[2].data()};
         cplx* Rp[3] = {R[0].data(), R[1].data(), R[2].data()};
+        static const double STAGE_T[3] = {0.0, 1.0 / 3.0, 3.0 / 4.0};  // cnrk2.jl:170
         for (int j = 0; j < 3; ++j) {
             const cplx* Uc[3] = {U[0], U[1], U[2]};
             nl->eval(Uc, Np, false);
+            if (forced) {
+                const double sw = std::sin(omega * (t + STAGE_T[j] * dt));
+                for (int i = 0; i < 3; ++i)
+                    for (size_t q = 0; q < S; ++q) N[i][q] += F0[i][q] + sw * F1[i][q];
+            }
             double old[2] = {grads[0], grads[1]};
             if (kind == 1) bulkgrad(Uc, old);
             for (int i = 0; i < 3; ++i) {
@@ -1008,6 +1019,7 @@ struct Problem {
             if (kind == 0) { grads[0] = c0; grads[1] = c1; }
             last_grad[0] = gout[0]; last_grad[1] = gout[1];
         }
+        t += dt;
     }
 };
 
@@ -1289,6 +1301,22 @@ int orc_step(void* h, double* U3, double* P, int nsteps) {
     size_t S = p->g.S();
     cplx* U[3] = {(cplx*)U3, (cplx*)U3 + S, (cplx*)U3 + 2 * S};
     for (int s = 0; s < nsteps; ++s) p->step(U, (cplx*)P);
+    return 0;
+}
+// forcing F(t) = F0 + sin(omega t) F1 (3 spectral fields each; F0 == NULL
+// removes it); t0 = time of the next step's first stage
+int orc_set_forcing(void* h, const double* F0, const double* F1, double omega, double t0) {
+    Problem* p = (Problem*)h;
+    size_t S = p->g.S();
+    p->forced = F0 != nullptr;
+    p->omega = omega;
+    p->t = t0;
+    for (int i = 0; i < 3; ++i) {
+        if (!F0) { p->F0[i].clear(); p->F1[i].clear(); continue; }
+        p->F0[i].assign((const cplx*)F0 + i * S, (const cplx*)F0 + (i + 1) * S);
+        if (F1) p->F1[i].assign((const cplx*)F1 + i * S, (const cplx*)F1 + (i + 1) * S);
+        else p->F1[i].assign(S, 0.0);
+    }
     return 0;
 }
 int orc_last_gradient(void* h, double* g2) {
