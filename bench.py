@@ -194,6 +194,107 @@ def bench_ensemble(cf, steps, plain_steps=20):
     }
 
 
+CLOSURE = dict(fine=(128, 97, 128), coarse=(32, 33, 32), L=(4 * math.pi, 4 * math.pi / 3), nu=1 / 2800, dt=0.005,
+               bulk=(1.0, 0.0), amp=0.3,
+               desc="learned subgrid closure: coarse Poiseuille 32x33x32 (Lx=4pi Lz=4pi/3 nu=1/2800 dt=0.005, bulk "
+                    "velocity (1, 0)); MLP 10-64-64-64-3 GELU, BF16 weights/activations on MFMA, FP32 accumulation; "
+                    "trained on box-filtered snapshots of a 128x97x128 FP64 plain run (perturbation amplitude 0.3)")
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_closure(cf, snapshots, every, eval_steps, epochs, sample_every=10):
+    """Config of the learned closure: data from the fine plain run, training
+    on the device, and coarse runs with and without the closure judged
+    against the filtered DNS over the next `eval_steps` steps."""
+    import numpy as np
+    import torch
+    from channelflow import closure as CL
+
+    w = CLOSURE
+    prof = lambda y: 1.5 * (1 - y ** 2)  # noqa: E731
+    gf, gc = cf.Grid(*w["fine"], *w["L"]), cf.Grid(*w["coarse"], *w["L"])
+    fine = cf.ChannelFlowProblem(gf, prof, w["nu"], w["dt"], bulkvelocity=w["bulk"])
+    coarse = cf.ChannelFlowProblem(gc, prof, w["nu"], w["dt"], bulkvelocity=w["bulk"])
+    st = cf.perturbation_state(fine, w["amp"], seed=11)
+    t0 = time.perf_counter()
+    Xs, Ts = [], []
+    for k in range(snapshots):
+        cf.step_(fine.scheme, fine.nlterm, st.velocity, st.stagepressure, 0.0, nsteps=every, **fine.constraint)
+        X, T = CL.snapshot_samples(fine, coarse, st.velocity)
+        Xs.append(X)
+        Ts.append(T)
+        if (k + 1) % 250 == 0:
+            torch.cuda.synchronize()
+            progress(f"closure: {k + 1}/{snapshots} snapshots")
+    ntest = max(1, snapshots // 10)
+    Xtr, Ttr = torch.cat(Xs[:-ntest]), torch.cat(Ts[:-ntest])
+    Xte, Tte = torch.cat(Xs[-ntest:]), torch.cat(Ts[-ntest:])
+    del Xs, Ts
+    torch.cuda.synchronize()
+    data_s = time.perf_counter() - t0
+    net = CL.SubgridMLP(seed=1)
+    net.fit_normalization(Xtr, Ttr)
+    t0 = time.perf_counter()
+    progress(f"closure: training on {Xtr.shape[0]} samples")
+    hist = net.train(Xtr, Ttr, epochs=epochs, batch=8192, lr=2e-3)
+    progress(f"closure: training loss by epoch {hist}")
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    train_samples = Xtr.shape[0]
+    test_loss, test_base = net.evaluate(Xte, Tte)
+    del Xtr, Ttr, Xte, Tte
+    torch.cuda.empty_cache()
+    # a posteriori: filtered DNS and coarse runs from the same filtered state
+    Ub = cf.box_filter(fine, coarse, st.velocity)
+    runs = {"plain": (Ub.clone(), None), "closure": (Ub.clone(), net.forcing(coarse))}
+    stats = {k: cf.ProfileStatistics(gc.Ny) for k in ("dns", "plain", "closure")}
+    ms = {}
+    for name, (U, force) in runs.items():
+        P = cf.pressure(U, coarse)
+        torch.cuda.synchronize()
+        tt = 0.0
+        for s in range(0, eval_steps, sample_every):
+            a = time.perf_counter()
+            cf.step_(coarse.scheme, coarse.nlterm, U, P, s * w["dt"], nsteps=sample_every, forcing=force,
+                     **coarse.constraint)
+            torch.cuda.synchronize()
+            tt += time.perf_counter() - a
+            stats[name].sample(coarse, U, (s + sample_every) * w["dt"])
+        ms[name] = 1000.0 * tt / eval_steps
+        progress(f"closure: coarse {name} run done ({ms[name]:.3f} ms/step)")
+    for s in range(0, eval_steps, sample_every):
+        cf.step_(fine.scheme, fine.nlterm, st.velocity, st.stagepressure, 0.0, nsteps=sample_every, **fine.constraint)
+        stats["dns"].sample(coarse, cf.box_filter(fine, coarse, st.velocity), (s + sample_every) * w["dt"])
+    ref_m, ref_c = stats["dns"].profiles()
+
+    def err(name):
+        m, c = stats[name].profiles()
+        em = float(np.linalg.norm(m[:, 0] - ref_m[:, 0]) / np.linalg.norm(ref_m[:, 0]))
+        ec = [float(np.linalg.norm(c[:, k] - ref_c[:, k]) / max(np.linalg.norm(ref_c[:, k]), 1e-300)) for k in range(6)]
+        return em, ec
+
+    em_p, ec_p = err("plain")
+    em_c, ec_c = err("closure")
+    names = ["uu", "vv", "ww", "uv", "uw", "vw"]
+    return {
+        "workload": w["desc"], "snapshots": snapshots, "steps_between_snapshots": every,
+        "train_samples": train_samples, "epochs": epochs, "train_loss_by_epoch": [round(h, 5) for h in hist],
+        "train_samples_per_s": round(epochs * train_samples / train_s, 1), "data_s": round(data_s, 2),
+        "train_s": round(train_s, 2),
+        "heldout_loss": round(test_loss, 5), "heldout_loss_no_model": round(test_base, 5),
+        "heldout_gain": round(1.0 - test_loss / test_base, 4),
+        "eval_steps": eval_steps,
+        "coarse_plain_steps_per_s": round(1000.0 / ms["plain"], 1),
+        "coarse_closure_steps_per_s": round(1000.0 / ms["closure"], 1),
+        "mean_profile_rel_err": {"plain": round(em_p, 5), "closure": round(em_c, 5)},
+        "reynolds_stress_rel_err": {"plain": dict(zip(names, [round(e, 4) for e in ec_p])),
+                                    "closure": dict(zip(names, [round(e, 4) for e in ec_c]))},
+    }
+
+
 def base_coefficients(problem):
     """Chebyshev coefficients of the problem's base profile (host DCT-I of its
     Lobatto-point values, chebcoeffs normalisation)."""
@@ -244,6 +345,11 @@ def main():
                     help="steps of the FP32-mixed 256x193x256 comparison (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=1000,
                     help="steps of the 64-instance batched ensemble (0: skip)")
+    ap.add_argument("--closure-snapshots", type=int, default=2000,
+                    help="filtered snapshots of the learned-closure workload (0: skip)")
+    ap.add_argument("--closure-every", type=int, default=5, help="fine steps between snapshots")
+    ap.add_argument("--closure-eval-steps", type=int, default=5000)
+    ap.add_argument("--closure-epochs", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -322,6 +428,7 @@ def main():
 
     forms = {}
     del prob, state, U, P
+    progress(f"headline done: {steps_per_s:.1f} steps/s")
     if args.mixed_steps > 0:
         torch.cuda.empty_cache()
         try:
@@ -334,6 +441,13 @@ def main():
             forms["batched"] = bench_ensemble(cf, args.ensemble_steps)
         except Exception as e:
             forms["batched"] = {"error": f"{type(e).__name__}: {e}"}
+    if args.closure_snapshots > 0:
+        torch.cuda.empty_cache()
+        try:
+            forms["closure"] = bench_closure(cf, args.closure_snapshots, args.closure_every, args.closure_eval_steps,
+                                             args.closure_epochs)
+        except Exception as e:
+            forms["closure"] = {"error": f"{type(e).__name__}: {e}"}
 
     out = {
         "metric": "CNRK2 steps/s & %HBM roofline, 128×97×128 FP64; mixed/batched/closure vs plain",

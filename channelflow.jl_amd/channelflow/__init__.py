@@ -26,7 +26,7 @@ __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
            "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
            "power_input", "plane_statistics", "ProfileStatistics", "flow", "device_gaussian_noise",
-           "savefield", "loadfield", "Ensemble"]
+           "savefield", "loadfield", "Ensemble", "velocity_gradient", "box_filter"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -414,6 +414,35 @@ def project_(U, problem):
     _check_vec(problem.grid, U)
     _lib.check(_lib.load().cf_project(problem.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _stream()))
     return U
+
+
+def velocity_gradient(problem, U, out=None, stream=None):
+    """Velocity-gradient tensor of the total velocity U + Ub e_x on the padded
+    grid: a (3, 3, Ny, Nzp, Nxp) float64 device tensor, [i, j] = dV_i/dx_j
+    with x_j = (x, y, z) (cf_velocity_gradient)."""
+    g = problem.grid
+    _check_vec(g, U)
+    if out is None:
+        out = _torch().empty((3, 3) + g.phys_shape, dtype=_torch().float64, device="cuda")
+    _lib.check(_lib.load().cf_velocity_gradient(problem.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(out),
+                                                _stream(stream)))
+    return out
+
+
+def box_filter(fine, coarse, U, out=None, widths=None, stream=None):
+    """Box-filtered coarse spectral field of the fine problem's U
+    (cf_box_filter): top-hat widths (dx, dz), default the coarse grid
+    spacings Lx/Nx and Lz/Nz."""
+    gf, gc = fine.grid, coarse.grid
+    _check_vec(gf, U)
+    if out is None:
+        out = gc.zeros_spec(3)
+    _check_vec(gc, out)
+    dx, dz = widths if widths is not None else (gc.Lx / gc.Nx, gc.Lz / gc.Nz)
+    _lib.check(_lib.load().cf_box_filter(fine.scheme.h, coarse.scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]),
+                                         _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), float(dx), float(dz),
+                                         _stream(stream)))
+    return out
 
 
 def pressure(U, problem, t=0.0):

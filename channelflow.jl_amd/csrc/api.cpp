@@ -95,6 +95,8 @@ struct cf_problem {
     // mixed-precision step: FP32 y-physical fields and periodic passes
     int mixed = 0;
     int KT32 = 1, zppt32 = 0, xppt32 = 0, xwaves32 = 0;
+    // velocity-gradient x pass (12 FFT lines): in-place width, wavefronts per block
+    int xpptg = 0, xwavesg = 0;
 };
 
 struct DeviceGuard {
@@ -312,6 +314,14 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         xa.ppt = h->xppt;
         h->xwaves = x_max_waves(xa);
         if (h->xwaves < 1) return bail(fail(CF_ERR_UNSUPPORTED, "streamwise line exceeds LDS"));
+        {
+            XArgs xg = xa;
+            xg.mode = X_GRADIENT;
+            xg.ppt = 0;
+            h->xpptg = x_inplace_ppt(xg);
+            xg.ppt = h->xpptg;
+            h->xwavesg = x_max_waves(xg);
+        }
         if (cfg->precision == CF_PRECISION_MIXED) {
             xa.mixed = 1;
             xa.ppt = 0;
@@ -332,7 +342,8 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
     }
     // workspaces
     if ((rc = dalloc(h, &h->six, 6 * (size_t)gd.S))) return bail(rc);
-    if ((rc = dalloc(h, &h->zbuf, (h->form == CF_FORM_CONVECTIVE ? 9 : 6) * (size_t)gd.Sz))) return bail(rc);
+    // 9 fields: the convective form and the velocity gradient need V, dV/dy, dV/dz
+    if ((rc = dalloc(h, &h->zbuf, 9 * (size_t)gd.Sz))) return bail(rc);
     if ((rc = dalloc(h, &h->Q, 3 * (size_t)gd.S))) return bail(rc);
     if ((rc = dalloc(h, &h->R, 3 * (size_t)gd.S))) return bail(rc);
     if ((rc = dalloc(h, &h->d_grad, 2))) return bail(rc);
@@ -674,6 +685,146 @@ extern "C" int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, 
         if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, nullptr, rem, b, t + (double)n * h->dt)) return rc;
     }
     if (t_out) *t_out = T;
+    return CF_OK;
+}
+
+// Velocity-gradient tensor of the total velocity V = U + Ub e_x on the padded
+// grid: the convective prologue (V, dV/dy), the z pass with dV/dz, and the
+// gradient x pass (dV/dx = i alpha k V) writing 9 real fields.
+extern "C" int cf_velocity_gradient(cf_problem* h, const double* U0, const double* U1, const double* U2, double* out,
+                                    void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[4] = {U0, U1, U2, out};
+    if (int rc = check_fields(f, 4)) return rc;
+    const GridDims& gd = h->gd;
+    if (gd.Nxp % 2 != 0 || h->xwavesg < 1) return fail(CF_ERR_UNSUPPORTED, "velocity gradient needs an even padded Nx");
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    YArgs a = prologue_args(h, U, false);
+    a.form = FORM_CONVECTIVE;
+    HIPTRY(launch_ykernel(Y_PROLOGUE, a, st));
+    double2* six[6];
+    double2* zb[9];
+    for (int i = 0; i < 6; ++i) six[i] = fld(h->six, gd.S, i);
+    for (int i = 0; i < 9; ++i) zb[i] = fld(h->zbuf, gd.Sz, i);
+    double2* zin[9] = {six[0], six[1], six[2], six[3], six[4], six[5], six[0], six[1], six[2]};
+    if (int rc = zpass(h, +1, 9, zin, zb, st, 0x1C0u)) return rc;
+    XArgs x = base_xargs(h);
+    x.mode = X_GRADIENT;
+    x.ppt = h->xpptg;
+    x.waves = h->xwavesg;
+    const size_t nphys = (size_t)gd.Nxp * gd.Nzp * gd.Ny;
+    for (int i = 0; i < 9; ++i) {
+        x.in[i] = zb[i];
+        x.rout[i] = out + nphys * i;
+    }
+    HIPTRY(launch_xpass(x, st));
+    return CF_OK;
+}
+
+// Box filter (closure data): fine problem's spectral U -> coarse problem's
+// spectral Ubar, both problems on the same box.
+extern "C" int cf_box_filter(cf_problem* fine, cf_problem* coarse, const double* U0, const double* U1, const double* U2,
+                             double* V0, double* V1, double* V2, double dx, double dz, void* stream) {
+    if (!fine || !coarse) return fail(CF_ERR_ARGUMENT, "null problem");
+    const void* f[6] = {U0, U1, U2, V0, V1, V2};
+    if (int rc = check_fields(f, 6)) return rc;
+    if (fine->device != coarse->device) return fail(CF_ERR_ARGUMENT, "problems must share a device");
+    if (std::fabs(fine->Lx - coarse->Lx) > 1e-12 * fine->Lx || std::fabs(fine->Lz - coarse->Lz) > 1e-12 * fine->Lz)
+        return fail(CF_ERR_ARGUMENT, "fine and coarse grids must share Lx and Lz");
+    const GridDims &F = fine->gd, &Cg = coarse->gd;
+    if (!(Cg.Nx / 2 < (F.Nx + 1) / 2 && Cg.Nz / 2 < (F.Nz + 1) / 2 && Cg.Ny <= F.Ny))
+        return fail(CF_ERR_DIMENSION, "the coarse grid must be strictly coarser in x and z and not finer in y");
+    if (!(std::isfinite(dx) && std::isfinite(dz) && dx >= 0 && dz >= 0)) return fail(CF_ERR_ARGUMENT, "bad filter widths");
+    DeviceGuard dg(fine->device);
+    FilterArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.f = F;
+    a.c = Cg;
+    a.dx = dx;
+    a.dz = dz;
+    a.in[0] = (const double2*)U0;
+    a.in[1] = (const double2*)U1;
+    a.in[2] = (const double2*)U2;
+    a.out[0] = (double2*)V0;
+    a.out[1] = (double2*)V1;
+    a.out[2] = (double2*)V2;
+    HIPTRY(launch_filter(a, (hipStream_t)stream));
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Subgrid-force MLP entry points (learned closure)
+// ---------------------------------------------------------------------------
+static MLPInferArgs mlp_args(const float* theta, const unsigned short* wbf, const float* norm, const double* grad9, int Nxp,
+                             int Nzp, int Ny) {
+    MLPInferArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.theta = theta;
+    a.wbf = wbf;
+    a.norm = norm;
+    a.plane = (long long)Nxp * Nzp;
+    a.npts = a.plane * Ny;
+    a.Ny = Ny;
+    for (int c = 0; c < 9; ++c) a.grad[c] = grad9 + (size_t)a.npts * c;
+    return a;
+}
+
+static int mlp_dims_ok(int Nxp, int Nzp, int Ny) {
+    if (Nxp < 1 || Nzp < 1 || Ny < 2) return fail(CF_ERR_ARGUMENT, "bad point grid");
+    if ((long long)Nxp * Nzp * Ny >= (1ll << 40)) return fail(CF_ERR_UNSUPPORTED, "point grid too large");
+    return 0;
+}
+
+extern "C" int cf_mlp_features(const double* grad9, const double* target3, int Nxp, int Nzp, int Ny, float* X, float* T,
+                               void* stream) {
+    if (!grad9 || !X || (target3 && !T)) return fail(CF_ERR_ARGUMENT, "null argument");
+    if (int rc = mlp_dims_ok(Nxp, Nzp, Ny)) return rc;
+    MLPInferArgs a = mlp_args(nullptr, nullptr, nullptr, grad9, Nxp, Nzp, Ny);
+    if (target3)  // the target fields travel in the output slots
+        for (int c = 0; c < 3; ++c) a.out[c] = const_cast<double*>(target3) + (size_t)a.npts * c;
+    HIPTRY(launch_mlp_features(a, X, target3 ? T : nullptr, (hipStream_t)stream));
+    return CF_OK;
+}
+
+extern "C" int cf_mlp_infer(const float* theta, const unsigned short* wbf, const float* norm, const double* grad9, int Nxp,
+                            int Nzp, int Ny, double* force3, void* stream) {
+    if (!theta || !wbf || !norm || !grad9 || !force3) return fail(CF_ERR_ARGUMENT, "null argument");
+    if (int rc = mlp_dims_ok(Nxp, Nzp, Ny)) return rc;
+    MLPInferArgs a = mlp_args(theta, wbf, norm, grad9, Nxp, Nzp, Ny);
+    for (int c = 0; c < 3; ++c) a.out[c] = force3 + (size_t)a.npts * c;
+    HIPTRY(launch_mlp_infer(a, (hipStream_t)stream));
+    return CF_OK;
+}
+
+extern "C" long long cf_mlp_work_floats(long long batch) {
+    return batch > 0 ? (long long)mlp_train_waves(batch) * (MLP_NPARAM + 1) : 0;
+}
+
+extern "C" int cf_mlp_grad(const float* theta, const unsigned short* wbf, const float* norm, const float* X, const float* T,
+                           const int* idx, long long batch, float* work, float* grad, void* stream) {
+    if (!theta || !wbf || !norm || !X || !T || !idx || !work || !grad) return fail(CF_ERR_ARGUMENT, "null argument");
+    if (batch < 1 || batch > (1ll << 30)) return fail(CF_ERR_ARGUMENT, "batch must be in 1..2^30");
+    MLPTrainArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.theta = theta;
+    a.wbf = wbf;
+    a.norm = norm;
+    a.X = X;
+    a.T = T;
+    a.idx = idx;
+    a.batch = batch;
+    a.part = work;
+    HIPTRY(launch_mlp_grad(a, grad, (hipStream_t)stream));
+    return CF_OK;
+}
+
+extern "C" int cf_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v, const float* grad, float lr, int step,
+                           void* stream) {
+    if (!theta || !wbf || !m || !v || !grad) return fail(CF_ERR_ARGUMENT, "null argument");
+    if (step < 1 || !(lr > 0.0f)) return fail(CF_ERR_ARGUMENT, "need step >= 1 and lr > 0");
+    HIPTRY(launch_mlp_adam(theta, wbf, m, v, grad, lr, step, (hipStream_t)stream));
     return CF_OK;
 }
 

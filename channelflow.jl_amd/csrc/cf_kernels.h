@@ -69,7 +69,7 @@ struct ZArgs {
     double2* out[9];
 };
 
-enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD, X_CONVECTIVE, X_DIVERGENCE };
+enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD, X_CONVECTIVE, X_DIVERGENCE, X_GRADIENT };
 
 struct XArgs {
     GridDims gd;
@@ -83,8 +83,46 @@ struct XArgs {
     const double2* in[9];  // (k, zp, y) complex, Nxh per line
     double2* out[6];
     const double* rin[3];  // (x, zp, y) real
-    double* rout[6];
+    double* rout[9];       // (x, zp, y) real outputs (X_INVERSE; X_GRADIENT: dV_i/dx_j at 3 i + j)
 };
+
+// Box filter of a fine spectral vector field onto a coarse grid (same box)
+struct FilterArgs {
+    GridDims f, c;   // fine and coarse grids
+    double dx, dz;   // top-hat widths
+    const double2* in[3];
+    double2* out[3];
+};
+hipError_t launch_filter(const FilterArgs& a, hipStream_t st);
+
+// Subgrid-force MLP (10 -> 64 -> 64 -> 64 -> 3): parameter count of theta
+constexpr int MLP_NPARAM = 10448;
+// norm: in_mean[10], in_scale[10], out_scale[3]
+struct MLPInferArgs {
+    const float* theta;          // FP32 parameters (biases read here)
+    const unsigned short* wbf;   // bfloat16 copy of theta (weights read here)
+    const float* norm;
+    const double* grad[9];       // dV_i/dx_j on the padded points
+    long long npts, plane;       // points, points per y plane (Nxp Nzp)
+    int Ny;
+    double* out[3];              // predicted force on the padded points
+};
+struct MLPTrainArgs {
+    const float* theta;
+    const unsigned short* wbf;
+    const float* norm;
+    const float* X;              // raw features, 10 per sample
+    const float* T;              // raw targets, 3 per sample
+    const int* idx;              // batch -> sample
+    long long batch;
+    float* part;                 // per-wave partials: waves x (MLP_NPARAM + 1)
+};
+hipError_t launch_mlp_infer(const MLPInferArgs& a, hipStream_t st);
+hipError_t launch_mlp_features(const MLPInferArgs& a, float* X, float* T, hipStream_t st);  // targets from a.out
+int mlp_train_waves(long long batch);
+hipError_t launch_mlp_grad(const MLPTrainArgs& a, float* grad, hipStream_t st);  // grad: MLP_NPARAM + 1 (loss last)
+hipError_t launch_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v, const float* grad, float lr, int step,
+                           hipStream_t st);
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st);
 unsigned y_blocks(const GridDims& gd);  // blocks of a y launch (partials of Y_DOT)

@@ -226,9 +226,14 @@ struct XCtx {
 //   divergence in V (3)                    out V_i V_j, i <= j (6)    (nonlinear.jl:157-160)
 // Only wave-level ordering is needed after the block loads the twiddles.
 // A block has a.waves wavefronts (64 * a.waves threads).
-__host__ __device__ constexpr int x_fft_lines_in(int mode) { return mode == X_CONVECTIVE ? 12 : (mode == X_DIVERGENCE ? 3 : 6); }
-__host__ __device__ constexpr int x_fields_in(int mode) { return mode == X_CONVECTIVE ? 9 : (mode == X_DIVERGENCE ? 3 : 6); }
-__host__ __device__ constexpr int x_fields_out(int mode) { return mode == X_DIVERGENCE ? 6 : 3; }
+//   gradient   in V, dV/dy, dV/dz (9);     out dV_i/dx_j on the padded points (9 real fields, no forward FFT)
+__host__ __device__ constexpr int x_fft_lines_in(int mode) {
+    return (mode == X_CONVECTIVE || mode == X_GRADIENT) ? 12 : (mode == X_DIVERGENCE ? 3 : 6);
+}
+__host__ __device__ constexpr int x_fields_in(int mode) {
+    return (mode == X_CONVECTIVE || mode == X_GRADIENT) ? 9 : (mode == X_DIVERGENCE ? 3 : 6);
+}
+__host__ __device__ constexpr int x_fields_out(int mode) { return mode == X_DIVERGENCE ? 6 : (mode == X_GRADIENT ? 0 : 3); }
 __host__ __device__ constexpr int x_buffer_lines(int mode) {
     return x_fft_lines_in(mode) > x_fields_out(mode) ? x_fft_lines_in(mode) : x_fields_out(mode);
 }
@@ -271,7 +276,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
             int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
             int f = q;
             bool dx = false;
-            if constexpr (MODE == X_CONVECTIVE) {
+            if constexpr (MODE == X_CONVECTIVE || MODE == X_GRADIENT) {
                 dx = (q >= 3 && q < 6);
                 f = q < 3 ? q : q - 3;
             }
@@ -287,6 +292,18 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
         }
         wave_sync();
         lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, NFFT, ld, +1, lane, 64);
+        if constexpr (MODE == X_GRADIENT) {
+            // line 3 + 3 d + i holds dV_i/dx_d (d = x, y, z): values at x = 2m, 2m+1
+            for (int idx = lane; idx < 9 * M; idx += 64) {
+                const int c = (int)fast_div((unsigned)idx, a.mul_m), m = idx - c * M;
+                const int i = c / 3, d = c - 3 * i;
+                const C v = bufA[(3 + 3 * d + i) * ld + m];
+                double* o = a.rout[c] + (size_t)gl * Nxp + 2 * m;
+                o[0] = v.x;
+                o[1] = v.y;
+            }
+            return;
+        }
         // products at x = 2j (real parts) and 2j+1 (imaginary parts), in place:
         // each lane reads all lines at its j before writing them
         for (int j = lane; j < M; j += 64) {
@@ -345,9 +362,9 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
         int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
         int f = q;
         bool dx = false;
-        if constexpr (MODE == X_CONVECTIVE) {
+        if constexpr (MODE == X_CONVECTIVE || MODE == X_GRADIENT) {
             dx = (q >= 3 && q < 6);
-            f = q < 3 ? q : (q < 6 ? q - 3 : q - 3);
+            f = q < 3 ? q : q - 3;
         }
         C Xk = bufB[f * ld + k];
         C Xm = bufB[f * ld + (M - k)];
@@ -361,6 +378,17 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
     }
     wave_sync();
     C* res = lds_fft<true>(bufA, bufB, a.plan, (const C*)tw, NFFT, ld, +1, lane, 64);
+    if constexpr (MODE == X_GRADIENT) {
+        for (int idx = lane; idx < 9 * M; idx += 64) {
+            const int c = (int)fast_div((unsigned)idx, a.mul_m), m = idx - c * M;
+            const int i = c / 3, d = c - 3 * i;
+            const C v = res[(3 + 3 * d + i) * ld + m];
+            double* o = a.rout[c] + (size_t)gl * Nxp + 2 * m;
+            o[0] = v.x;
+            o[1] = v.y;
+        }
+        return;
+    }
     C* nxt = (res == bufA) ? bufB : bufA;
     // pointwise products at x = 2j (real parts) and x = 2j+1 (imaginary parts)
     for (int j = lane; j < M; j += 64) {
@@ -549,11 +577,12 @@ hipError_t launch_zpass_batch(int dir, int ppt, const ZArgs* tab, const unsigned
 }
 
 static bool x_wave_path(const XArgs& a) {
-    return a.gd.Nxp % 2 == 0 && (a.mode == X_ROTATIONAL || a.mode == X_CONVECTIVE || a.mode == X_DIVERGENCE);
+    return a.gd.Nxp % 2 == 0 &&
+           (a.mode == X_ROTATIONAL || a.mode == X_CONVECTIVE || a.mode == X_DIVERGENCE || a.mode == X_GRADIENT);
 }
 
 int x_phys_lines(int mode) {
-    return mode == X_ROTATIONAL ? 6 : mode == X_CONVECTIVE ? 12 : mode == X_DIVERGENCE ? 9 : 1;
+    return mode == X_ROTATIONAL ? 6 : (mode == X_CONVECTIVE || mode == X_GRADIENT) ? 12 : mode == X_DIVERGENCE ? 9 : 1;
 }
 
 static size_t x_wave_lds(const XArgs& a, int waves) {
@@ -649,6 +678,13 @@ hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
     if (a.mode == X_ROTATIONAL) CF_XMODE(X_ROTATIONAL)
     if (a.mode == X_CONVECTIVE) CF_XMODE(X_CONVECTIVE)
     if (a.mode == X_DIVERGENCE) CF_XMODE(X_DIVERGENCE)
+    if (a.mode == X_GRADIENT && !a.mixed) {
+        switch (a.ppt) {
+            case 9: CF_XWAVE(X_GRADIENT, 9, double2)
+            case 18: CF_XWAVE(X_GRADIENT, 18, double2)
+            default: CF_XWAVE(X_GRADIENT, 0, double2)
+        }
+    }
     return hipErrorInvalidValue;
 #undef CF_XMODE
 #undef CF_XWAVE

@@ -105,6 +105,53 @@ int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, 
 int cf_step_timed(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int constraint,
                   const double* values, double* t_out, double ms[5], void* stream);
 
+/* Velocity-gradient tensor of the total velocity V = U + Ub e_x at the padded
+ * physical points (the derivatives of operators.jl:14-83 and _batchdiff!,
+ * batchedinfluence.jl:32-43, each followed by InverseFFT!, ffts.jl:151-166):
+ * out (DEVICE) holds 9 padded physical fields, field 3 i + j = dV_i/dx_j with
+ * x_j = (x, y, z).  Needs an even padded Nx.  Uses the problem's workspaces. */
+int cf_velocity_gradient(cf_problem* h, const double* U0, const double* U1, const double* U2, double* out, void* stream);
+
+/* Box filter for the learned closure's data (no counterpart in the
+ * reference; the filtered-DNS reference of the closure workload): spectral
+ * U of `fine` -> spectral Ubar of `coarse` (same Lx, Lz; coarse strictly
+ * coarser in x and z, not finer in y).  Each retained coarse mode takes the
+ * fine coefficient of the same wavenumbers times the top-hat transfer
+ * functions sin(alpha k dx/2)/(alpha k dx/2) * sin(beta l dz/2)/(beta l dz/2);
+ * coarse Nyquist planes are zero; Chebyshev degrees above the coarse Ny-1
+ * are dropped. */
+int cf_box_filter(cf_problem* fine, cf_problem* coarse, const double* U0, const double* U1, const double* U2, double* V0,
+                  double* V1, double* V2, double dx, double dz, void* stream);
+
+/* Learned subgrid closure (no counterpart in the reference; the closure
+ * workload's network).  MLP 10 -> 64 -> 64 -> 64 -> 3, GELU (erf form) after
+ * the hidden layers, bfloat16 weights and activations on the matrix cores
+ * with FP32 accumulation.  theta: CF_MLP_NPARAM floats, layout
+ * [W1 64x16 | b1 64 | W2 64x64 | b2 64 | W3 64x64 | b3 64 | W4 16x64 | b4 16]
+ * (row-major, output index first; input columns 10..15 of W1 and output
+ * rows 3..15 of W4 zero); wbf: its bfloat16 copy (bits, round to nearest
+ * even); norm: in_mean[10], in_scale[10], out_scale[3].  Inputs of a point:
+ * dV_i/dx_j (9, as cf_velocity_gradient) and the wall distance 1 - |y|,
+ * normalised (x - mean) * scale; output y * out_scale.  All pointers are
+ * DEVICE pointers; the grid is the padded (Nxp, Nzp, Ny) point grid. */
+#define CF_MLP_NPARAM 10448
+/* raw features X (10 per point) and, if target3 is given, raw targets T
+ * (3 per point) from 9 gradient fields and 3 target force fields */
+int cf_mlp_features(const double* grad9, const double* target3, int Nxp, int Nzp, int Ny, float* X, float* T, void* stream);
+/* predicted force (3 padded physical fields) from 9 gradient fields */
+int cf_mlp_infer(const float* theta, const unsigned short* wbf, const float* norm, const double* grad9, int Nxp, int Nzp,
+                 int Ny, double* force3, void* stream);
+/* floats of the work buffer cf_mlp_grad needs for a batch */
+long long cf_mlp_work_floats(long long batch);
+/* gradient of L = (1/batch) sum_b sum_{i<3} (y_bi - T_{idx_b,i}/out_scale_i)^2
+ * over the samples idx[0..batch) of (X, T): grad[0..CF_MLP_NPARAM) and the
+ * loss at grad[CF_MLP_NPARAM]; deterministic (fixed-order reduction) */
+int cf_mlp_grad(const float* theta, const unsigned short* wbf, const float* norm, const float* X, const float* T,
+                const int* idx, long long batch, float* work, float* grad, void* stream);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8; step counts from 1) on
+ * theta, refreshing wbf */
+int cf_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v, const float* grad, float lr, int step, void* stream);
+
 /* forcing(tstage, U, F) (cnrk2.jl:173-175, forcing.jl:3-11).  During each
  * stage of cf_step / cf_advance / cf_advance_to / cf_step_timed, after the
  * nonlinear term is enqueued, the library calls fn(user, tstage, U0, U1, U2,

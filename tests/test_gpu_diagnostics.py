@@ -92,3 +92,36 @@ def test_plane_statistics_match_physical_grid(gpu, dims):
     gm, gcv = st.profiles()
     assert np.abs(gm - pm).max() < 1e-13 * np.abs(pm).max()
     assert np.abs(gcv - pc).max() < 1e-12 * np.abs(pc).max()
+
+
+@pytest.mark.parametrize("dims", [(8, 17, 8), (16, 33, 12), (12, 65, 10), (4, 255, 3), (10, 97, 12)])
+def test_velocity_gradient_matches_oracle(gpu, dims):
+    """cf_velocity_gradient: dV_i/dx_j of V = U + Ub e_x at the padded points
+    against the oracle's ddx1/ddx2/ddx3 followed by its inverse transform;
+    for U = 0 the Poiseuille shear dUb/dy = -2y exactly."""
+    import torch
+    cf = gpu
+    say("velocity gradient", dims)
+    gc, go, prob = make(cf, dims, profile=lambda y: 1 - y ** 2)
+    if (3 * dims[0] + 1) // 2 % 2:
+        with pytest.raises(NotImplementedError):
+            cf.velocity_gradient(prob, gc.zeros_spec(3))
+        return
+    rng = np.random.default_rng(41 + sum(dims))
+    U = hermitian_random(go, rng, 3, 0.3)
+    V = U.copy()
+    V[0, :, 0, 0] += O.chebcoeffs(1 - go.y ** 2)
+    got = cf.velocity_gradient(prob, torch.as_tensor(U, device="cuda")).cpu().numpy()
+    worst = 0.0
+    for i in range(3):
+        for j, op in enumerate(("ddx1", "ddx2", "ddx3")):
+            ref = O.inverse(go, O.operator(go, op, V[i]))
+            worst = max(worst, np.abs(got[i, j] - ref).max() / max(np.abs(ref).max(), 1e-300))
+    say(f"  worst rel err {worst:.2e}")
+    assert worst < 1e-12
+    lam = cf.velocity_gradient(prob, gc.zeros_spec(3)).cpu().numpy()
+    y = go.y[:, None, None]
+    # derivative of the transformed profile: coefficient roundoff amplified by ~n^2
+    tol = 2e-16 * go.Ny ** 2
+    assert np.abs(lam[0, 1] - (-2 * y)).max() < tol
+    assert np.abs(np.delete(lam.reshape(9, -1), 1, axis=0)).max() < tol
