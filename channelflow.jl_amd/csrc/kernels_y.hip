@@ -539,9 +539,20 @@ __device__ __forceinline__ void yprologue_body(const YArgs& a, unsigned block) {
     inverse_six<L, SC>(tile, a, g, mi, U);
 }
 
+// The single-problem prologue keeps its own copy of the body: passing the
+// byval kernel argument by reference to yprologue_body makes the compiler
+// materialise part of it in scratch memory.
 template <int L, typename SC>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
-    yprologue_body<L, SC>(a, blockIdx.x);
+    extern __shared__ double2 smem[];
+    load_tables<L>(smem, a);
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
+    Group<L> g(tile.tab);
+    ModeInfo mi = mode_info(a.gd, tile.s);
+    CVec U[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
+    inverse_six<L, SC>(tile, a, g, mi, U);
 }
 
 // Batched ensemble launches: instance i owns blocks [start[i], start[i+1])
