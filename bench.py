@@ -374,6 +374,7 @@ def main():
     cf.require_device()
     w = WORKLOADS[args.workload]
     g, prof, kw, prob = make_problem(cf, w)
+    tiles = cf.tune(prob)  # tile shapes by timing (setup; results do not depend on them)
     state = cf.perturbation_state(prob, w["amp"], seed=42)
     U, P = state.velocity, state.stagepressure
     U0, P0 = U.cpu().numpy(), P.cpu().numpy()
@@ -464,6 +465,7 @@ def main():
         "data": "synthetic (seeded divergence-free perturbation, kappa^-4 spectrum, amplitude 0.05)",
         "config": {"workload": w["desc"], "grid": list(w["N"]), "form": "rotational",
                    "constraint": "bulkvelocity (1.0, 0.0)" if w["bulk"] else "pressuregradient (0, 0)",
+                   "tiles": {"z_tile_k": tiles[0], "z_points_per_thread": tiles[1], "x_waves_per_block": tiles[2]},
                    "step_algorithmic_bytes": step_bytes,
                    "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
                    "kernel_breakdown": breakdown},

@@ -26,7 +26,7 @@ __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
            "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
            "power_input", "plane_statistics", "ProfileStatistics", "flow", "device_gaussian_noise",
-           "savefield", "loadfield", "Ensemble", "velocity_gradient", "box_filter"]
+           "savefield", "loadfield", "Ensemble", "velocity_gradient", "box_filter", "tune"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -353,6 +353,16 @@ class Flow:
 
 def flow(problem):
     return Flow(problem)
+
+
+def tune(problem, stream=None):
+    """Pick the problem's z tile width and x wavefronts per block by timing
+    them on the device (cf_problem_tune); returns (KT, z points per thread,
+    x wavefronts).  Results of the step are identical for any tiling."""
+    _lib.check(_lib.load().cf_problem_tune(problem.scheme.h, _stream(stream)))
+    out = (C.c_int * 3)()
+    _lib.check(_lib.load().cf_problem_tiles(problem.scheme.h, out))
+    return tuple(out)
 
 
 def last_gradient(scheme):

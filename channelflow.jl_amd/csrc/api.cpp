@@ -1190,6 +1190,99 @@ extern "C" int cf_random_state(cf_problem* h, double epsilon, unsigned long long
     return cf_pressure(h, U0, U1, U2, P, stream);
 }
 
+// Tile autotuner of one problem (FP64 step): times the z passes of the
+// nonlinear term for each in-place tile width and the x pass for each
+// wavefront count on the problem's workspaces, and keeps the fastest.  The
+// results of every pass do not depend on the tiling.
+extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    DeviceGuard dg(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t a, b;
+    HIPTRY(hipEventCreate(&a));
+    HIPTRY(hipEventCreate(&b));
+    auto timed = [&](auto&& fn, float* best) -> int {
+        *best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIPTRY(hipEventRecord(a, st));
+            for (int k = 0; k < 3; ++k)
+                if (int rc = fn()) return rc;
+            HIPTRY(hipEventRecord(b, st));
+            HIPTRY(hipEventSynchronize(b));
+            float t = 0;
+            HIPTRY(hipEventElapsedTime(&t, a, b));
+            *best = std::min(*best, t);
+        }
+        return 0;
+    };
+    int rc = 0;
+    // z passes: both directions of the form's nonlinear term
+    const int keepKT = h->KT, keepPPT = h->zppt;
+    int bestKT = keepKT, bestPPT = keepPPT;
+    float best = 1e30f;
+    for (int kt = 1; kt <= std::min(32, h->gd.Nxh) && !rc; ++kt) {
+        const int ppt = z_inplace_ppt(h->pz, kt);
+        if (!ppt && !(kt == keepKT && keepPPT == 0)) continue;
+        if (z_lds_bytes([&] {
+                ZArgs z;
+                std::memset(&z, 0, sizeof(z));
+                z.gd = h->gd;
+                z.KT = kt;
+                z.ld = h->zld;
+                z.ppt = ppt;
+                return z;
+            }()) > 64 * 1024)
+            continue;
+        h->KT = kt;
+        h->zppt = ppt;
+        const NonlinearArgs na = nonlinear_args(h, false);
+        float ms;
+        rc = timed([&] {
+            HIPTRY(launch_zpass(+1, na.zin, st));
+            HIPTRY(launch_zpass(-1, na.zout, st));
+            return 0;
+        }, &ms);
+        if (!rc && ms < best) {
+            best = ms;
+            bestKT = kt;
+            bestPPT = ppt;
+        }
+    }
+    h->KT = bestKT;
+    h->zppt = bestPPT;
+    // x pass: wavefronts per block (even padded Nx)
+    if (!rc && h->gd.Nxp % 2 == 0) {
+        const int maxw = h->xwaves;
+        int bestW = maxw;
+        best = 1e30f;
+        for (int w = 1; w <= maxw && !rc; ++w) {
+            h->xwaves = w;
+            const NonlinearArgs na = nonlinear_args(h, false);
+            float ms;
+            rc = timed([&] {
+                HIPTRY(launch_xpass(na.x, st));
+                return 0;
+            }, &ms);
+            if (!rc && ms < best) {
+                best = ms;
+                bestW = w;
+            }
+        }
+        h->xwaves = bestW;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+extern "C" int cf_problem_tiles(const cf_problem* h, int out[3]) {
+    if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
+    out[0] = h->KT;
+    out[1] = h->zppt;
+    out[2] = h->xwaves;
+    return CF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Ensembles: many independent problems advanced together.  Each kernel class
 // of the step is one launch per group of instances that share its template

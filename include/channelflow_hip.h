@@ -168,6 +168,16 @@ typedef int (*cf_forcing_fn)(void* user, double tstage, const double* U0, const 
                              double* F0, double* F1, double* F2, void* stream);
 int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, double* F0, double* F1, double* F2);
 
+/* Tile autotuner (no counterpart in the reference; a performance setting):
+ * times the nonlinear term's z passes for each tile width and its x pass for
+ * each wavefront count on the problem's workspaces, on `stream`, and keeps
+ * the fastest for the FP64 step.  Results of every call are identical for
+ * any tiling.  Synchronises the stream. */
+int cf_problem_tune(cf_problem* h, void* stream);
+/* current tiles: out[0] z tile width, out[1] z in-place points per thread
+ * (0: two buffers), out[2] x wavefronts per block */
+int cf_problem_tiles(const cf_problem* h, int out[3]);
+
 /* Uniform pressure gradient (dPdx, dPdz) returned by the last Stokes solve of
  * the last step (stokes.jl:101-107).  Synchronises the stream; HOST output. */
 int cf_last_gradient(cf_problem* h, double out[2], void* stream);

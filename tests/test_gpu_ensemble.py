@@ -69,3 +69,19 @@ def test_ensemble_rejects_bad_inputs(gpu):
         cf.Ensemble([cf.CouetteFlow(g, 1 / 400, 0.01, precision="mixed")], [s1])
     with pytest.raises(NotImplementedError):
         cf.Ensemble([cf.CouetteFlow(g, 1 / 400, 0.01, form="divergence")], [s1])
+
+
+def test_problem_tune_keeps_results_bit_identical(gpu):
+    """cf_problem_tune changes only tile shapes: the step is bit-identical."""
+    import torch
+    cf = gpu
+    dims = (32, 33, 32)
+    a_p = _problem(cf, dims, 1 / 400, "couette", "rotating")
+    b_p = _problem(cf, dims, 1 / 400, "couette", "rotating")
+    tiles = cf.tune(b_p)
+    say(f"  tuned tiles (KT, z ppt, x waves) {tiles}")
+    s = cf.random_state(a_p, 0.1, 9)
+    a, b = s.copy(), s.copy()
+    cf.step_(a_p.scheme, a_p.nlterm, a.velocity, a.stagepressure, 0.0, nsteps=4)
+    cf.step_(b_p.scheme, b_p.nlterm, b.velocity, b.stagepressure, 0.0, nsteps=4)
+    assert torch.equal(a.velocity, b.velocity) and torch.equal(a.stagepressure, b.stagepressure)
