@@ -166,8 +166,12 @@ class SubgridMLP:
                                             g.Nxp, g.Nzp, g.Ny, cf._ptr(out), cf._stream()))
         return out
 
-    def forcing(self, problem):
-        """forcing(t, U, F) adding the predicted subgrid force (spectral) to F."""
+    def forcing(self, problem, modes="all"):
+        """forcing(t, U, F) adding the predicted subgrid force (spectral) to F.
+        modes="mean" keeps only its plane-mean part (the (0,0) Fourier mode,
+        a function of y), the part that sets the mean profile."""
+        if modes not in ("all", "mean"):
+            raise ValueError("modes must be 'all' or 'mean'")
         cf = _cf()
         torch = cf._torch()
         g = problem.grid
@@ -180,7 +184,10 @@ class SubgridMLP:
             self.predict(problem, U, grad=gbuf, out=fbuf)
             for c in range(3):
                 _lib.check(_lib.load().cf_forward(h, cf._ptr(fbuf[c]), cf._ptr(spec[c]), cf._stream()))
-            F.add_(spec)
+            if modes == "mean":
+                F[:, :, 0, 0] += spec[:, :, 0, 0]
+            else:
+                F.add_(spec)
 
         return force
 

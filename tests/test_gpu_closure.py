@@ -171,3 +171,14 @@ def test_closure_training_gains_over_no_model(gpu):
     d = (a - b).abs().max().item()
     say(f"  closure changes the coarse state by {d:.3e}")
     assert np.isfinite(d) and d > 0
+    # the plane-mean variant forces only the (0,0) mode: after one step the
+    # other modes differ only through the nonlinear coupling of later stages
+    c, pc = s.clone(), sp.clone()
+    e, pe = s.clone(), sp.clone()
+    cf.step_(coarse.scheme, coarse.nlterm, c, pc, 0.0, **coarse.constraint)
+    cf.step_(coarse.scheme, coarse.nlterm, e, pe, 0.0, forcing=net.forcing(coarse, modes="mean"), **coarse.constraint)
+    dm = (c - e)[:, :, 0, 0].abs().max().item()
+    do = (c - e).abs().clone()
+    do[:, :, 0, 0] = 0
+    say(f"  mean-mode forcing: (0,0) change {dm:.3e}, other modes {do.max().item():.3e}")
+    assert dm > 0 and do.max().item() < 1e-3 * dm
