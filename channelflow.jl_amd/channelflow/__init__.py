@@ -263,7 +263,10 @@ class _Forcing:
     handle for one call: the library calls back at every stage; the callback
     zeroes F and lets `forcing` add its spectral acceleration into F with
     device operations enqueued on the step's stream (the callback runs while
-    the step is being enqueued, so it must not read U on the host)."""
+    the step is being enqueued, so it must not read U on the host).  The
+    stage's nonlinear term is live in the problem's workspaces at that
+    point: a forcing that needs transforms or gradients runs them on another
+    problem of the same grid."""
 
     def __init__(self, scheme, U, forcing):
         self.scheme, self.U, self.forcing = scheme, U, forcing

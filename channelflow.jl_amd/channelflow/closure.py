@@ -175,13 +175,17 @@ class SubgridMLP:
         cf = _cf()
         torch = cf._torch()
         g = problem.grid
+        # the step's nonlinear term is live in the problem's own workspaces
+        # while the callback runs: gradients and transforms use a helper
+        # problem (same grid, base flow and form) with workspaces of its own
+        helper = cf.ChannelFlowProblem(g, problem.baseflow, problem.scheme.nu, problem.scheme.dt, form=problem.form)
         gbuf = torch.empty((3, 3) + g.phys_shape, dtype=torch.float64, device="cuda")
         fbuf = torch.empty((3,) + g.phys_shape, dtype=torch.float64, device="cuda")
         spec = g.zeros_spec(3)
-        h = problem.scheme.h
+        h = helper.scheme.h
 
         def force(t, U, F):
-            self.predict(problem, U, grad=gbuf, out=fbuf)
+            self.predict(helper, U, grad=gbuf, out=fbuf)
             for c in range(3):
                 _lib.check(_lib.load().cf_forward(h, cf._ptr(fbuf[c]), cf._ptr(spec[c]), cf._stream()))
             if modes == "mean":
@@ -189,6 +193,7 @@ class SubgridMLP:
             else:
                 F.add_(spec)
 
+        force.helper = helper  # keeps the helper's handle alive with the callable
         return force
 
     # -- persistence ----------------------------------------------------------

@@ -799,7 +799,7 @@ extern "C" int cf_mlp_infer(const float* theta, const unsigned short* wbf, const
 }
 
 extern "C" long long cf_mlp_work_floats(long long batch) {
-    return batch > 0 ? (long long)mlp_train_waves(batch) * (MLP_NPARAM + 1) : 0;
+    return batch > 0 ? mlp_work_floats(batch) : 0;
 }
 
 extern "C" int cf_mlp_grad(const float* theta, const unsigned short* wbf, const float* norm, const float* X, const float* T,

@@ -120,6 +120,7 @@ struct MLPTrainArgs {
 hipError_t launch_mlp_infer(const MLPInferArgs& a, hipStream_t st);
 hipError_t launch_mlp_features(const MLPInferArgs& a, float* X, float* T, hipStream_t st);  // targets from a.out
 int mlp_train_waves(long long batch);
+long long mlp_work_floats(long long batch);  // partial buffer of launch_mlp_grad
 hipError_t launch_mlp_grad(const MLPTrainArgs& a, float* grad, hipStream_t st);  // grad: MLP_NPARAM + 1 (loss last)
 hipError_t launch_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v, const float* grad, float lr, int step,
                            hipStream_t st);

@@ -334,12 +334,24 @@ __global__ void __launch_bounds__(128) mlp_train_kernel(MLPTrainArgs a) {
     }
 }
 
-// grad[j] = sum over waves of part[w][j] (fixed order); grad[NPAR] = loss
-__global__ void __launch_bounds__(256) mlp_reduce_kernel(const float* __restrict__ part, int nw, float* __restrict__ grad) {
+// Two-level fixed-order reduction of the per-wave partials:
+// chunk[c][j] = sum_{w in chunk c} part[w][j], then grad[j] = sum_c chunk[c][j]
+// (grad[NPAR] = loss).  Chunks of MLP_RCHUNK waves run in parallel.
+constexpr int MLP_RCHUNK = 32;
+__global__ void __launch_bounds__(256) mlp_reduce_chunks_kernel(const float* __restrict__ part, int nw,
+                                                                float* __restrict__ chunk) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > NPAR) return;
+    const int c = blockIdx.y, w0 = c * MLP_RCHUNK, w1 = min(nw, w0 + MLP_RCHUNK);
+    float s = 0.f;
+    for (int w = w0; w < w1; ++w) s += part[(long long)w * (NPAR + 1) + j];
+    chunk[(long long)c * (NPAR + 1) + j] = s;
+}
+__global__ void __launch_bounds__(256) mlp_reduce_kernel(const float* __restrict__ chunk, int nc, float* __restrict__ grad) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > NPAR) return;
     float s = 0.f;
-    for (int w = 0; w < nw; ++w) s += part[(long long)w * (NPAR + 1) + j];
+    for (int c = 0; c < nc; ++c) s += chunk[(long long)c * (NPAR + 1) + j];
     grad[j] = s;
 }
 
@@ -378,13 +390,21 @@ hipError_t launch_mlp_features(const MLPInferArgs& a, float* X, float* T, hipStr
 }
 
 int mlp_train_waves(long long batch) { return (int)((batch + 31) / 32); }
+// work floats: wave partials, then the chunk sums
+long long mlp_work_floats(long long batch) {
+    const long long nw = mlp_train_waves(batch), nc = (nw + MLP_RCHUNK - 1) / MLP_RCHUNK;
+    return (nw + nc) * (NPAR + 1);
+}
 
 hipError_t launch_mlp_grad(const MLPTrainArgs& a, float* grad, hipStream_t st) {
     const size_t lds = mlp_train_lds();
     (void)hipFuncSetAttribute((const void*)mlp_train_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int nw = mlp_train_waves(a.batch);
+    const int nw = mlp_train_waves(a.batch), nc = (nw + MLP_RCHUNK - 1) / MLP_RCHUNK;
+    float* chunk = a.part + (long long)nw * (NPAR + 1);
     hipLaunchKernelGGL(mlp_train_kernel, dim3((unsigned)((nw + 1) / 2)), dim3(128), lds, st, a);
-    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((NPAR + 1 + 255) / 256), dim3(256), 0, st, a.part, nw, grad);
+    hipLaunchKernelGGL(mlp_reduce_chunks_kernel, dim3((NPAR + 1 + 255) / 256, (unsigned)nc), dim3(256), 0, st, a.part, nw,
+                       chunk);
+    hipLaunchKernelGGL(mlp_reduce_kernel, dim3((NPAR + 1 + 255) / 256), dim3(256), 0, st, chunk, nc, grad);
     return hipGetLastError();
 }
 

@@ -158,7 +158,10 @@ int cf_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v, const flo
  * F0, F1, F2, stream) on the host with tstage = t + (0, 1/3, 3/4) dt.  fn
  * must ENQUEUE on `stream` device work that overwrites F0..F2 (spectral
  * fields owned by the caller, given here) with the forcing acceleration; it
- * must not read U on the host.  The library then adds F into the nonlinear
+ * must not read U on the host, and must not call entry points on the same
+ * handle that use its workspaces (transforms, nonlinear term, gradient,
+ * projection, diagnostics): the stage's nonlinear term is live in them.  A
+ * second handle on the same grid serves such work.  The library then adds F into the nonlinear
  * term N before the bulk-gradient evaluation and the Q update, as the
  * reference's forcing adds into N.  fn returns 0 on success; a nonzero
  * return stops the step with CF_ERR_ARGUMENT (earlier stages may already

@@ -106,3 +106,23 @@ def test_forcing_callback_error_propagates(gpu):
     with pytest.raises(RuntimeError, match="on purpose"):
         cf.step_(prob.scheme, prob.nlterm, s.velocity, s.stagepressure, 0.0, forcing=bad)
     torch.cuda.synchronize()
+
+
+def test_forcing_using_a_helper_problem_leaves_the_step_intact(gpu):
+    """A forcing that computes gradients (which use a problem's workspaces)
+    on a helper problem of the same grid and adds nothing leaves the step
+    bit-identical to the unforced one."""
+    import torch
+    cf = gpu
+    gc, go, prob = make(cf, (16, 33, 12))
+    helper = cf.ChannelFlowProblem(gc, prob.baseflow, 1 / 400, 0.01)
+    s = cf.random_state(prob, 0.1, 4)
+    a, b = s.copy(), s.copy()
+    cf.step_(prob.scheme, prob.nlterm, a.velocity, a.stagepressure, 0.0, nsteps=2)
+    gbuf = torch.empty((3, 3) + gc.phys_shape, dtype=torch.float64, device="cuda")
+
+    def quiet(t, U, F):
+        cf.velocity_gradient(helper, U, out=gbuf)
+
+    cf.step_(prob.scheme, prob.nlterm, b.velocity, b.stagepressure, 0.0, nsteps=2, forcing=quiet)
+    assert torch.equal(a.velocity, b.velocity) and torch.equal(a.stagepressure, b.stagepressure)
