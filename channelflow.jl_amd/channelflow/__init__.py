@@ -384,10 +384,15 @@ def FFT(problem, u):
     single = u.dim() == 3
     uu = u.unsqueeze(0) if single else u
     out = g.zeros_spec(uu.shape[0])
+    if uu.dtype != torch.float64 or tuple(uu.shape[1:]) != g.phys_shape:
+        raise DimensionMismatch("forward transform requires padded physical input")
+    if uu.shape[0] == 3:  # one launch per pass for the vector field
+        src = uu.contiguous()
+        _lib.check(_lib.load().cf_forward3(problem.scheme.h, _ptr(src), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                           _stream()))
+        return out
     for c in range(uu.shape[0]):
         src = uu[c].contiguous()
-        if src.dtype != torch.float64 or tuple(src.shape) != g.phys_shape:
-            raise DimensionMismatch("forward transform requires padded physical input")
         _lib.check(_lib.load().cf_forward(problem.scheme.h, _ptr(src), _ptr(out[c]), _stream()))
     return out[0] if single else out
 
@@ -398,6 +403,13 @@ def IFFT(problem, U):
     single = U.dim() == 3
     UU = U.unsqueeze(0) if single else U
     out = g.zeros_phys(UU.shape[0])
+    if UU.shape[0] == 3:  # one launch per pass for the vector field
+        srcs = [UU[c].contiguous() for c in range(3)]
+        for src in srcs:
+            _check_spec(g, src)
+        _lib.check(_lib.load().cf_inverse3(problem.scheme.h, _ptr(srcs[0]), _ptr(srcs[1]), _ptr(srcs[2]), _ptr(out),
+                                           _stream()))
+        return out
     for c in range(UU.shape[0]):
         src = UU[c].contiguous()
         _check_spec(g, src)

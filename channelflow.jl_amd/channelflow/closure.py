@@ -186,8 +186,8 @@ class SubgridMLP:
 
         def force(t, U, F):
             self.predict(helper, U, grad=gbuf, out=fbuf)
-            for c in range(3):
-                _lib.check(_lib.load().cf_forward(h, cf._ptr(fbuf[c]), cf._ptr(spec[c]), cf._stream()))
+            _lib.check(_lib.load().cf_forward3(h, cf._ptr(fbuf), cf._ptr(spec[0]), cf._ptr(spec[1]), cf._ptr(spec[2]),
+                                               cf._stream()))
             if modes == "mean":
                 F[:, :, 0, 0] += spec[:, :, 0, 0]
             else:

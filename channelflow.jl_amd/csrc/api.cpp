@@ -878,48 +878,85 @@ extern "C" int cf_last_gradient(cf_problem* h, double out[2], void* stream) {
     return CF_OK;
 }
 
+// nf (1..3) fields per launch: one x, one z and one y launch for all of them
+static int forward_fields(cf_problem* h, int nf, const double* u, double* const* U, hipStream_t st) {
+    const GridDims& gd = h->gd;
+    const size_t nphys = (size_t)gd.Nxp * gd.Nzp * gd.Ny;
+    XArgs x = base_xargs(h);
+    x.mode = X_FORWARD;
+    x.nfields = nf;
+    double2* zi[3];
+    double2* zo[3];
+    YArgs a = base_yargs(h);
+    a.nfields = nf;
+    for (int c = 0; c < nf; ++c) {
+        x.rin[c] = u + nphys * c;
+        x.out[c] = fld(h->zbuf, gd.Sz, c);
+        zi[c] = fld(h->zbuf, gd.Sz, c);
+        zo[c] = fld(h->six, gd.S, c);
+        a.Nphys[c] = zo[c];
+        a.Rout[c] = (double2*)U[c];
+    }
+    HIPTRY(launch_xpass(x, st));
+    if (int rc = zpass(h, -1, nf, zi, zo, st)) return rc;
+    HIPTRY(launch_ykernel(Y_DCT_FWD, a, st));
+    return CF_OK;
+}
+
+static int inverse_fields(cf_problem* h, int nf, double* const* U, double* u, hipStream_t st) {
+    const GridDims& gd = h->gd;
+    const size_t nphys = (size_t)gd.Nxp * gd.Nzp * gd.Ny;
+    YArgs a = base_yargs(h);
+    a.nfields = nf;
+    double2* zi[3];
+    double2* zo[3];
+    XArgs x = base_xargs(h);
+    x.mode = X_INVERSE;
+    x.nfields = nf;
+    for (int c = 0; c < nf; ++c) {
+        a.Rin[c] = (const double2*)U[c];
+        a.out6[c] = fld(h->six, gd.S, c);
+        zi[c] = a.out6[c];
+        zo[c] = fld(h->zbuf, gd.Sz, c);
+        x.in[c] = zo[c];
+        x.rout[c] = u + nphys * c;
+    }
+    HIPTRY(launch_ykernel(Y_DCT_INV, a, st));
+    if (int rc = zpass(h, +1, nf, zi, zo, st)) return rc;
+    HIPTRY(launch_xpass(x, st));
+    return CF_OK;
+}
+
 extern "C" int cf_forward(cf_problem* h, const double* u, double* U, void* stream) {
     if (!h || !u || !U) return fail(CF_ERR_ARGUMENT, "null argument");
     DeviceGuard dg(h->device);
-    hipStream_t st = (hipStream_t)stream;
-    const GridDims& gd = h->gd;
-    XArgs x = base_xargs(h);
-    x.mode = X_FORWARD;
-    x.nfields = 1;
-    x.rin[0] = u;
-    x.out[0] = h->zbuf;
-    HIPTRY(launch_xpass(x, st));
-    double2* zi[1] = {h->zbuf};
-    double2* zo[1] = {h->six};
-    if (int rc = zpass(h, -1, 1, zi, zo, st)) return rc;
-    YArgs a = base_yargs(h);
-    a.nfields = 1;
-    a.Nphys[0] = h->six;
-    a.Rout[0] = (double2*)U;
-    HIPTRY(launch_ykernel(Y_DCT_FWD, a, st));
-    (void)gd;
-    return CF_OK;
+    double* Us[1] = {U};
+    return forward_fields(h, 1, u, Us, (hipStream_t)stream);
+}
+
+extern "C" int cf_forward3(cf_problem* h, const double* u, double* U0, double* U1, double* U2, void* stream) {
+    if (!h || !u) return fail(CF_ERR_ARGUMENT, "null argument");
+    const void* f[3] = {U0, U1, U2};
+    if (int rc = check_fields(f, 3)) return rc;
+    DeviceGuard dg(h->device);
+    double* Us[3] = {U0, U1, U2};
+    return forward_fields(h, 3, u, Us, (hipStream_t)stream);
 }
 
 extern "C" int cf_inverse(cf_problem* h, const double* U, double* u, void* stream) {
     if (!h || !u || !U) return fail(CF_ERR_ARGUMENT, "null argument");
     DeviceGuard dg(h->device);
-    hipStream_t st = (hipStream_t)stream;
-    YArgs a = base_yargs(h);
-    a.nfields = 1;
-    a.Rin[0] = (const double2*)U;
-    a.out6[0] = h->six;
-    HIPTRY(launch_ykernel(Y_DCT_INV, a, st));
-    double2* zi[1] = {h->six};
-    double2* zo[1] = {h->zbuf};
-    if (int rc = zpass(h, +1, 1, zi, zo, st)) return rc;
-    XArgs x = base_xargs(h);
-    x.mode = X_INVERSE;
-    x.nfields = 1;
-    x.in[0] = h->zbuf;
-    x.rout[0] = u;
-    HIPTRY(launch_xpass(x, st));
-    return CF_OK;
+    double* Us[1] = {const_cast<double*>(U)};
+    return inverse_fields(h, 1, Us, u, (hipStream_t)stream);
+}
+
+extern "C" int cf_inverse3(cf_problem* h, const double* U0, const double* U1, const double* U2, double* u, void* stream) {
+    if (!h || !u) return fail(CF_ERR_ARGUMENT, "null argument");
+    const void* f[3] = {U0, U1, U2};
+    if (int rc = check_fields(f, 3)) return rc;
+    DeviceGuard dg(h->device);
+    double* Us[3] = {const_cast<double*>(U0), const_cast<double*>(U1), const_cast<double*>(U2)};
+    return inverse_fields(h, 3, Us, u, (hipStream_t)stream);
 }
 
 // nonlinear term of U into spectral N (3 fields)
@@ -1183,9 +1220,7 @@ extern "C" int cf_random_state(cf_problem* h, double epsilon, unsigned long long
     if (!h->d_noise)
         if (int rc = dalloc(h, &h->d_noise, 3 * (size_t)n)) return rc;
     if (int rc = cf_gaussian_noise(h, epsilon, seed, h->d_noise, stream)) return rc;
-    double* U[3] = {U0, U1, U2};
-    for (int c = 0; c < 3; ++c)
-        if (int rc = cf_forward(h, h->d_noise + (size_t)c * n, U[c], stream)) return rc;
+    if (int rc = cf_forward3(h, h->d_noise, U0, U1, U2, stream)) return rc;
     if (int rc = cf_project(h, U0, U1, U2, stream)) return rc;
     return cf_pressure(h, U0, U1, U2, P, stream);
 }

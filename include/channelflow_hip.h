@@ -189,6 +189,10 @@ int cf_last_gradient(cf_problem* h, double out[2], void* stream);
 int cf_forward(cf_problem* h, const double* u, double* U, void* stream);
 /* InverseFFT! (src/ffts.jl:151-166): resolved spectrum U -> padded physical u */
 int cf_inverse(cf_problem* h, const double* U, double* u, void* stream);
+/* the same for a vector field (FFT!/IFFT! of VectorField, ffts.jl:198-227):
+ * u holds 3 contiguous padded physical fields; one launch per pass for all three */
+int cf_forward3(cf_problem* h, const double* u, double* U0, double* U1, double* U2, void* stream);
+int cf_inverse3(cf_problem* h, const double* U0, const double* U1, const double* U2, double* u, void* stream);
 
 /* NonLinearTerm of the problem's form, V = U + Ub e_x: rotating N = FFT(V x curl V)
  * (src/nonlinear.jl:173-193), convective N = -FFT((V.grad)V) (113-140) or

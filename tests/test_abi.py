@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "channelflow_hip.h")
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(cf_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(cf_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_header_symbols():

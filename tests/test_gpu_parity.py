@@ -293,3 +293,20 @@ def test_flow_shortened_final_step(gpu, span_steps):
     env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, max(n, 1))), [U0, P0], _split((U1, P1)), trials=2)
     assert_parity(f"flow {span_steps}", _split((st.velocity.cpu().numpy(), st.stagepressure.cpu().numpy())),
                   _split((U1, P1)), env)
+
+
+@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 17, 9), (16, 33, 12), (10, 97, 12)])
+def test_vector_transforms_equal_single_field(gpu, dims):
+    """cf_forward3 / cf_inverse3 (one launch per pass for three fields) give
+    the single-field transforms bit for bit."""
+    import torch
+    cf = gpu
+    gc, go, prob = make(cf, dims)
+    rng = np.random.default_rng(sum(dims))
+    u = torch.as_tensor(rng.standard_normal((3,) + gc.phys_shape), device="cuda")
+    U3 = cf.FFT(prob, u)
+    for c in range(3):
+        assert torch.equal(U3[c], cf.FFT(prob, u[c]))
+    v3 = cf.IFFT(prob, U3)
+    for c in range(3):
+        assert torch.equal(v3[c], cf.IFFT(prob, U3[c]))
