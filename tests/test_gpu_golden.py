@@ -3,8 +3,8 @@
 * Waleffe equilibria (RotatingForm, the default hot path): the GPU trajectory
   must reproduce validation/results/waleffe.csv (energy, dissipation, wall
   power to 17 printed digits) — a well-conditioned, reference-produced pin.
-* Exact viscous decay and Tollmien-Schlichting growth on the GPU (north-star
-  analytic checks), with the reference's own assertion bounds
+* Exact viscous decay and Tollmien-Schlichting growth on the GPU (the
+  analytic checks of the step), with the reference's own assertion bounds
   (validation/viscous_decay.jl:223-259, tollmien_schlichting.jl:128-157).
 """
 import numpy as np
