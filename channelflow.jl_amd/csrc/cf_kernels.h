@@ -20,6 +20,18 @@ enum { Y_STAGE = 0, Y_PROLOGUE, Y_SOLVE, Y_NEGLAP, Y_PRESSURE, Y_DCT_FWD, Y_DCT_
 // nonlinear forms (src/nonlinear.jl:113-193); values match CF_FORM_* of the C ABI
 enum { FORM_ROTATING = 0, FORM_CONVECTIVE = 1, FORM_DIVERGENCE = 2 };
 
+// Mode order of a Y_STAGE launch.  on = 0: block b owns modes b*M .. b*M+M-1.
+// on = 1: the blocks below nyq_first own the nreg non-Nyquist modes (nxr per
+// z row, z row zskip left out), so a launch whose regular blocks fill whole
+// rounds of the machine has no partly filled last round; blocks from
+// nyq_first on (nyq_blocks of them) handle the nnyq Nyquist modes
+// elementwise.
+struct YOrder {
+    int on, nxr, zskip, nnyq;
+    unsigned nyq_first, nyq_blocks;
+    long long nreg;
+};
+
 struct YArgs {
     GridDims gd;
     FFTPlan dct;  // length 2P
@@ -55,6 +67,7 @@ struct YArgs {
     double* grad_out;  // 2 doubles (device) or null
     int nfields;
     double* partial;   // Y_DOT / Y_DISSIPATION: one partial sum per block (device)
+    YOrder order;      // Y_STAGE: set by the launcher (zero: plain order)
 };
 
 struct ZArgs {

@@ -80,6 +80,20 @@ struct Tile {
         bufB = bufA + W * ld;
         stash = bufA + mw * 12 * L;
     }
+    // Non-Nyquist-first order (YOrder): the wave's first slot sw0 becomes a
+    // mode index.  W divides nxr, so the wave's W slots lie in one z row and
+    // its modes stay consecutive; a wave past the last non-Nyquist mode is
+    // empty (modes >= B: zero loads, no stores).
+    __device__ __forceinline__ void order_modes(const YOrder& o) {
+        if (sw0 >= o.nreg) {
+            sw0 = gd.B;
+        } else {
+            const unsigned q = (unsigned)sw0 / (unsigned)o.nxr;
+            const int iz = (int)q + ((int)q >= o.zskip ? 1 : 0);
+            sw0 = (long long)iz * gd.Nxh + ((unsigned)sw0 - q * (unsigned)o.nxr);
+        }
+        s = sw0 + mw;
+    }
     // spectral field column of this group's mode (zero beyond P / absent mode)
     __device__ __forceinline__ CVec load(const double2* __restrict__ src, const Group<L>& g) const {
         CVec v;
@@ -420,6 +434,7 @@ __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
+    if (a.order.on) tile.order_modes(a.order);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny;
@@ -518,10 +533,49 @@ __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     if (a.out6[0]) inverse_six<L, SC>(tile, a, g, mi, Un);
 }
 
+// Nyquist modes of a stage under the non-Nyquist-first order, elementwise
+// (blk-th of order.nyq_blocks blocks).  The result equals ystage_body's for
+// such a mode: the transformed nonlinear term is zero there
+// (normalize_forward!), so Q = A_j Q_old + nl_sign*0 (+ F), and U, P and the
+// y-physical outputs are zero.  The x Nyquist column ix = Nx/2 (every iz)
+// comes first, then the z Nyquist row iz = Nz/2 (ix < nxr).
+template <typename SC>
+__device__ __forceinline__ void ystage_nyquist(const YArgs& a, unsigned blk) {
+    const YOrder& o = a.order;
+    const GridDims& gd = a.gd;
+    const unsigned nnyq = (unsigned)o.nnyq;
+    const unsigned total = nnyq * (unsigned)gd.Ny;
+    const unsigned ncol = (gd.Nx % 2 == 0) ? (unsigned)gd.Nz : 0u;
+    const bool six = a.out6[0] != nullptr, vort = a.form != FORM_DIVERGENCE;
+    for (unsigned e = blk * YNT + threadIdx.x; e < total; e += o.nyq_blocks * YNT) {
+        const unsigned n = e / nnyq, q = e - n * nnyq;
+        const long long s = q < ncol ? (long long)q * gd.Nxh + gd.Nx / 2 : (long long)(gd.Nz / 2) * gd.Nxh + (q - ncol);
+        const long long i = s + gd.B * n;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double2 Qc = a.Ncoef[c] ? a.Ncoef[c][i] : cscale(czero(), a.nl_sign);
+            if (a.Fcoef[c]) Qc = cadd(Qc, a.Fcoef[c][i]);
+            if (a.stage != 0) Qc = cfma(a.Aj, a.Q[c][i], Qc);
+            if (a.store_Q) a.Q[c][i] = Qc;
+            a.U[c][i] = czero();
+        }
+        a.P[i] = czero();
+        if (six) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                if (f < 3 || vort) reinterpret_cast<SC*>(a.out6[f])[i] = from_c64<SC>(czero());
+        }
+    }
+}
+
 template <int L, typename SC>
 // two waves per SIMD (<= 256 registers per lane): one wave per SIMD leaves
 // the wave-serial scans and LDS FFTs unhidden
 __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
+    if (a.order.on && blockIdx.x >= a.order.nyq_first) {  // block-uniform
+        ystage_nyquist<SC>(a, blockIdx.x - a.order.nyq_first);
+        return;
+    }
     ystage_body<L, SC>(a, blockIdx.x);
 }
 
@@ -926,6 +980,29 @@ static void allow_lds(K kernel, size_t lds) {
     if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// Non-Nyquist-first order of a Y_STAGE launch (YOrder); returns its grid,
+// or 0 (plain order) when the grid has no Nyquist modes or a wave's W modes
+// would straddle z rows.
+template <int L>
+static unsigned y_stage_order(YArgs& a) {
+    constexpr int M = YNT / L, W = 64 / L;
+    const GridDims& gd = a.gd;
+    const bool xe = gd.Nx % 2 == 0, ze = gd.Nz % 2 == 0;
+    const int nxr = xe ? gd.Nx / 2 : gd.Nxh, nzr = ze ? gd.Nz - 1 : gd.Nz;
+    if ((!xe && !ze) || nxr <= 0 || nzr <= 0 || nxr % W != 0) return 0;
+    YOrder& o = a.order;
+    o.on = 1;
+    o.nxr = nxr;
+    o.zskip = ze ? gd.Nz / 2 : gd.Nz;
+    o.nreg = (long long)nxr * nzr;
+    o.nnyq = (int)(gd.B - o.nreg);
+    o.nyq_first = (unsigned)((o.nreg + M - 1) / M);
+    const long long elems = (long long)o.nnyq * gd.Ny;  // about four per thread
+    o.nyq_blocks = (unsigned)((elems + 4 * YNT - 1) / (4 * YNT));
+    if (o.nyq_blocks == 0) o.nyq_blocks = 1;
+    return o.nyq_first + o.nyq_blocks;
+}
+
 template <int L>
 static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     constexpr int M = YNT / L;
@@ -943,10 +1020,14 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     allow_lds(ydot_kernel<L>, lds);
     allow_lds(ystats_kernel<L>, lds);
     switch (which) {
-        case Y_STAGE:
-            if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, float2>), grid, dim3(YNT), lds, st, a);
-            else hipLaunchKernelGGL((ystage_kernel<L, double2>), grid, dim3(YNT), lds, st, a);
+        case Y_STAGE: {
+            YArgs b = a;
+            const unsigned nb = y_stage_order<L>(b);
+            const dim3 g2 = nb ? dim3(nb) : grid;
+            if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, float2>), g2, dim3(YNT), lds, st, b);
+            else hipLaunchKernelGGL((ystage_kernel<L, double2>), g2, dim3(YNT), lds, st, b);
             break;
+        }
         case Y_PROLOGUE:
             if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, float2>), grid, dim3(YNT), lds, st, a);
             else hipLaunchKernelGGL((yprologue_kernel<L, double2>), grid, dim3(YNT), lds, st, a);

@@ -214,7 +214,12 @@ def _split(up):
     return [U[0], U[1], U[2], P]
 
 
-@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 17, 9), (16, 17, 16), (6, 9, 5), (8, 97, 6), (4, 129, 6)])
+# The stage kernel puts non-Nyquist modes first when a wavefront's modes fit
+# in one z row: (8, 97, 6) x and z Nyquist sets; (8, 65, 5) only the x
+# column; (9, 129, 8) only the z row; (10, 65, 8) keeps the plain order
+# (5 modes per row, 2 per wavefront); (9, 17, 9) has no Nyquist modes.
+@pytest.mark.parametrize("dims", [(8, 17, 8), (9, 17, 9), (16, 17, 16), (6, 9, 5), (8, 97, 6), (4, 129, 6),
+                                  (8, 65, 5), (9, 129, 8), (10, 65, 8)])
 def test_step_small(gpu, dims):
     say("step small", dims)
     U, P, Ur, Pr, _, _, env = run_pair(gpu, dims, 5)
