@@ -97,6 +97,9 @@ struct cf_problem {
     int KT32 = 1, zppt32 = 0, xppt32 = 0, xwaves32 = 0;
     // velocity-gradient x pass (12 FFT lines): in-place width, wavefronts per block
     int xpptg = 0, xwavesg = 0;
+    // rotating form, FP64: split x pass (X_ROT_SPLIT, three LDS lines) in use,
+    // its in-place width and wavefronts per block
+    int xsplit = 0, xppts = 0, xwavess = 0;
 };
 
 struct DeviceGuard {
@@ -314,6 +317,15 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         xa.ppt = h->xppt;
         h->xwaves = x_max_waves(xa);
         if (h->xwaves < 1) return bail(fail(CF_ERR_UNSUPPORTED, "streamwise line exceeds LDS"));
+        if (xa.mode == X_ROTATIONAL) {
+            XArgs xs = xa;
+            xs.mode = X_ROT_SPLIT;
+            xs.ppt = 0;
+            h->xppts = x_inplace_ppt(xs);
+            xs.ppt = h->xppts;
+            h->xwavess = h->xppts ? x_max_waves(xs) : 0;
+            h->xsplit = h->xwavess > 0 ? 1 : 0;
+        }
         {
             XArgs xg = xa;
             xg.mode = X_GRADIENT;
@@ -474,6 +486,11 @@ static NonlinearArgs nonlinear_args(cf_problem* h, bool mixed) {
     if (h->form == CF_FORM_ROTATING) {
         na.zin = make_zargs(h, 6, six, zb, 0, mixed);
         x.mode = X_ROTATIONAL;
+        if (h->xsplit && !mixed) {
+            x.mode = X_ROT_SPLIT;
+            x.ppt = h->xppts;
+            x.waves = h->xwavess;
+        }
         for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
         for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a wavefront reads its line first
     } else if (h->form == CF_FORM_CONVECTIVE) {
@@ -1285,25 +1302,32 @@ extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
     }
     h->KT = bestKT;
     h->zppt = bestPPT;
-    // x pass: wavefronts per block (even padded Nx)
+    // x pass: wavefronts per block (even padded Nx), and for the rotating
+    // form the six-line or the split (three-line) variant
     if (!rc && h->gd.Nxp % 2 == 0) {
-        const int maxw = h->xwaves;
-        int bestW = maxw;
+        const int maxw = h->xwaves, maxws = h->xwavess, keepSplit = h->xsplit;
+        int bestW = maxw, bestWs = maxws, bestSplit = keepSplit;
         best = 1e30f;
-        for (int w = 1; w <= maxw && !rc; ++w) {
-            h->xwaves = w;
-            const NonlinearArgs na = nonlinear_args(h, false);
-            float ms;
-            rc = timed([&] {
-                HIPTRY(launch_xpass(na.x, st));
-                return 0;
-            }, &ms);
-            if (!rc && ms < best) {
-                best = ms;
-                bestW = w;
+        for (int split = 0; split <= (maxws > 0 ? 1 : 0) && !rc; ++split) {
+            h->xsplit = split;
+            for (int w = 1; w <= (split ? maxws : maxw) && !rc; ++w) {
+                (split ? h->xwavess : h->xwaves) = w;
+                const NonlinearArgs na = nonlinear_args(h, false);
+                float ms;
+                rc = timed([&] {
+                    HIPTRY(launch_xpass(na.x, st));
+                    return 0;
+                }, &ms);
+                if (!rc && ms < best) {
+                    best = ms;
+                    bestSplit = split;
+                    (split ? bestWs : bestW) = w;
+                }
             }
         }
+        h->xsplit = rc ? keepSplit : bestSplit;
         h->xwaves = bestW;
+        h->xwavess = bestWs;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
@@ -1314,7 +1338,7 @@ extern "C" int cf_problem_tiles(const cf_problem* h, int out[3]) {
     if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
     out[0] = h->KT;
     out[1] = h->zppt;
-    out[2] = h->xwaves;
+    out[2] = h->xsplit ? h->xwavess : h->xwaves;  // of the x pass variant in use
     return CF_OK;
 }
 

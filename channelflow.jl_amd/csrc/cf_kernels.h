@@ -82,7 +82,10 @@ struct ZArgs {
     double2* out[9];
 };
 
-enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD, X_CONVECTIVE, X_DIVERGENCE, X_GRADIENT };
+// X_ROT_SPLIT: the rotating form's products with the velocity lines
+// transformed first and held in registers, so the LDS region holds three
+// lines instead of six (more wavefronts per CU)
+enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD, X_CONVECTIVE, X_DIVERGENCE, X_GRADIENT, X_ROT_SPLIT };
 
 struct XArgs {
     GridDims gd;
@@ -167,7 +170,7 @@ hipError_t launch_ykernel_batch(int which, int L, const YArgs* tab, const unsign
 hipError_t launch_zpass_batch(int dir, int ppt, const ZArgs* tab, const unsigned* start, int n, unsigned nblocks,
                               size_t lds, hipStream_t st);
 hipError_t launch_xpass_batch(int mode, int ppt, int waves, const XArgs* tab, const unsigned* start, int n,
-                              unsigned nblocks, size_t lds, hipStream_t st);  // rotating / convective, even Nxp
+                              unsigned nblocks, size_t lds, hipStream_t st);  // rotating (six-line or split) / convective, even Nxp
 unsigned z_blocks(const ZArgs& a);  // grid of one instance's z pass
 unsigned x_blocks(const XArgs& a);  // grid of one instance's even-Nxp x pass
 

@@ -228,7 +228,7 @@ struct XCtx {
 // A block has a.waves wavefronts (64 * a.waves threads).
 //   gradient   in V, dV/dy, dV/dz (9);     out dV_i/dx_j on the padded points (9 real fields, no forward FFT)
 __host__ __device__ constexpr int x_fft_lines_in(int mode) {
-    return (mode == X_CONVECTIVE || mode == X_GRADIENT) ? 12 : (mode == X_DIVERGENCE ? 3 : 6);
+    return (mode == X_CONVECTIVE || mode == X_GRADIENT) ? 12 : ((mode == X_DIVERGENCE || mode == X_ROT_SPLIT) ? 3 : 6);
 }
 __host__ __device__ constexpr int x_fields_in(int mode) {
     return (mode == X_CONVECTIVE || mode == X_GRADIENT) ? 9 : (mode == X_DIVERGENCE ? 3 : 6);
@@ -269,7 +269,58 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
     const long long gl = (long long)block * a.waves + wave;
     if (gl >= nlines) return;  // whole wavefront leaves; no block barrier follows
     C* bufA = smem + 2 * M + 1 + (size_t)wave * x_region_slots(MODE, PPT, ld);
-    if constexpr (PPT > 0) {
+    if constexpr (MODE == X_ROT_SPLIT) {
+        // rotating form, three LDS lines: velocity lines first, their physical
+        // values kept in registers (points j = lane + 64 i; 3 M <= 64 PPT
+        // bounds i), then the vorticity lines in the same buffer
+        static_assert(PPT > 0, "the split rotating pass is in place");
+        constexpr int JM = (PPT + 2) / 3;
+        C u[3][JM];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            for (int idx = lane; idx < 3 * M; idx += 64) {
+                const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
+                const C Xk = x_coef<C>(a, 3 * half + q, gl, k, M);
+                const C Xc = cconj(x_coef<C>(a, 3 * half + q, gl, M - k, M));
+                const C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
+                bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
+            }
+            wave_sync();
+            lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, 3, ld, +1, lane, 64);
+            if (half == 0) {
+#pragma unroll
+                for (int i = 0; i < JM; ++i) {
+                    const int j = lane + 64 * i;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) u[q][i] = j < M ? bufA[q * ld + j] : cplx<C>(0, 0);
+                }
+                wave_sync();  // every lane has read before the vorticity lines overwrite
+            }
+        }
+        // V x Omega at x = 2j (real parts) and 2j+1 (imaginary parts), in place
+#pragma unroll
+        for (int i = 0; i < JM; ++i) {
+            const int j = lane + 64 * i;
+            if (j < M) {
+                const C w0 = bufA[0 * ld + j], w1 = bufA[1 * ld + j], w2 = bufA[2 * ld + j];
+                const C u0 = u[0][i], u1 = u[1][i], u2 = u[2][i];
+                bufA[0 * ld + j] = cplx<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
+                bufA[1 * ld + j] = cplx<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
+                bufA[2 * ld + j] = cplx<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+            }
+        }
+        wave_sync();
+        lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, 3, ld, -1, lane, 64);
+        for (int idx = lane; idx < 3 * Nxh; idx += 64) {
+            const int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
+            const C Zk = bufA[c * ld + k];
+            const C Zc = cconj(bufA[c * ld + ((M - k) % M)]);
+            const C w = twx[k];  // exp(-2 pi i k / Nxp)
+            reinterpret_cast<C*>(a.out[c])[(size_t)gl * Nxh + k] =
+                csub(cscale(cadd(Zk, Zc), T(0.5)), cscale(cmuli(cmul(w, csub(Zk, Zc))), T(0.5)));
+        }
+        return;
+    } else if constexpr (PPT > 0) {
         // pack straight from HBM into the single buffer; FFT line q reads
         // field src(q), times i alpha k for the convective x derivatives
         for (int idx = lane; idx < NFFT * M; idx += 64) {
@@ -577,8 +628,8 @@ hipError_t launch_zpass_batch(int dir, int ppt, const ZArgs* tab, const unsigned
 }
 
 static bool x_wave_path(const XArgs& a) {
-    return a.gd.Nxp % 2 == 0 &&
-           (a.mode == X_ROTATIONAL || a.mode == X_CONVECTIVE || a.mode == X_DIVERGENCE || a.mode == X_GRADIENT);
+    return a.gd.Nxp % 2 == 0 && (a.mode == X_ROTATIONAL || a.mode == X_CONVECTIVE || a.mode == X_DIVERGENCE ||
+                                 a.mode == X_GRADIENT || a.mode == X_ROT_SPLIT);
 }
 
 int x_phys_lines(int mode) {
@@ -592,6 +643,11 @@ static size_t x_wave_lds(const XArgs& a, int waves) {
 
 int x_inplace_ppt(const XArgs& a) {
     if (a.gd.Nxp % 2 != 0) return 0;
+    if (a.mode == X_ROT_SPLIT) {  // in place only
+        for (int p : {5, 9})
+            if (fft_inplace_fits(a.plan, x_buffer_lines(a.mode), p, 64)) return p;
+        return 0;
+    }
     for (int p : {9, 18})
         if (fft_inplace_fits(a.plan, x_buffer_lines(a.mode), p, 64)) return p;
     return 0;
@@ -632,6 +688,10 @@ hipError_t launch_xpass_batch(int mode, int ppt, int waves, const XArgs* tab, co
         if (ppt == 18) CF_XB(M, 18)      \
         if (ppt == 0) CF_XB(M, 0)        \
     }
+    if (mode == X_ROT_SPLIT) {
+        if (ppt == 5) CF_XB(X_ROT_SPLIT, 5)
+        if (ppt == 9) CF_XB(X_ROT_SPLIT, 9)
+    }
     if (mode == X_ROTATIONAL) CF_XBM(X_ROTATIONAL)
     if (mode == X_CONVECTIVE) CF_XBM(X_CONVECTIVE)
 #undef CF_XBM
@@ -643,7 +703,8 @@ hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
     long long nlines = (long long)a.gd.Nzp * a.gd.Ny;
     const bool wave = x_wave_path(a);
     if (wave && (a.waves < 1 || a.waves > XNT / 64)) return hipErrorInvalidValue;
-    if (wave && a.ppt != 0 && a.ppt != 9 && a.ppt != 18) return hipErrorInvalidValue;
+    if (wave && a.mode == X_ROT_SPLIT && a.ppt != 5 && a.ppt != 9) return hipErrorInvalidValue;
+    if (wave && a.mode != X_ROT_SPLIT && a.ppt != 0 && a.ppt != 9 && a.ppt != 18) return hipErrorInvalidValue;
     if (a.mixed && (!wave || a.ppt == 0)) return hipErrorInvalidValue;  // FP32 fields: in-place wave path only
     const int per_block = wave ? a.waves : a.LPB;
     dim3 grid((unsigned)((nlines + per_block - 1) / per_block));
@@ -674,6 +735,14 @@ hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
             case 18: CF_XWAVE(MODE, 18, double2)          \
             default: CF_XWAVE(MODE, 0, double2)           \
         }                                                 \
+    }
+    if (a.mode == X_ROT_SPLIT) {
+        if (a.mixed) {
+            if (a.ppt == 5) CF_XWAVE(X_ROT_SPLIT, 5, float2)
+            CF_XWAVE(X_ROT_SPLIT, 9, float2)
+        }
+        if (a.ppt == 5) CF_XWAVE(X_ROT_SPLIT, 5, double2)
+        CF_XWAVE(X_ROT_SPLIT, 9, double2)
     }
     if (a.mode == X_ROTATIONAL) CF_XMODE(X_ROTATIONAL)
     if (a.mode == X_CONVECTIVE) CF_XMODE(X_CONVECTIVE)
