@@ -100,6 +100,8 @@ struct cf_problem {
     // rotating form, FP64: split x pass (X_ROT_SPLIT, three LDS lines) in use,
     // its in-place width and wavefronts per block
     int xsplit = 0, xppts = 0, xwavess = 0;
+    // the same for the mixed step's FP32 x pass
+    int xsplit32 = 0, xppts32 = 0, xwavess32 = 0;
 };
 
 struct DeviceGuard {
@@ -340,6 +342,15 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
             h->xppt32 = x_inplace_ppt(xa);
             xa.ppt = h->xppt32;
             h->xwaves32 = x_max_waves(xa);
+            if (xa.mode == X_ROTATIONAL) {
+                XArgs xs = xa;
+                xs.mode = X_ROT_SPLIT;
+                xs.ppt = 0;
+                h->xppts32 = x_inplace_ppt(xs);
+                xs.ppt = h->xppts32;
+                h->xwavess32 = h->xppts32 ? x_max_waves(xs) : 0;
+                h->xsplit32 = h->xwavess32 > 0 ? 1 : 0;
+            }
         }
     }
     if (cfg->precision == CF_PRECISION_MIXED) {
@@ -486,10 +497,10 @@ static NonlinearArgs nonlinear_args(cf_problem* h, bool mixed) {
     if (h->form == CF_FORM_ROTATING) {
         na.zin = make_zargs(h, 6, six, zb, 0, mixed);
         x.mode = X_ROTATIONAL;
-        if (h->xsplit && !mixed) {
+        if (mixed ? h->xsplit32 : h->xsplit) {
             x.mode = X_ROT_SPLIT;
-            x.ppt = h->xppts;
-            x.waves = h->xwavess;
+            x.ppt = mixed ? h->xppts32 : h->xppts;
+            x.waves = mixed ? h->xwavess32 : h->xwavess;
         }
         for (int i = 0; i < 6; ++i) x.in[i] = zb[i];
         for (int i = 0; i < 3; ++i) x.out[i] = zb[i];  // in place: a wavefront reads its line first

@@ -653,6 +653,16 @@ __device__ __noinline__ void stockham_pass_generic(const C* src, C* dst, int n, 
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): block b runs on XCD b % 8 as that XCD's (b / 8)-th block.  The
+// returned logical index gives XCD x a contiguous range of logical blocks,
+// so neighbouring tiles that share cache lines share an L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned nblocks) {
+    unsigned q = nblocks / NXCD, r = nblocks % NXCD, x = b % NXCD, slot = b / NXCD;
+    return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 // Instance of a batched (ensemble) launch owning block b: the largest i < n
 // with start[i] <= b (start is ascending, start[0] = 0).  Uniform per block.
 __device__ __forceinline__ int batch_instance(const unsigned* __restrict__ start, int n, unsigned b) {

@@ -25,9 +25,11 @@ enum { FORM_ROTATING = 0, FORM_CONVECTIVE = 1, FORM_DIVERGENCE = 2 };
 // z row, z row zskip left out), so a launch whose regular blocks fill whole
 // rounds of the machine has no partly filled last round; blocks from
 // nyq_first on (nyq_blocks of them) handle the nnyq Nyquist modes
-// elementwise.
+// elementwise.  xcd = 1: the regular blocks run in XCD-contiguous order
+// (xcd_logical), so blocks sharing the cache lines of unaligned row segments
+// share one L2.
 struct YOrder {
-    int on, nxr, zskip, nnyq;
+    int on, xcd, nxr, zskip, nnyq;
     unsigned nyq_first, nyq_blocks;
     long long nreg;
 };

@@ -19,16 +19,6 @@ namespace cf {
 constexpr int ZNT = 256;
 constexpr int XNT = 256;
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2): block b runs on XCD b % 8 as that XCD's (b / 8)-th block.  The
-// returned logical index gives XCD x a contiguous range of logical blocks,
-// so neighbouring tiles that share cache lines share an L2.
-constexpr int NXCD = 8;
-__device__ __forceinline__ unsigned xcd_logical(unsigned b, unsigned nblocks) {
-    unsigned q = nblocks / NXCD, r = nblocks % NXCD, x = b % NXCD, slot = b / NXCD;
-    return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-}
-
 // z inverse (dir=+1): (k,l,y) -> (k,zp,y); z forward (dir=-1): (k,zp,y) -> (k,l,y)
 // 1-D grid of ntiles * Ny * nfields blocks, k tile fastest in logical order.
 // C = double2: FP64 fields; C = float2: FP32 fields of the mixed step (in-place
