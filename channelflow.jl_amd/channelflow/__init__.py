@@ -13,6 +13,7 @@ device tensor of shape (Ny, Nz, Nxh) (= (k, l, n) column-major), a vector
 field stacks three of them, a padded physical field is a float64 tensor of
 shape (Ny, Nzp, Nxp) (= (x, z, y) column-major).
 """
+import contextlib
 import ctypes as C
 import math
 
@@ -296,14 +297,61 @@ class _Forcing:
         return False
 
 
+class _Nonlinear:
+    """nonlinear(tstage, U, N) (cnrk2.jl:121-126,172) given by the caller in
+    place of the problem's NonLinearTerm, installed for one call: at every
+    stage the library calls back and `nonlinear` must overwrite N with the
+    signed nonlinear acceleration of the total velocity, with device
+    operations enqueued on the step's stream.  The same workspace rule as for
+    forcing applies."""
+
+    def __init__(self, scheme, U, nonlinear):
+        self.scheme, self.U, self.nonlinear = scheme, U, nonlinear
+        N = getattr(scheme, "_nl_buffer", None)
+        if N is None:
+            N = scheme._nl_buffer = scheme.grid.zeros_spec(3)
+        self.N = N
+        self.error = None
+
+        def cb(user, tstage, u0, u1, u2, n0, n1, n2, stream):
+            try:
+                self.nonlinear(tstage, self.U, self.N)
+                return 0
+            except BaseException as e:  # reported after the call returns
+                self.error = e
+                return 1
+
+        self.fn = _lib.FORCING_FN(cb)
+
+    def __enter__(self):
+        N = self.N
+        _lib.check(_lib.load().cf_set_nonlinear(self.scheme.h, self.fn, None, _ptr(N[0]), _ptr(N[1]), _ptr(N[2])))
+        return self
+
+    def __exit__(self, *exc):
+        _lib.load().cf_set_nonlinear(self.scheme.h, _lib.FORCING_FN(), None, None, None, None)
+        if self.error is not None:
+            raise self.error
+        return False
+
+
 def step_(scheme, nlterm, U, P, t, forcing=None, pressuregradient=None, bulkvelocity=None, nsteps=1, stream=None):
     """Advance (U, P) in place by scheme.dt (nsteps times); return the new time.
-    forcing(tstage, U, F) adds a spectral acceleration into F at the stage
-    times t + (0, 1/3, 3/4) dt (cnrk2.jl:149-207)."""
+    nlterm(tstage, U, N) overwrites N with the signed nonlinear acceleration
+    (the problem's NonLinearTerm runs on the device path of its form; any
+    other callable is called back at each stage); forcing(tstage, U, F) adds
+    a spectral acceleration into F.  Both are sampled at the stage times
+    t + (0, 1/3, 3/4) dt (cnrk2.jl:149-207)."""
     if forcing is not None and not callable(forcing):
         raise ArgumentError("forcing must be callable as forcing(t, U, F)")
-    if not isinstance(nlterm, NonLinearTerm) or nlterm.scheme is not scheme:
-        raise ArgumentError("nlterm must be the problem's nonlinear term")
+    if isinstance(nlterm, NonLinearTerm):
+        if nlterm.scheme is not scheme:
+            raise ArgumentError("nlterm belongs to another problem")
+        custom = None
+    elif callable(nlterm):
+        custom = nlterm
+    else:
+        raise ArgumentError("nlterm must be callable as nonlinear(t, U, N)")
     kind, vals = _constraint(pressuregradient, bulkvelocity)
     g = scheme.grid
     _check_vec(g, U)
@@ -315,11 +363,12 @@ def step_(scheme, nlterm, U, P, t, forcing=None, pressuregradient=None, bulkvelo
         _lib.check(_lib.load().cf_advance(scheme.h, _ptr(U[0]), _ptr(U[1]), _ptr(U[2]), _ptr(P), float(t),
                                           int(nsteps), kind, arr, C.byref(tout), _stream(stream)))
 
-    if forcing is None:
+    with contextlib.ExitStack() as hooks:
+        if custom is not None:
+            hooks.enter_context(_Nonlinear(scheme, U, custom))
+        if forcing is not None:
+            hooks.enter_context(_Forcing(scheme, U, forcing))
         run()
-    else:
-        with _Forcing(scheme, U, forcing):
-            run()
     return tout.value
 
 

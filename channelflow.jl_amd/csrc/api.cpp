@@ -92,6 +92,11 @@ struct cf_problem {
     cf_forcing_fn force_fn = nullptr;
     void* force_user = nullptr;
     double2* F[3] = {nullptr, nullptr, nullptr};
+    // nonlinear(tstage, U, N) hook replacing the built-in form: host callback
+    // and the caller's N fields (coefficients)
+    cf_forcing_fn nl_fn = nullptr;
+    void* nl_user = nullptr;
+    double2* NL[3] = {nullptr, nullptr, nullptr};
     // mixed-precision step: FP32 y-physical fields and periodic passes
     int mixed = 0;
     int KT32 = 1, zppt32 = 0, xppt32 = 0, xwaves32 = 0;
@@ -646,23 +651,36 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
     const GridDims& gd = h->gd;
     const bool mixed = h->mixed != 0;
     const bool forced = h->force_fn != nullptr;
+    const bool custom = h->nl_fn != nullptr;  // caller's nonlinear(tstage, U, N) in coefficients
     if (!(dt > 0.0)) dt = h->dt;
     for (int step = 0; step < nsteps; ++step) {
         if (int rc = tmark(tm, -1, st)) return rc;
-        if (int rc = enqueue_prologue(h, U, st, mixed)) return rc;
+        if (!custom)
+            if (int rc = enqueue_prologue(h, U, st, mixed)) return rc;
         if (int rc = tmark(tm, 0, st)) return rc;
         for (int j = 0; j < 3; ++j) {
-            if (int rc = enqueue_nonlinear_phys(h, st, tm, mixed)) return rc;
+            const double ts = t0 + (double)step * dt + STAGE_T[j] * dt;
             double2* Ncoef[3] = {fld(h->R, gd.S, 0), fld(h->R, gd.S, 1), fld(h->R, gd.S, 2)};
-            if (h->form == CF_FORM_DIVERGENCE)
-                if (int rc = enqueue_nonlinear_coef(h, Ncoef, st)) return rc;
+            if (custom) {  // nonlinear(tstage, U, N) overwrites N (cnrk2.jl:121-126,173)
+                for (int i = 0; i < 3; ++i) Ncoef[i] = h->NL[i];
+                if (h->nl_fn(h->nl_user, ts, (const double*)U[0], (const double*)U[1], (const double*)U[2],
+                             (double*)h->NL[0], (double*)h->NL[1], (double*)h->NL[2], st) != 0)
+                    return fail(CF_ERR_ARGUMENT, "nonlinear callback failed");
+            } else {
+                if (int rc = enqueue_nonlinear_phys(h, st, tm, mixed)) return rc;
+                if (h->form == CF_FORM_DIVERGENCE)
+                    if (int rc = enqueue_nonlinear_coef(h, Ncoef, st)) return rc;
+            }
             if (forced) {
-                const double ts = t0 + (double)step * dt + STAGE_T[j] * dt;
                 if (h->force_fn(h->force_user, ts, (const double*)U[0], (const double*)U[1], (const double*)U[2],
                                 (double*)h->F[0], (double*)h->F[1], (double*)h->F[2], st) != 0)
                     return fail(CF_ERR_ARGUMENT, "forcing callback failed");
             }
-            const YArgs a = stage_args(h, U, P, j, c, dt, banks ? banks[j] : &h->stage[j], mixed, Ncoef, forced);
+            YArgs a = stage_args(h, U, P, j, c, dt, banks ? banks[j] : &h->stage[j], mixed, Ncoef, forced);
+            if (custom) {  // N arrives in coefficients; no y-physical fields are needed
+                for (int i = 0; i < 3; ++i) a.Ncoef[i] = h->NL[i];
+                for (int i = 0; i < 6; ++i) a.out6[i] = nullptr;
+            }
             HIPTRY(launch_ykernel(Y_STAGE, a, st));
             if (int rc = tmark(tm, 4, st)) return rc;
         }
@@ -867,6 +885,20 @@ extern "C" int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, doubl
     h->F[0] = fn ? (double2*)F0 : nullptr;
     h->F[1] = fn ? (double2*)F1 : nullptr;
     h->F[2] = fn ? (double2*)F2 : nullptr;
+    return CF_OK;
+}
+
+extern "C" int cf_set_nonlinear(cf_problem* h, cf_forcing_fn fn, void* user, double* N0, double* N1, double* N2) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    if (fn) {
+        const void* f[3] = {N0, N1, N2};
+        if (int rc = check_fields(f, 3)) return rc;
+    }
+    h->nl_fn = fn;
+    h->nl_user = fn ? user : nullptr;
+    h->NL[0] = fn ? (double2*)N0 : nullptr;
+    h->NL[1] = fn ? (double2*)N1 : nullptr;
+    h->NL[2] = fn ? (double2*)N2 : nullptr;
     return CF_OK;
 }
 
@@ -1619,6 +1651,7 @@ extern "C" int cf_ensemble_create(cf_problem* const* problems, double* const* fi
         if (h->device != problems[0]->device) return fail(CF_ERR_ARGUMENT, "ensemble problems must share a device");
         if (h->mixed) return fail(CF_ERR_UNSUPPORTED, "ensembles run the FP64 step");
         if (h->force_fn) return fail(CF_ERR_UNSUPPORTED, "a forced problem cannot join an ensemble (host callback)");
+        if (h->nl_fn) return fail(CF_ERR_UNSUPPORTED, "a problem with a nonlinear callback cannot join an ensemble");
         if (h->form == CF_FORM_DIVERGENCE || h->gd.Nxp % 2 != 0)
             return fail(CF_ERR_UNSUPPORTED, "ensembles need the rotating or convective form and an even padded Nx");
         for (int f = 0; f < 4; ++f) all.push_back(fields[4 * i + f]);

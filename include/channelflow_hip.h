@@ -171,6 +171,21 @@ typedef int (*cf_forcing_fn)(void* user, double tstage, const double* U0, const 
                              double* F0, double* F1, double* F2, void* stream);
 int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, double* F0, double* F1, double* F2);
 
+/* nonlinear(tstage, U, N) (cnrk2.jl:121-126,172): the step's nonlinear term
+ * supplied by the caller in place of the problem's form.  At each stage of
+ * cf_step / cf_advance / cf_advance_to / cf_step_timed the library calls
+ * fn(user, tstage, U0, U1, U2, N0, N1, N2, stream) on the host with
+ * tstage = t + (0, 1/3, 3/4) dt, before the forcing callback.  fn must
+ * ENQUEUE on `stream` device work that OVERWRITES N0..N2 (spectral fields
+ * owned by the caller, given here) with the signed nonlinear acceleration of
+ * the total velocity U + Ub e_x, the term the stage adds to Q as is.  The
+ * same restrictions as for the forcing callback apply (no host reads of U,
+ * no calls on this handle that use its workspaces).  The step then runs no
+ * transforms of its own.  A nonzero return stops the step with
+ * CF_ERR_ARGUMENT.  fn = NULL restores the problem's form.  Such a problem
+ * cannot join an ensemble. */
+int cf_set_nonlinear(cf_problem* h, cf_forcing_fn fn, void* user, double* N0, double* N1, double* N2);
+
 /* Tile autotuner (no counterpart in the reference; a performance setting):
  * times the nonlinear term's z passes for each tile width and its x pass for
  * each wavefront count on the problem's workspaces, on `stream`, and keeps
