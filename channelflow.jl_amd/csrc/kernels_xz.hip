@@ -18,6 +18,7 @@ namespace cf {
 
 constexpr int ZNT = 256;
 constexpr int XNT = 256;
+constexpr int XLOADS = 2;  // x pass: pack iterations whose HBM loads are issued together
 
 // z inverse (dir=+1): (k,l,y) -> (k,zp,y); z forward (dir=-1): (k,zp,y) -> (k,l,y)
 // 1-D grid of ntiles * Ny * nfields blocks, k tile fastest in logical order.
@@ -44,32 +45,67 @@ __device__ __forceinline__ void zpass_body(const ZArgs& a, unsigned lb) {
     const C zero = cplx<C>(T(0), T(0));
     if constexpr (PPT > 0)  // in-place mode: the second buffer holds the twiddle table
         for (int i = tid; i < Nzp; i += ZNT) bufB[i] = from_c64<C>(a.plan.tw[i]);
+    // Loads are issued in batches of ZLOADS per thread before their LDS
+    // stores, so each thread keeps several HBM requests in flight (a store
+    // right after each load would wait for every load in turn).
+    constexpr int ZLOADS = 4;
+    const int total = KT * Nzp;
     if (DIR > 0) {
         const C* in = reinterpret_cast<const C*>(a.in[f]) + (size_t)Nxh * Nz * y;
         const bool dz = (a.dz_mask >> f) & 1u;  // emit d/dz: multiply by i beta l
-        for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
-            int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
-            int k = k0 + kk;
-            C v = zero;
-            if (k < Nxh && !(xeven && k == gd.Nx / 2)) {
-                int l = -1;
-                if (z < npos) l = (zeven && z == Nz / 2) ? -1 : z;
-                else if (z >= Nzp - nneg) l = npos + (z - (Nzp - nneg));
-                if (l >= 0) {
-                    v = in[k + (size_t)Nxh * l];
-                    if (dz) v = cmuli(cscale(v, T(gd.beta * (l <= Nz / 2 ? l : l - Nz))));
+        for (int base = tid; base < total; base += ZLOADS * ZNT) {
+            C v[ZLOADS];
+            int lw[ZLOADS];  // wavenumber label l of the loaded value (dz scale)
+#pragma unroll
+            for (int b = 0; b < ZLOADS; ++b) {
+                const int idx = base + b * ZNT;
+                v[b] = zero;
+                lw[b] = 0;
+                if (idx < total) {
+                    int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
+                    int k = k0 + kk;
+                    if (k < Nxh && !(xeven && k == gd.Nx / 2)) {
+                        int l = -1;
+                        if (z < npos) l = (zeven && z == Nz / 2) ? -1 : z;
+                        else if (z >= Nzp - nneg) l = npos + (z - (Nzp - nneg));
+                        if (l >= 0) {
+                            v[b] = in[k + (size_t)Nxh * l];
+                            lw[b] = l <= Nz / 2 ? l : l - Nz;
+                        }
+                    }
                 }
             }
-            bufA[kk * ld + z] = v;
+#pragma unroll
+            for (int b = 0; b < ZLOADS; ++b) {
+                const int idx = base + b * ZNT;
+                if (idx < total) {
+                    int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
+                    bufA[kk * ld + z] = dz ? cmuli(cscale(v[b], T(gd.beta * lw[b]))) : v[b];
+                }
+            }
         }
     } else {
         const C* in = reinterpret_cast<const C*>(a.in[f]) + (size_t)Nxh * Nzp * y;
-        for (int idx = tid; idx < KT * Nzp; idx += ZNT) {
-            int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
-            int k = k0 + kk;
-            C v = zero;
-            if (k < Nxh) v = in[k + (size_t)Nxh * z];
-            bufA[kk * ld + z] = v;
+        for (int base = tid; base < total; base += ZLOADS * ZNT) {
+            C v[ZLOADS];
+#pragma unroll
+            for (int b = 0; b < ZLOADS; ++b) {
+                const int idx = base + b * ZNT;
+                v[b] = zero;
+                if (idx < total) {
+                    int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
+                    int k = k0 + kk;
+                    if (k < Nxh) v[b] = in[k + (size_t)Nxh * z];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < ZLOADS; ++b) {
+                const int idx = base + b * ZNT;
+                if (idx < total) {
+                    int z = (int)fast_div((unsigned)idx, a.mul_kt), kk = idx - z * KT;
+                    bufA[kk * ld + z] = v[b];
+                }
+            }
         }
     }
     __syncthreads();
@@ -268,12 +304,29 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
         C u[3][JM];
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
-            for (int idx = lane; idx < 3 * M; idx += 64) {
-                const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
-                const C Xk = x_coef<C>(a, 3 * half + q, gl, k, M);
-                const C Xc = cconj(x_coef<C>(a, 3 * half + q, gl, M - k, M));
-                const C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
-                bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
+            // XLOADS iterations' HBM loads in flight before their LDS stores
+            for (int base = lane; base < 3 * M; base += XLOADS * 64) {
+                C Xk[XLOADS], Xm[XLOADS];
+#pragma unroll
+                for (int b = 0; b < XLOADS; ++b) {
+                    const int idx = base + b * 64;
+                    Xk[b] = Xm[b] = cplx<C>(0, 0);
+                    if (idx < 3 * M) {
+                        const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
+                        Xk[b] = x_coef<C>(a, 3 * half + q, gl, k, M);
+                        Xm[b] = x_coef<C>(a, 3 * half + q, gl, M - k, M);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < XLOADS; ++b) {
+                    const int idx = base + b * 64;
+                    if (idx < 3 * M) {
+                        const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
+                        const C Xc = cconj(Xm[b]);
+                        const C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
+                        bufA[q * ld + k] = cadd(cadd(Xk[b], Xc), cmuli(cmul(w, csub(Xk[b], Xc))));
+                    }
+                }
             }
             wave_sync();
             lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, 3, ld, +1, lane, 64);

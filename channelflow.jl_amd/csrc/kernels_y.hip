@@ -121,12 +121,27 @@ struct Tile {
     __device__ __forceinline__ CVec dct_forward(const SC* __restrict__ src, const FFTPlan& plan, double norm,
                                                 const Group<L>& g, bool zero_mode) {
         const int P = gd.P;
-        for (int idx = lane; idx < (P + 1) * W; idx += 64) {
-            int n = idx / W, m = idx - n * W;
-            long long sm = sw0 + m;
-            double2 v = (sm < gd.B) ? to_c64(src[sm + gd.B * n]) : czero();
-            bufA[m * ld + n] = v;
-            if (n >= 1 && n <= P - 1) bufA[m * ld + 2 * P - n] = v;
+        // YLOADS rows' HBM loads in flight before their LDS stores
+        constexpr int YLOADS = 4;
+        for (int base = lane; base < (P + 1) * W; base += YLOADS * 64) {
+            SC r[YLOADS];
+#pragma unroll
+            for (int b = 0; b < YLOADS; ++b) {
+                const int idx = base + b * 64;
+                const int n = idx / W, m = idx - n * W;
+                const long long sm = sw0 + m;
+                r[b] = (idx < (P + 1) * W && sm < gd.B) ? src[sm + gd.B * n] : from_c64<SC>(czero());
+            }
+#pragma unroll
+            for (int b = 0; b < YLOADS; ++b) {
+                const int idx = base + b * 64;
+                if (idx < (P + 1) * W) {
+                    const int n = idx / W, m = idx - n * W;
+                    const double2 v = to_c64(r[b]);
+                    bufA[m * ld + n] = v;
+                    if (n >= 1 && n <= P - 1) bufA[m * ld + 2 * P - n] = v;
+                }
+            }
         }
         wave_sync();
         double2* res = lds_fft<true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
