@@ -366,23 +366,37 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
     } else if constexpr (PPT > 0) {
         // pack straight from HBM into the single buffer; FFT line q reads
         // field src(q), times i alpha k for the convective x derivatives
-        for (int idx = lane; idx < NFFT * M; idx += 64) {
-            int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
-            int f = q;
-            bool dx = false;
-            if constexpr (MODE == X_CONVECTIVE || MODE == X_GRADIENT) {
-                dx = (q >= 3 && q < 6);
-                f = q < 3 ? q : q - 3;
+        for (int base = lane; base < NFFT * M; base += XLOADS * 64) {
+            C Xk[XLOADS], Xm[XLOADS];  // loads of XLOADS iterations in flight together
+#pragma unroll
+            for (int b = 0; b < XLOADS; ++b) {
+                const int idx = base + b * 64;
+                Xk[b] = Xm[b] = cplx<C>(0, 0);
+                if (idx < NFFT * M) {
+                    const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
+                    int f = q;
+                    if constexpr (MODE == X_CONVECTIVE || MODE == X_GRADIENT) f = q < 3 ? q : q - 3;
+                    Xk[b] = x_coef<C>(a, f, gl, k, M);
+                    Xm[b] = x_coef<C>(a, f, gl, M - k, M);
+                }
             }
-            C Xk = x_coef<C>(a, f, gl, k, M);
-            C Xm = x_coef<C>(a, f, gl, M - k, M);
-            if (dx) {
-                Xk = cmuli(cscale(Xk, T(a.gd.alpha * k)));
-                Xm = cmuli(cscale(Xm, T(a.gd.alpha * (M - k))));
+#pragma unroll
+            for (int b = 0; b < XLOADS; ++b) {
+                const int idx = base + b * 64;
+                if (idx < NFFT * M) {
+                    const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
+                    C xk = Xk[b], xm = Xm[b];
+                    if constexpr (MODE == X_CONVECTIVE || MODE == X_GRADIENT) {
+                        if (q >= 3 && q < 6) {
+                            xk = cmuli(cscale(xk, T(a.gd.alpha * k)));
+                            xm = cmuli(cscale(xm, T(a.gd.alpha * (M - k))));
+                        }
+                    }
+                    const C Xc = cconj(xm);
+                    const C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
+                    bufA[q * ld + k] = cadd(cadd(xk, Xc), cmuli(cmul(w, csub(xk, Xc))));
+                }
             }
-            C Xc = cconj(Xm);
-            C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
-            bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
         }
         wave_sync();
         lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, NFFT, ld, +1, lane, 64);
