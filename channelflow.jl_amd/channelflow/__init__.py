@@ -259,25 +259,51 @@ def _check_vec(g, U):
         _check_spec(g, U[c])
 
 
+@contextlib.contextmanager
+def _on_stream(handle):
+    """Make the library's stream (a hipStream_t passed to a callback) torch's
+    current stream, so that device work a callback enqueues is ordered with
+    the step's kernels on that stream."""
+    torch = _torch()
+    cur = torch.cuda.current_stream()
+    if handle is None or int(handle) == cur.cuda_stream:
+        yield
+        return
+    with torch.cuda.stream(torch.cuda.ExternalStream(int(handle))):
+        yield
+
+
+def _scheme_buffer(scheme, name):
+    """A three-component coefficient buffer owned by the scheme: it lives as
+    long as the problem handle, so kernels still queued on any stream never
+    read memory that the caching allocator has handed out again."""
+    buf = getattr(scheme, name, None)
+    if buf is None:
+        buf = scheme.grid.zeros_spec(3)
+        setattr(scheme, name, buf)
+    return buf
+
+
 class _Forcing:
     """forcing(tstage, U, F) (cnrk2.jl:173-175) installed on the problem
     handle for one call: the library calls back at every stage; the callback
     zeroes F and lets `forcing` add its spectral acceleration into F with
     device operations enqueued on the step's stream (the callback runs while
-    the step is being enqueued, so it must not read U on the host).  The
-    stage's nonlinear term is live in the problem's workspaces at that
-    point: a forcing that needs transforms or gradients runs them on another
-    problem of the same grid."""
+    the step is being enqueued, with that stream made torch's current one,
+    so it must not read U on the host).  The stage's nonlinear term is live
+    in the problem's workspaces at that point: a forcing that needs
+    transforms or gradients runs them on another problem of the same grid."""
 
     def __init__(self, scheme, U, forcing):
         self.scheme, self.U, self.forcing = scheme, U, forcing
-        self.F = scheme.grid.zeros_spec(3)
+        self.F = _scheme_buffer(scheme, "_forcing_buffer")
         self.error = None
 
         def cb(user, tstage, u0, u1, u2, f0, f1, f2, stream):
             try:
-                self.F.zero_()
-                self.forcing(tstage, self.U, self.F)
+                with _on_stream(stream):
+                    self.F.zero_()
+                    self.forcing(tstage, self.U, self.F)
                 return 0
             except BaseException as e:  # reported after the call returns
                 self.error = e
@@ -302,20 +328,18 @@ class _Nonlinear:
     place of the problem's NonLinearTerm, installed for one call: at every
     stage the library calls back and `nonlinear` must overwrite N with the
     signed nonlinear acceleration of the total velocity, with device
-    operations enqueued on the step's stream.  The same workspace rule as for
-    forcing applies."""
+    operations enqueued on the step's stream (made torch's current stream
+    during the callback).  The same workspace rule as for forcing applies."""
 
     def __init__(self, scheme, U, nonlinear):
         self.scheme, self.U, self.nonlinear = scheme, U, nonlinear
-        N = getattr(scheme, "_nl_buffer", None)
-        if N is None:
-            N = scheme._nl_buffer = scheme.grid.zeros_spec(3)
-        self.N = N
+        self.N = _scheme_buffer(scheme, "_nl_buffer")
         self.error = None
 
         def cb(user, tstage, u0, u1, u2, n0, n1, n2, stream):
             try:
-                self.nonlinear(tstage, self.U, self.N)
+                with _on_stream(stream):
+                    self.nonlinear(tstage, self.U, self.N)
                 return 0
             except BaseException as e:  # reported after the call returns
                 self.error = e

@@ -948,9 +948,10 @@ struct Problem {
     std::vector<cplx> Q[3], N[3], R[3];
     double last_grad[2] = {0, 0};
     // forcing(tstage, U, N) (cnrk2.jl:173-175, forcing.jl:3-11) of the form
-    // F(t) = F0 + sin(omega t) F1, added into N at the stage times; t advances by dt per step
+    // F(t) = F0 + sin(omega t) F1 + cos(omega t) F2, added into N at the stage times;
+    // t advances by dt per step
     bool forced = false;
-    std::vector<cplx> F0[3], F1[3];
+    std::vector<cplx> F0[3], F1[3], F2[3];
     double omega = 0.0, t = 0.0;
     Problem(const Grid& g_, const double* basevals, double nu_, double dt_, int form, int kind_, double c0_, double c1_)
         : g(g_), nu(nu_), dt(dt_), kind(kind_), c0(c0_), c1(c1_) {
@@ -967,7 +968,11 @@ struct Problem {
         for (int j = 0; j < 3; ++j) if (!solvers[j]->modes->ok || !solvers[j]->mean->ok) return false;
         return true;
     }
-    // _bulkpressuregradient cnrk2.jl:98-107
+    // _bulkpressuregradient cnrk2.jl:98-107.  `bulkgrad_mutation` plants a
+    // known mistake for the tests that show the closed-form checks catch it
+    // (0 = none, 1 = drop the shear term, 2 = drop the base term, 3 = drop
+    // B[N], 4 = flip the shear sign); it is never set outside those tests.
+    static int bulkgrad_mutation;
     void bulkgrad(const cplx* const U[3], double out[2]) {
         int Ny = g.Ny;
         size_t B = g.B();
@@ -977,7 +982,12 @@ struct Problem {
             for (int m = 0; m < Ny; ++m) { u[m] = U[c][B * m]; n[m] = N[c][B * m]; }
             cplx shear = diff_right(u.data(), Ny) - diff_left(u.data(), Ny);
             double bs = i == 0 ? nu * bulkmean(curv.data(), Ny) : 0.0;
-            out[i] = nu * shear.real() / 2.0 + bs + bulkmean(n.data(), Ny).real();
+            double sh = nu * shear.real() / 2.0, bn = bulkmean(n.data(), Ny).real();
+            if (bulkgrad_mutation == 1) sh = 0.0;
+            if (bulkgrad_mutation == 2) bs = 0.0;
+            if (bulkgrad_mutation == 3) bn = 0.0;
+            if (bulkgrad_mutation == 4) sh = -sh;
+            out[i] = sh + bs + bn;
         }
     }
     // step! cnrk2.jl:149-207 (forcing = NoForcing)
@@ -992,8 +1002,9 @@ struct Problem {
             nl->eval(Uc, Np, false);
             if (forced) {
                 const double sw = std::sin(omega * (t + STAGE_T[j] * dt));
+                const double cw = std::cos(omega * (t + STAGE_T[j] * dt));
                 for (int i = 0; i < 3; ++i)
-                    for (size_t q = 0; q < S; ++q) N[i][q] += F0[i][q] + sw * F1[i][q];
+                    for (size_t q = 0; q < S; ++q) N[i][q] += F0[i][q] + sw * F1[i][q] + cw * F2[i][q];
             }
             double old[2] = {grads[0], grads[1]};
             if (kind == 1) bulkgrad(Uc, old);
@@ -1109,6 +1120,8 @@ static double spectral_dot(const Grid& g, const cplx* U, const cplx* V) {
 // C ABI for the Python test harness (tests/, smoke(), bench cpu_baseline).
 // Complex arrays are interleaved double pairs.  Return 0 on success.
 // ===========================================================================
+int Problem::bulkgrad_mutation = 0;
+
 extern "C" {
 
 const char* orc_last_error() { return g_error.c_str(); }
@@ -1303,20 +1316,28 @@ int orc_step(void* h, double* U3, double* P, int nsteps) {
     for (int s = 0; s < nsteps; ++s) p->step(U, (cplx*)P);
     return 0;
 }
-// forcing F(t) = F0 + sin(omega t) F1 (3 spectral fields each; F0 == NULL
-// removes it); t0 = time of the next step's first stage
-int orc_set_forcing(void* h, const double* F0, const double* F1, double omega, double t0) {
+// forcing F(t) = F0 + sin(omega t) F1 + cos(omega t) F2 (3 spectral fields
+// each; F0 == NULL removes it, F1/F2 == NULL mean zero); t0 = time of the
+// next step's first stage
+int orc_set_forcing(void* h, const double* F0, const double* F1, const double* F2, double omega, double t0) {
     Problem* p = (Problem*)h;
     size_t S = p->g.S();
     p->forced = F0 != nullptr;
     p->omega = omega;
     p->t = t0;
     for (int i = 0; i < 3; ++i) {
-        if (!F0) { p->F0[i].clear(); p->F1[i].clear(); continue; }
+        if (!F0) { p->F0[i].clear(); p->F1[i].clear(); p->F2[i].clear(); continue; }
         p->F0[i].assign((const cplx*)F0 + i * S, (const cplx*)F0 + (i + 1) * S);
         if (F1) p->F1[i].assign((const cplx*)F1 + i * S, (const cplx*)F1 + (i + 1) * S);
         else p->F1[i].assign(S, 0.0);
+        if (F2) p->F2[i].assign((const cplx*)F2 + i * S, (const cplx*)F2 + (i + 1) * S);
+        else p->F2[i].assign(S, 0.0);
     }
+    return 0;
+}
+// test hook: plant a known mistake in the bulk gradient (see Problem::bulkgrad)
+int orc_set_bulkgrad_mutation(int m) {
+    Problem::bulkgrad_mutation = m;
     return 0;
 }
 int orc_last_gradient(void* h, double* g2) {

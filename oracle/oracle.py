@@ -60,7 +60,8 @@ def _declare(L):
     L.orc_nonlinear.argtypes = [C.c_void_p, _cp, _cp, _i]
     L.orc_step.argtypes = [C.c_void_p, _cp, _cp, _i]
     L.orc_last_gradient.argtypes = [C.c_void_p, _dp]
-    L.orc_set_forcing.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+    L.orc_set_bulkgrad_mutation.argtypes = [_i]
+    L.orc_set_forcing.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double]
     L.orc_project.argtypes = [C.c_void_p, _cp]
     L.orc_pressure.argtypes = [C.c_void_p, _cp, _cp]
     L.orc_pressure_poisson.argtypes = [_i, _i, _i, _d, _d, _d, _cp, _cp, _cp]
@@ -75,6 +76,12 @@ class OracleError(RuntimeError):
 def _check(rc):
     if rc != 0:
         raise OracleError(lib().orc_last_error().decode())
+
+
+def set_bulkgrad_mutation(m):
+    """Test hook: plant a known mistake in the oracle's bulk gradient
+    (0 none, 1 drop shear, 2 drop base, 3 drop B[N], 4 flip shear sign)."""
+    lib().orc_set_bulkgrad_mutation(int(m))
 
 
 def set_threads(n):
@@ -308,18 +315,20 @@ class Problem:
     def step_inplace(self, U3, P, nsteps=1):
         _check(lib().orc_step(self.h, U3, P, int(nsteps)))
 
-    def set_forcing(self, F0=None, F1=None, omega=0.0, t0=0.0):
-        """forcing(t, U, N) += F0 + sin(omega t) F1 at the stage times
-        (cnrk2.jl:173-175); F0=None removes it.  t0: time of the next step."""
+    def set_forcing(self, F0=None, F1=None, omega=0.0, t0=0.0, F2=None):
+        """forcing(t, U, N) += F0 + sin(omega t) F1 + cos(omega t) F2 at the
+        stage times (cnrk2.jl:173-175); F0=None removes it.  t0: time of the
+        next step."""
         if F0 is None:
-            _check(lib().orc_set_forcing(self.h, None, None, 0.0, float(t0)))
+            _check(lib().orc_set_forcing(self.h, None, None, None, 0.0, float(t0)))
             self._forcing = None
             return
         F0 = _c(np.asarray(F0)).copy()
         F1 = None if F1 is None else _c(np.asarray(F1)).copy()
-        self._forcing = (F0, F1)  # keep alive
+        F2 = None if F2 is None else _c(np.asarray(F2)).copy()
+        self._forcing = (F0, F1, F2)  # keep alive
         _check(lib().orc_set_forcing(self.h, F0.ctypes.data, None if F1 is None else F1.ctypes.data,
-                                     float(omega), float(t0)))
+                                     None if F2 is None else F2.ctypes.data, float(omega), float(t0)))
 
     def last_gradient(self):
         g2 = np.zeros(2)

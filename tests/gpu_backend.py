@@ -31,7 +31,7 @@ def _aux(g):
 
 
 def forward(g, u):
-    return cf.FFT(_aux(g), torch.as_tensor(u, device="cuda")).cpu().numpy()
+    return cf.FFT(_aux(g), torch.as_tensor(np.array(u), device="cuda")).cpu().numpy()
 
 
 def inverse(g, U):

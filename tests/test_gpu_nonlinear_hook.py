@@ -58,7 +58,7 @@ def test_caller_nonlinear_matches_oracle(gpu, dims, form):
         return ref.step(a, b, n)
 
     U1, P1 = ostep(U0, P0)
-    env = roundoff_envelope(lambda a, b: _split(ostep(a, b)), [U0, P0], _split((U1, P1)), trials=2)
+    env = roundoff_envelope(lambda a, b: _split(ostep(a, b)), [U0, P0], _split((U1, P1)))
     assert_parity(f"caller nonlinear {dims} {form}", _split((U.cpu().numpy(), P.cpu().numpy())), _split((U1, P1)), env)
     ref.set_forcing(None)
 
@@ -112,5 +112,5 @@ def test_caller_nonlinear_error_propagates(gpu):
     U, P = torch.as_tensor(U0, device="cuda"), torch.as_tensor(P0, device="cuda")
     cf.step_(prob.scheme, prob.nlterm, U, P, 0.0, nsteps=2, **prob.constraint)
     U1, P1 = ref.step(U0, P0, 2)
-    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, 2)), [U0, P0], _split((U1, P1)), trials=2)
+    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, 2)), [U0, P0], _split((U1, P1)))
     assert_parity("after error", _split((U.cpu().numpy(), P.cpu().numpy())), _split((U1, P1)), env)

@@ -33,7 +33,7 @@ def perturb(x, rng, eps=1e-16):
     return x * (1 + eps * rng.standard_normal(x.shape))
 
 
-def roundoff_envelope(fn, inputs, base, trials=3):
+def roundoff_envelope(fn, inputs, base, trials=8):
     """Largest relative change of each oracle output when its inputs are
     perturbed by 1e-16 relative: the problem's own FP64 sensitivity.  Two
     correct FP64 implementations cannot be expected to agree closer than this."""
@@ -205,7 +205,7 @@ def run_pair(cf, dims, nsteps, L=(2 * np.pi / 1.14, 2 * np.pi / 2.5), profile=la
     t = cf.step_(prob.scheme, prob.nlterm, U, P, 0.0, nsteps=nsteps, **prob.constraint)
     assert abs(t - nsteps * dt) < 1e-12
     U1, P1 = ref.step(U0, P0, nsteps)
-    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, nsteps)), [U0, P0], _split((U1, P1)), trials=2)
+    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, nsteps)), [U0, P0], _split((U1, P1)))
     return U.cpu().numpy(), P.cpu().numpy(), U1, P1, prob, ref, env
 
 
@@ -295,7 +295,7 @@ def test_flow_shortened_final_step(gpu, span_steps):
     U1, P1 = ref.step(U0, P0, n) if n else (U0, P0)
     if rem > 1e-10 * dt:
         U1, P1 = O.Problem(go, lambda y: y, nu, rem).step(U1, P1, 1)
-    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, max(n, 1))), [U0, P0], _split((U1, P1)), trials=2)
+    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, max(n, 1))), [U0, P0], _split((U1, P1)))
     assert_parity(f"flow {span_steps}", _split((st.velocity.cpu().numpy(), st.stagepressure.cpu().numpy())),
                   _split((U1, P1)), env)
 
