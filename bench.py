@@ -322,20 +322,71 @@ def make_problem(cf, w):
     return g, prof, kw, cf.ChannelFlowProblem(g, prof, w["nu"], w["dt"], **kw)
 
 
-def cpu_baseline(w, prof, kw, U0, P0, nsteps):
-    """Oracle (CPU restatement) on `nsteps` steps of the same workload."""
+def _timed_samples(ref, U0, P0, warmup, samples):
+    """benchmarks/step.jl:31-53: setup outside timing, the state restored
+    before every sample, `warmup` untimed steps, then `samples` timed single
+    steps; returns the per-step seconds."""
+    import numpy as np
+    secs = []
+    for i in range(warmup + samples):
+        U, P = np.array(U0, copy=True), np.array(P0, copy=True)
+        t0 = time.perf_counter()
+        ref.step_inplace(U, P, 1)
+        if i >= warmup:
+            secs.append(time.perf_counter() - t0)
+    return secs
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# the reference's published single-thread step (Julia + FFTW, Xeon Gold 6336Y):
+# plane Couette, rotational form, nu = 1/400, dt = 0.002, Lx = Lz = 2 pi,
+# grid 96x97x96 (benchmarks/results/xeon-1.csv:8)
+REF_1THREAD_96 = 0.6229
+
+
+def cpu_baseline(w, prof, kw, U0, P0, warmup, samples, samples_1t):
+    """The oracle (C++ restatement of the reference CPU step) timed on the
+    host: the same workload on all threads (min and mean over `samples`
+    steps), and single-threaded on the reference's own 96x97x96 benchmark
+    case, whose Julia + FFTW time is published, to state the restatement's
+    speed against the reference's."""
     import numpy as np
     from oracle import oracle as O
 
     Nx, Ny, Nz = w["N"]
     go = O.Grid(Nx, Ny, Nz, *w["L"])
+    threads = os.cpu_count() or 1
+    O.set_threads(min(threads, 16))
     ref = O.Problem(go, prof, w["nu"], w["dt"], **kw)
-    U, P = np.ascontiguousarray(U0), np.ascontiguousarray(P0)
-    ref.step_inplace(U, P, 1)  # warm-up (page-in, thread pool)
-    t0 = time.perf_counter()
-    ref.step_inplace(U, P, nsteps)
-    dt = time.perf_counter() - t0
-    return nsteps / dt, O.num_threads(), dt
+    secs = _timed_samples(ref, U0, P0, warmup, samples)
+    used = O.num_threads()
+    out = {"min_s": round(min(secs), 4), "mean_s": round(float(np.mean(secs)), 4), "threads": used,
+           "warmup": warmup, "samples": samples}
+    one = None
+    if samples_1t > 0:
+        O.set_threads(1)
+        g96 = O.Grid(96, 97, 96, 2 * math.pi, 2 * math.pi)
+        r96 = O.Problem(g96, lambda y: y, 1 / 400, 0.002)
+        rng = np.random.default_rng(0)
+        U96, P96 = r96.random_state(0.1 * rng.standard_normal((3,) + g96.phys_shape))
+        s1 = _timed_samples(r96, U96, P96, 1, samples_1t)
+        one = {"case": "plane Couette 96x97x96, Lx=Lz=2pi, nu=1/400, dt=0.002, rotational (the reference's "
+                       "benchmark case)",
+               "threads": 1, "warmup": 1, "samples": samples_1t, "min_s": round(min(s1), 4),
+               "mean_s": round(float(np.mean(s1)), 4), "reference_julia_fftw_1thread_s": REF_1THREAD_96,
+               "oracle_over_reference": round(min(s1) / REF_1THREAD_96, 2)}
+        O.set_threads(used)
+    return out, one
 
 
 def main():
@@ -343,7 +394,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="timed oracle samples, all threads (0: skip)")
+    ap.add_argument("--cpu-warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps-1thread", type=int, default=3,
+                    help="timed single-thread oracle samples on the reference's 96x97x96 case")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--workload", default="poiseuille128", choices=sorted(WORKLOADS))
     ap.add_argument("--mixed-steps", type=int, default=500,
@@ -427,10 +481,13 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        cps, cores, secs = cpu_baseline(w, prof, kw, U0, P0, args.cpu_steps)
-        cpu = {"value": round(cps, 5), "unit": "steps/s", "cores": cores, "kind": "port",
-               "sample": f"{args.cpu_steps} oracle steps of the same workload after 1 warm-up step "
-                         f"({secs:.1f} s, {cores} OpenMP threads)"}
+        allc, one = cpu_baseline(w, prof, kw, U0, P0, args.cpu_warmup, args.cpu_steps, args.cpu_steps_1thread)
+        cpu = {"value": round(1.0 / allc["min_s"], 5), "unit": "steps/s", "cores": allc["threads"], "kind": "port",
+               "sample": f"the oracle on the same workload: {allc['warmup']} warm-up steps, then "
+                         f"{allc['samples']} single steps each from the restored initial state "
+                         f"(benchmarks/step.jl:31-53 protocol); value = 1/min, {allc['threads']} OpenMP threads",
+               "min_s": allc["min_s"], "mean_s": allc["mean_s"], "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+               "gpu_over_cpu": round(steps_per_s * allc["min_s"], 1), "one_thread": one}
 
     forms = {}
     del prob, state, U, P

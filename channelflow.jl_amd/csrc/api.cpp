@@ -172,21 +172,47 @@ static int make_bank(cf_problem* h, double nu, double lambda, DevBank& db) {
     return 0;
 }
 
+// Banks beyond the three stage banks (shortened final steps of cf_advance_to,
+// cf_stokes_solve at other (nu, lambda)) are cached most-recent first, at
+// most EXTRA_BANKS of them: a caller that varies the span continuously
+// (Newton or periodic-orbit searches) keeps a bounded footprint.
+static const size_t EXTRA_BANKS = 6;
+
+static void free_bank(cf_problem* h, DevBank& db) {
+    for (void* p : {(void*)db.facV, (void*)db.mconst, (void*)db.resp}) {
+        auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+        if (it == h->allocs.end()) continue;
+        (void)hipFree(p);
+        h->allocs.erase(it);
+    }
+    db = DevBank{};
+}
+
 static int get_bank(cf_problem* h, double nu, double lambda, DevBank** out) {
     for (int j = 0; j < 3; ++j)
         if (h->stage[j].nu == nu && h->stage[j].lambda == lambda) {
             *out = &h->stage[j];
             return 0;
         }
-    for (auto& e : h->extra)
-        if (e.first.first == nu && e.first.second == lambda) {
-            *out = &e.second;
+    for (auto it = h->extra.begin(); it != h->extra.end(); ++it)
+        if (it->first.first == nu && it->first.second == lambda) {
+            h->extra.splice(h->extra.begin(), h->extra, it);  // list nodes keep their addresses
+            *out = &h->extra.front().second;
             return 0;
         }
+    if (h->extra.size() >= EXTRA_BANKS) {
+        // work enqueued earlier, on any stream, may still read the banks
+        // about to be freed
+        HIPTRY(hipDeviceSynchronize());
+        while (h->extra.size() >= EXTRA_BANKS) {
+            free_bank(h, h->extra.back().second);
+            h->extra.pop_back();
+        }
+    }
     DevBank db;
     if (int rc = make_bank(h, nu, lambda, db)) return rc;
-    h->extra.push_back({{nu, lambda}, db});
-    *out = &h->extra.back().second;
+    h->extra.push_front({{nu, lambda}, db});
+    *out = &h->extra.front().second;
     return 0;
 }
 

@@ -2,11 +2,14 @@
 // (src/initialization.jl:32-51): epsilon * N(0, 1) samples for u, v, w.
 //
 // Stream: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
-// numbers: as easy as 1, 2, 3", SC 2011), key = (seed low, seed high),
-// counter = (element index / 2, component, 0, 0); each counter block gives
-// two uniform pairs, turned into two normals by the Box-Muller transform.
-// Element index = x + Nxp (z + Nzp y) of a component, the reference's
-// evaluation order (physicalfield.jl:39-43).
+// numbers: as easy as 1, 2, 3", SC 2011), key = (seed bits 0-31, seed bits
+// 32-63).  Counter words, for the pair q = element index / 2 of component c:
+//   (q bits 0-31, q bits 32-63, c, 0).
+// One counter block gives 128 bits: words 0,1 -> u1 = ((w0 << 21) ^ (w1 >> 11)
+// masked to 53 bits, + 1) / 2^53 in (0, 1], words 2,3 -> u2 the same way;
+// Box-Muller turns them into the normals of elements 2q (cos) and 2q + 1
+// (sin), each times epsilon.  Element index = x + Nxp (z + Nzp y) of a
+// component, the reference's evaluation order (physicalfield.jl:39-43).
 #include "cf_device.h"
 #include "cf_kernels.h"
 

@@ -1,9 +1,23 @@
 // ORACLE — TEST INFRASTRUCTURE ONLY.
 //
-// CPU restatement (C++17, FP64) of ChannelFlow.jl's CPU/FFTW CNRK2 step path.
+// CPU restatement (C++17) of ChannelFlow.jl's CPU/FFTW CNRK2 step path.
 // It is the checker for the HIP implementation: only tests/, the smoke()
 // entry point and bench.py's cpu_baseline leg may load it.  Nothing in the
 // product library links or calls this file.
+//
+// Two builds of this one source:
+//   libcforacle.so    real = double       the FP64 restatement (the oracle)
+//   libcforacle_x.so  real = long double  (-DORC_EXTENDED) the same algorithm
+//                     with every operation and every derived constant
+//                     (wavenumbers, lambda, CNRK2 coefficients, tau
+//                     coefficients, FFT twiddles) in x87 extended precision
+//                     (64-bit significand).  It is the accuracy reference
+//                     for FP64 results of ill-conditioned cases: both the
+//                     oracle and the device are judged by their distance
+//                     from it.
+// The C ABI takes and returns FP64 arrays in both builds; the extended build
+// converts them on entry and rounds on exit, so a multi-step call keeps the
+// state in extended precision between steps.
 //
 // Every function cites the reference file:line it restates.  Where the
 // reference calls the un-vendored ChebyshevHelmoltzSolvers.jl (CHS v0.1.0,
@@ -21,7 +35,11 @@
 //   influence/Stokes  test_helmoltz.jl / test_batchedinfluence.jl /
 //                     test_stokes.jl manufactured solutions + dense system
 //   step              validation/results/decay.csv, waleffe.csv, ts_*.csv
+//   bulk gradient     laminar Poiseuille G = -3 nu and a manufactured forced
+//                     (0,0)-mode solution (second order in dt)
 //   diagnostics       test_postprocessing.jl closed forms
+//   extended build    same checks, plus its Helmholtz/Stokes solves against
+//                     exact long-double solutions far below FP64 roundoff
 //
 // Array conventions (reference layouts, column-major):
 //   spectral (k,l,n): index k + Nxh*(l + Nz*n), interleaved complex128
@@ -35,13 +53,23 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 
-typedef std::complex<double> cplx;
+#ifdef ORC_EXTENDED
+typedef long double real;
+#else
+typedef double real;
+#endif
+typedef std::complex<real> cplx;
 static const long double PI_L = 3.141592653589793238462643383279502884L;
+
+// T deduced from the arrays only; scalar arguments convert to it
+template <typename T>
+struct ident { typedef T type; };
 
 static thread_local std::string g_error;
 
@@ -63,7 +91,7 @@ struct FFTPlan {
         tw.resize(n);
         for (int j = 0; j < n; ++j) {
             long double a = -2.0L * PI_L * (long double)j / (long double)n;
-            tw[j] = cplx((double)cosl(a), (double)sinl(a));
+            tw[j] = cplx((real)cosl(a), (real)sinl(a));
         }
     }
     // out[k] = sum_j in[j*stride] * exp(sign*2*pi*i*j*k/n)
@@ -123,7 +151,7 @@ struct PlanCache {
 
 // Real forward transform (FFTW rfft convention, unnormalised, e^{-i}):
 // X[k] = sum_j x[j] e^{-2 pi i jk/N}, k = 0..N/2.
-static void rfft_line(const FFTPlan& half, const FFTPlan& full, int N, const double* x, cplx* X,
+static void rfft_line(const FFTPlan& half, const FFTPlan& full, int N, const real* x, cplx* X,
                       std::vector<cplx>& a, std::vector<cplx>& b) {
     if (N % 2 == 0 && N >= 2) {
         int M = N / 2;
@@ -133,7 +161,7 @@ static void rfft_line(const FFTPlan& half, const FFTPlan& full, int N, const dou
         for (int k = 0; k <= M; ++k) {
             cplx Zk = b[k % M], Zc = std::conj(b[(M - k) % M]);
             cplx w = full.tw[k % N];
-            X[k] = 0.5 * (Zk + Zc) - 0.5 * cplx(0, 1) * w * (Zk - Zc);
+            X[k] = (real)0.5 * (Zk + Zc) - (real)0.5 * cplx(0, 1) * w * (Zk - Zc);
         }
     } else {
         a.resize(N); b.resize(N);
@@ -146,7 +174,7 @@ static void rfft_line(const FFTPlan& half, const FFTPlan& full, int N, const dou
 // Real backward transform (FFTW brfft convention, unnormalised, e^{+i}):
 // x[j] = sum_{k=0}^{N-1} C[k] e^{+2 pi i jk/N}, C Hermitian-extended from
 // X[0..N/2]; imaginary parts of DC and (even N) Nyquist are ignored.
-static void brfft_line(const FFTPlan& half, const FFTPlan& full, int N, const cplx* Xin, double* x,
+static void brfft_line(const FFTPlan& half, const FFTPlan& full, int N, const cplx* Xin, real* x,
                        std::vector<cplx>& a, std::vector<cplx>& b) {
     if (N % 2 == 0 && N >= 2) {
         int M = N / 2;
@@ -188,9 +216,9 @@ static void redft00_line(const FFTPlan& ext, int P, const cplx* x, cplx* X, std:
 // ---------------------------------------------------------------------------
 struct Grid {
     int Nx, Ny, Nz;
-    double Lx, Lz;
+    real Lx, Lz;
     int Nxh, Nxp, Nzp;
-    Grid(int nx, int ny, int nz, double lx, double lz) : Nx(nx), Ny(ny), Nz(nz), Lx(lx), Lz(lz) {
+    Grid(int nx, int ny, int nz, real lx, real lz) : Nx(nx), Ny(ny), Nz(nz), Lx(lx), Lz(lz) {
         Nxp = (3 * Nx + 1) / 2;  // cld(3n, 2), grids.jl:63
         Nzp = (3 * Nz + 1) / 2;
         Nxh = Nx / 2 + 1;        // spectralsize, grids.jl:71-74
@@ -199,17 +227,11 @@ struct Grid {
     size_t B() const { return (size_t)Nxh * Nz; }
     size_t phys() const { return (size_t)Nxp * Nzp * Ny; }
     int lmode(int iz) const { return iz <= (Nz >> 1) ? iz : iz - Nz; }  // indexing.jl:16-17
-    double alpha() const { return 2.0 * M_PI / Lx; }
-    double beta() const { return 2.0 * M_PI / Lz; }
+    real alpha() const { return (real)2.0 * (real)PI_L / Lx; }
+    real beta() const { return (real)2.0 * (real)PI_L / Lz; }
     bool xnyq(int ix) const { return (Nx % 2 == 0) && ix == Nx / 2; }
     bool znyq(int iz) const { return (Nz % 2 == 0) && iz == Nz / 2; }
 };
-
-// Lobatto nodes, upper wall first (grids.jl:28)
-static double lobatto(int j, int Ny) {
-    long double a = PI_L * (long double)j / (long double)(Ny - 1);
-    return (double)cosl(a);
-}
 
 // ---------------------------------------------------------------------------
 // Transforms (src/ffts.jl, src/transforms/chebyshev.jl)
@@ -247,7 +269,7 @@ struct Transforms {
 
     // ForwardFFT! (ffts.jl:65-82): rfft x, fft z, truncate, DCT-I,
     // normalize_forward! (ffts.jl:254-269).
-    void forward(const double* u, cplx* U) {
+    void forward(const real* u, cplx* U) {
         const Grid& G = g;
         int nkp_ = nkp();
         // rfft along x for every (z, y) line
@@ -277,13 +299,13 @@ struct Transforms {
                 }
         }
         cheb(resolved.data(), U, false, G.B());
-        normalize_forward(U, 1.0 / ((double)G.Nxp * G.Nzp * (G.Ny - 1)));
+        normalize_forward(U, (real)1.0 / ((real)G.Nxp * G.Nzp * (G.Ny - 1)));
     }
 
-    void normalize_forward(cplx* U, double normalization) {
+    void normalize_forward(cplx* U, real normalization) {
         const Grid& G = g;
         for (int iy = 0; iy < G.Ny; ++iy) {
-            double endpoint = (iy == 0 || iy == G.Ny - 1) ? 0.5 : 1.0;
+            real endpoint = (iy == 0 || iy == G.Ny - 1) ? 0.5 : 1.0;
             for (int iz = 0; iz < G.Nz; ++iz)
                 for (int ix = 0; ix < G.Nxh; ++ix) {
                     size_t i = ix + (size_t)G.Nxh * (iz + (size_t)G.Nz * iy);
@@ -294,7 +316,7 @@ struct Transforms {
 
     // InverseFFT! (ffts.jl:151-166): icheb (endpoint x2), embed x0.5
     // (copy_to_padded!, 242-251), zero_nyquist! (306-312), brfft.
-    void inverse(const cplx* U, double* u) {
+    void inverse(const cplx* U, real* u) {
         const Grid& G = g;
         int nkp_ = nkp();
         std::vector<cplx> resolved(G.S());
@@ -310,9 +332,9 @@ struct Transforms {
                     std::fill(a.begin(), a.end(), cplx(0.0));
                     if (!G.xnyq((int)k)) {
                         for (int l = 0; l < npos; ++l)
-                            if (!G.znyq(l)) a[l] = 0.5 * resolved[k + G.Nxh * (l + (size_t)G.Nz * y)];
+                            if (!G.znyq(l)) a[l] = (real)0.5 * resolved[k + G.Nxh * (l + (size_t)G.Nz * y)];
                         for (int l = 0; l < nneg; ++l)
-                            a[G.Nzp - nneg + l] = 0.5 * resolved[k + G.Nxh * (npos + l + (size_t)G.Nz * y)];
+                            a[G.Nzp - nneg + l] = (real)0.5 * resolved[k + G.Nxh * (npos + l + (size_t)G.Nz * y)];
                     }
                     zplan->exec(a.data(), b.data(), +1);
                     for (int z = 0; z < G.Nzp; ++z) padded[(size_t)k + nkp_ * (z + (size_t)G.Nzp * y)] = b[z];
@@ -333,24 +355,24 @@ struct Transforms {
 // ---------------------------------------------------------------------------
 // ddx1! operators.jl:14-23
 static void ddx1(const Grid& g, cplx* out, const cplx* U, bool add) {
-    double a = g.alpha();
+    real a = g.alpha();
     for (int n = 0; n < g.Ny; ++n)
         for (int iz = 0; iz < g.Nz; ++iz)
             for (int ix = 0; ix < g.Nxh; ++ix) {
                 size_t i = ix + (size_t)g.Nxh * (iz + (size_t)g.Nz * n);
-                cplx v = cplx(0, 1) * (double)ix * a * U[i];
+                cplx v = cplx(0, 1) * (real)ix * a * U[i];
                 out[i] = add ? out[i] + v : v;
             }
 }
 // ddx3! operators.jl:74-83
 static void ddx3(const Grid& g, cplx* out, const cplx* U, bool add) {
-    double b = g.beta();
+    real b = g.beta();
     for (int n = 0; n < g.Ny; ++n)
         for (int iz = 0; iz < g.Nz; ++iz) {
             int l = g.lmode(iz);
             for (int ix = 0; ix < g.Nxh; ++ix) {
                 size_t i = ix + (size_t)g.Nxh * (iz + (size_t)g.Nz * n);
-                cplx v = cplx(0, 1) * (double)l * b * U[i];
+                cplx v = cplx(0, 1) * (real)l * b * U[i];
                 out[i] = add ? out[i] + v : v;
             }
         }
@@ -361,9 +383,9 @@ template <typename T>
 static void batchdiff(T* out, const T* a, size_t B, int Ny) {
     for (size_t s = 0; s < B; ++s) out[s + B * (Ny - 1)] = T(0.0);
     if (Ny >= 2)
-        for (size_t s = 0; s < B; ++s) out[s + B * (Ny - 2)] = 2.0 * (Ny - 1) * a[s + B * (Ny - 1)];
+        for (size_t s = 0; s < B; ++s) out[s + B * (Ny - 2)] = (real)(2.0 * (Ny - 1)) * a[s + B * (Ny - 1)];
     for (int j = Ny - 3; j >= 0; --j)  // 0-based j
-        for (size_t s = 0; s < B; ++s) out[s + B * j] = out[s + B * (j + 2)] + 2.0 * (j + 1) * a[s + B * (j + 1)];
+        for (size_t s = 0; s < B; ++s) out[s + B * j] = out[s + B * (j + 2)] + (real)(2.0 * (j + 1)) * a[s + B * (j + 1)];
     for (size_t s = 0; s < B; ++s) out[s] *= 0.5;
 }
 static void ddx2(const Grid& g, cplx* out, const cplx* U, bool add) {
@@ -375,7 +397,7 @@ static void ddx2(const Grid& g, cplx* out, const cplx* U, bool add) {
 }
 // laplacian! operators.jl:96-125 (non-aliased fused recurrence)
 static void laplacian(const Grid& g, cplx* b, const cplx* a) {
-    double a2 = g.alpha() * g.alpha(), b2 = g.beta() * g.beta();
+    real a2 = g.alpha() * g.alpha(), b2 = g.beta() * g.beta();
     int Ny = g.Ny;
     size_t B = g.B();
     for (int j = Ny; j >= 1; --j)  // 1-based as in the reference
@@ -383,14 +405,14 @@ static void laplacian(const Grid& g, cplx* b, const cplx* a) {
             int l = g.lmode(iz);
             for (int ix = 0; ix < g.Nxh; ++ix) {
                 size_t s = ix + (size_t)g.Nxh * iz;
-                double k2 = a2 * ix * ix + b2 * (double)l * l;
+                real k2 = a2 * ix * ix + b2 * (real)l * l;
                 cplx d2 = 0.0;
                 auto A = [&](int jj) { return a[s + B * (jj - 1)]; };
                 auto Bv = [&](int jj) { return b[s + B * (jj - 1)]; };
                 if (j + 2 <= Ny)
-                    d2 = 4.0 * j * (j + 1) * A(j + 2) + (2.0 * (j + 1) / (j + 2)) * (Bv(j + 2) + k2 * A(j + 2));
-                if (j + 4 <= Ny) d2 -= ((double)j / (j + 2)) * (Bv(j + 4) + k2 * A(j + 4));
-                b[s + B * (j - 1)] = (j == 1 ? d2 / 2.0 : d2) - k2 * A(j);
+                    d2 = (real)(4.0 * j * (j + 1)) * A(j + 2) + ((real)2.0 * (j + 1) / (j + 2)) * (Bv(j + 2) + k2 * A(j + 2));
+                if (j + 4 <= Ny) d2 -= ((real)j / (j + 2)) * (Bv(j + 4) + k2 * A(j + 4));
+                b[s + B * (j - 1)] = (j == 1 ? d2 / (real)2.0 : d2) - k2 * A(j);
             }
         }
 }
@@ -398,16 +420,16 @@ static void laplacian(const Grid& g, cplx* b, const cplx* a) {
 static void curl(const Grid& g, cplx* const O[3], const cplx* const U[3]) {
     ddx2(g, O[0], U[2], false);
     ddx2(g, O[2], U[0], false);
-    double al = g.alpha(), be = g.beta();
+    real al = g.alpha(), be = g.beta();
     const cplx I(0, 1);
     for (int iy = 0; iy < g.Ny; ++iy)
         for (int iz = 0; iz < g.Nz; ++iz) {
             int l = g.lmode(iz);
             for (int ix = 0; ix < g.Nxh; ++ix) {
                 size_t i = ix + (size_t)g.Nxh * (iz + (size_t)g.Nz * iy);
-                O[0][i] -= I * (double)l * be * U[1][i];
-                O[1][i] = I * ((double)l * be * U[0][i] - (double)ix * al * U[2][i]);
-                O[2][i] = I * (double)ix * al * U[1][i] - O[2][i];
+                O[0][i] -= I * (real)l * be * U[1][i];
+                O[1][i] = I * ((real)l * be * U[0][i] - (real)ix * al * U[2][i]);
+                O[2][i] = I * (real)ix * al * U[1][i] - O[2][i];
             }
         }
 }
@@ -417,13 +439,13 @@ static void curl(const Grid& g, cplx* const O[3], const cplx* const U[3]) {
 // derivative functionals) and _bulkmean (meanmode.jl:13-19).
 // ---------------------------------------------------------------------------
 // chebcoeffs of Lobatto values: DCT-I / P with halved endpoints (test/helpers.jl:5-11)
-static void chebcoeffs(int Ny, const double* values, double* coeffs) {
+static void chebcoeffs(int Ny, const real* values, real* coeffs) {
     int P = Ny - 1;
-    std::vector<double> out(Ny);
+    std::vector<real> out(Ny);
     for (int k = 0; k <= P; ++k) {
         long double acc = values[0] + ((k % 2) ? -values[P] : values[P]);
         for (int j = 1; j < P; ++j) acc += 2.0L * values[j] * cosl(PI_L * (long double)j * k / P);
-        out[k] = (double)(acc / P);
+        out[k] = (real)(acc / P);
     }
     out[0] *= 0.5; out[P] *= 0.5;
     std::copy(out.begin(), out.end(), coeffs);
@@ -433,19 +455,19 @@ static void diff1(T* out, const T* a, int Ny) { batchdiff(out, a, 1, Ny); }
 template <typename T>
 static T diff_right(const T* a, int Ny) {  // phi'(+1) = sum n^2 a_n
     T acc = T(0.0);
-    for (int n = 0; n < Ny; ++n) acc += (double)n * n * a[n];
+    for (int n = 0; n < Ny; ++n) acc += (real)n * n * a[n];
     return acc;
 }
 template <typename T>
 static T diff_left(const T* a, int Ny) {  // phi'(-1) = sum (-1)^(n+1) n^2 a_n
     T acc = T(0.0);
-    for (int n = 0; n < Ny; ++n) acc += ((n % 2) ? 1.0 : -1.0) * (double)n * n * a[n];
+    for (int n = 0; n < Ny; ++n) acc += (real)((n % 2) ? 1.0 : -1.0) * (real)n * n * a[n];
     return acc;
 }
 template <typename T>
 static T bulkmean(const T* a, int Ny) {  // _bulkmean, meanmode.jl:13-19
     T v = a[0];
-    for (int n = 2; n <= Ny - 1; n += 2) v += a[n] / (1.0 - (double)n * n);
+    for (int n = 2; n <= Ny - 1; n += 2) v += a[n] / (real)(1.0 - (double)n * n);
     return v;
 }
 
@@ -458,34 +480,34 @@ static T bulkmean(const T* a, int Ny) {  // _bulkmean, meanmode.jl:13-19
 // ---------------------------------------------------------------------------
 struct Helmholtz {
     int P = 0;
-    double t0 = 1, t1 = 0;
+    real t0 = 1, t1 = 0;
     bool neumann = false;
     // per parity p (0 even, 1 odd): unknowns x_j = a_{2j+p}, j = 0..J_p
     int J[2];
-    std::vector<double> lo[2], up[2], s[2], alpha[2], gam[2], w[2];
-    double sumwg[2];
+    std::vector<real> lo[2], up[2], s[2], alpha[2], gam[2], w[2];
+    real sumwg[2];
 
-    static double cfac(int m) { return m == 0 ? 2.0 : 1.0; }
+    static real cfac(int m) { return m == 0 ? 2.0 : 1.0; }
     Helmholtz() {}
-    Helmholtz(int P_, double t0_, double t1_, bool neu) { setup(P_, t0_, t1_, neu); }
-    void setup(int P_, double t0_, double t1_, bool neu) {
+    Helmholtz(int P_, real t0_, real t1_, bool neu) { setup(P_, t0_, t1_, neu); }
+    void setup(int P_, real t0_, real t1_, bool neu) {
         P = P_; t0 = t0_; t1 = t1_; neumann = neu;
         for (int p = 0; p < 2; ++p) {
             int Jp = (P - p) / 2;  // largest j with 2j+p <= P
             J[p] = Jp;
             lo[p].assign(Jp + 1, 0.0); up[p].assign(Jp + 1, 0.0); s[p].assign(Jp + 1, 0.0);
             alpha[p].assign(Jp + 2, 0.0); gam[p].assign(Jp + 1, 0.0); w[p].assign(Jp + 1, 0.0);
-            std::vector<double> d(Jp + 1, 0.0);
+            std::vector<real> d(Jp + 1, 0.0);
             for (int j = 1; j <= Jp; ++j) {
                 int n = 2 * j + p;
-                double en = (n <= P - 2) ? 1.0 : 0.0, en2 = (n + 2 <= P - 2) ? 1.0 : 0.0;
-                lo[p][j] = -t1 * cfac(n - 2) / (4.0 * n * (n - 1));
-                d[j] = t0 + t1 * en / (2.0 * ((double)n * n - 1.0));
-                up[p][j] = -t1 * en2 / (4.0 * n * (n + 1));
+                real en = (n <= P - 2) ? 1.0 : 0.0, en2 = (n + 2 <= P - 2) ? 1.0 : 0.0;
+                lo[p][j] = -t1 * cfac(n - 2) / (real)(4.0 * n * (n - 1));
+                d[j] = t0 + t1 * en / (real)(2.0 * ((double)n * n - 1.0));
+                up[p][j] = -t1 * en2 / (real)(4.0 * n * (n + 1));
             }
             alpha[p][Jp + 1] = 0.0;
             for (int j = Jp; j >= 1; --j) {
-                s[p][j] = 1.0 / (d[j] + up[p][j] * alpha[p][j + 1]);
+                s[p][j] = (real)1.0 / (d[j] + up[p][j] * alpha[p][j + 1]);
                 alpha[p][j] = -lo[p][j] * s[p][j];
             }
             gam[p][0] = 1.0;
@@ -493,27 +515,27 @@ struct Helmholtz {
             sumwg[p] = 0.0;
             for (int j = 0; j <= Jp; ++j) {
                 int n = 2 * j + p;
-                w[p][j] = neumann ? (double)n * n : 1.0;
+                w[p][j] = neumann ? (real)n * n : (real)1.0;
                 sumwg[p] += w[p][j] * gam[p][j];
             }
         }
     }
-    // Solve; rhs/out length P+1, T = double or cplx.  gp, gm: wall data at
+    // Solve; rhs/out length P+1, T = real or cplx.  gp, gm: wall data at
     // y=+1, y=-1 (values, or derivatives for Neumann).
     template <typename T>
-    void solve(T* out, const T* f, T gp, T gm) const {
+    void solve(T* out, const T* f, typename ident<T>::type gp, typename ident<T>::type gm) const {
         std::vector<T> rhs(P + 1), beta(P + 2), delta(P + 1);
         auto F = [&](int m) -> T { return (m >= 0 && m <= P - 2) ? f[m] : T(0.0); };
         for (int p = 0; p < 2; ++p) {
             int Jp = J[p];
             // boundary datum for this parity
             T g;
-            if (!neumann) g = (p == 0) ? 0.5 * (gp + gm) : 0.5 * (gp - gm);
-            else g = (p == 0) ? 0.5 * (gp - gm) : 0.5 * (gp + gm);
+            if (!neumann) g = (p == 0) ? (real)0.5 * (gp + gm) : (real)0.5 * (gp - gm);
+            else g = (p == 0) ? (real)0.5 * (gp - gm) : (real)0.5 * (gp + gm);
             for (int j = 1; j <= Jp; ++j) {
                 int n = 2 * j + p;
-                rhs[j] = cfac(n - 2) * F(n - 2) / (4.0 * n * (n - 1)) - F(n) / (2.0 * ((double)n * n - 1.0)) +
-                         F(n + 2) / (4.0 * n * (n + 1));
+                rhs[j] = cfac(n - 2) * F(n - 2) / (real)(4.0 * n * (n - 1)) - F(n) / (real)(2.0 * ((double)n * n - 1.0)) +
+                         F(n + 2) / (real)(4.0 * n * (n + 1));
             }
             beta[Jp + 1] = T(0.0);
             for (int j = Jp; j >= 1; --j) beta[j] = s[p][j] * (rhs[j] - up[p][j] * beta[j + 1]);
@@ -533,18 +555,18 @@ struct Helmholtz {
 // ---------------------------------------------------------------------------
 struct InfluenceMode {
     int Ny, P;
-    double k, l, nu, lambda, shift;
+    real k, l, nu, lambda, shift;
     Helmholtz pres, vel;
-    std::vector<double> pp, pm, vp, vm, p0, v0;
-    double Ainv[4];
-    double sigma0[2];
+    std::vector<real> pp, pm, vp, vm, p0, v0;
+    real Ainv[4];
+    real sigma0[2];
     bool ok = true;
-    InfluenceMode(int Ny_, double k_, double l_, double nu_, double lam_) : Ny(Ny_), P(Ny_ - 1), k(k_), l(l_), nu(nu_), lambda(lam_) {
-        double kappa2 = k * k + l * l;
+    InfluenceMode(int Ny_, real k_, real l_, real nu_, real lam_) : Ny(Ny_), P(Ny_ - 1), k(k_), l(l_), nu(nu_), lambda(lam_) {
+        real kappa2 = k * k + l * l;
         shift = lambda + nu * kappa2;
         pres.setup(P, 1.0, kappa2, false);
         vel.setup(P, nu, shift, false);
-        std::vector<double> work(Ny, 0.0), zero(Ny, 0.0);
+        std::vector<real> work(Ny, 0.0), zero(Ny, 0.0);
         pp.assign(Ny, 0); pm.assign(Ny, 0); vp.assign(Ny, 0); vm.assign(Ny, 0); p0.assign(Ny, 0); v0.assign(Ny, 0);
         pres.solve(pp.data(), zero.data(), 1.0, 0.0);
         diff1(work.data(), pp.data(), Ny);
@@ -552,17 +574,17 @@ struct InfluenceMode {
         pres.solve(pm.data(), zero.data(), 0.0, 1.0);
         diff1(work.data(), pm.data(), Ny);
         vel.solve(vm.data(), work.data(), 0.0, 0.0);
-        double A = diff_right(vp.data(), Ny), B = diff_right(vm.data(), Ny);
-        double C = diff_left(vp.data(), Ny), D = diff_left(vm.data(), Ny);
-        double scale = std::max(std::max(std::fabs(A), std::fabs(B)), std::max(std::fabs(C), std::fabs(D)));
+        real A = diff_right(vp.data(), Ny), B = diff_right(vm.data(), Ny);
+        real C = diff_left(vp.data(), Ny), D = diff_left(vm.data(), Ny);
+        real scale = std::max(std::max(std::fabs(A), std::fabs(B)), std::max(std::fabs(C), std::fabs(D)));
         if (!(std::isfinite(scale) && scale > 0)) ok = false;
         A /= scale; B /= scale; C /= scale; D /= scale;
-        double det = A * D - B * C;
+        real det = A * D - B * C;
         if (!(std::fabs(det) > 2.220446049250313e-16 * std::max(std::fabs(A * D), std::fabs(B * C)))) ok = false;
         Ainv[0] = D / det / scale; Ainv[1] = -B / det / scale; Ainv[2] = -C / det / scale; Ainv[3] = A / det / scale;
         std::fill(work.begin(), work.end(), 0.0);
         work[P - 1] = work[P] = 1.0;
-        std::vector<double> dw(Ny);
+        std::vector<real> dw(Ny);
         diff1(dw.data(), work.data(), Ny);
         pres.solve(p0.data(), dw.data(), 0.0, 0.0);
         diff1(work.data(), p0.data(), Ny);
@@ -573,31 +595,31 @@ struct InfluenceMode {
         sigma0[0] = shift * v0[P - 1] + work[P - 1];
         sigma0[1] = shift * v0[P];
     }
-    void correct(double* p, double* v) const {  // _influence_correction! influence.jl:161-177
-        double bp = -diff_right(v, Ny), bm = -diff_left(v, Ny);
-        double dp = Ainv[0] * bp + Ainv[1] * bm, dm = Ainv[2] * bp + Ainv[3] * bm;
+    void correct(real* p, real* v) const {  // _influence_correction! influence.jl:161-177
+        real bp = -diff_right(v, Ny), bm = -diff_left(v, Ny);
+        real dp = Ainv[0] * bp + Ainv[1] * bm, dm = Ainv[2] * bp + Ainv[3] * bm;
         for (int n = 0; n < Ny; ++n) {
             p[n] += dp * pp[n] + dm * pm[n];
             v[n] += dp * vp[n] + dm * vm[n];
         }
     }
-    void solve_pv(double* p, double* v, const double* r, const double* Ry) const {  // influence.jl:282-316
-        std::vector<double> work(Ny);
+    void solve_pv(real* p, real* v, const real* r, const real* Ry) const {  // influence.jl:282-316
+        std::vector<real> work(Ny);
         pres.solve(p, r, 0.0, 0.0);
         diff1(work.data(), p, Ny);
         for (int n = 0; n < Ny; ++n) work[n] -= Ry[n];
         vel.solve(v, work.data(), 0.0, 0.0);
         correct(p, v);
         diff1(work.data(), p, Ny);
-        double sNm1 = (shift * v[P - 1] + work[P - 1] - Ry[P - 1]) / (1.0 - sigma0[0]);
-        double sN = (shift * v[P] + work[P] - Ry[P]) / (1.0 - sigma0[1]);
+        real sNm1 = (shift * v[P - 1] + work[P - 1] - Ry[P - 1]) / ((real)1.0 - sigma0[0]);
+        real sN = (shift * v[P] + work[P] - Ry[P]) / ((real)1.0 - sigma0[1]);
         for (int n = 0; n <= P; ++n) {
             p[n] += ((n % 2 == 0) ? sNm1 : sN) * p0[n];
             v[n] += ((n % 2 == 0) ? sN : sNm1) * v0[n];
         }
     }
     void solve(cplx* u, cplx* v, cplx* w, cplx* p, const cplx* Rx, const cplx* Ry, const cplx* Rz) const {  // 205-253
-        std::vector<double> pc(Ny), vc(Ny), r(Ny), ry(Ny);
+        std::vector<real> pc(Ny), vc(Ny), r(Ny), ry(Ny);
         for (int im = 0; im < 2; ++im) {
             for (int n = 0; n < Ny; ++n) ry[n] = im ? Ry[n].imag() : Ry[n].real();
             diff1(r.data(), ry.data(), Ny);
@@ -612,7 +634,7 @@ struct InfluenceMode {
         for (int c = 0; c < 2; ++c) {
             cplx* out = c == 0 ? u : w;
             const cplx* src = c == 0 ? Rx : Rz;
-            double kk = c == 0 ? k : l;
+            real kk = c == 0 ? k : l;
             for (int n = 0; n < Ny; ++n) {
                 pc[n] = -kk * p[n].imag() - src[n].real();
                 vc[n] = kk * p[n].real() - src[n].imag();
@@ -631,16 +653,16 @@ struct InfluenceMode {
 struct BatchedInfluence {
     int Ny, P;
     size_t B;
-    double nu, lambda;
-    std::vector<double> k, l, shift;
+    real nu, lambda;
+    std::vector<real> k, l, shift;
     std::vector<Helmholtz> ph, vh;              // per system
-    std::vector<double> pp, pm, vp, vm, p0, v0;  // (B, Ny)
-    std::vector<double> ia, ib, ic, id;         // influence inverse, row-major
-    std::vector<double> sig1, sig2;
+    std::vector<real> pp, pm, vp, vm, p0, v0;  // (B, Ny)
+    std::vector<real> ia, ib, ic, id;         // influence inverse, row-major
+    std::vector<real> sig1, sig2;
     bool ok = true;
     std::string err;
 
-    BatchedInfluence(int Ny_, const std::vector<double>& k_, const std::vector<double>& l_, double nu_, double lam_)
+    BatchedInfluence(int Ny_, const std::vector<real>& k_, const std::vector<real>& l_, real nu_, real lam_)
         : Ny(Ny_), P(Ny_ - 1), B(k_.size()), nu(nu_), lambda(lam_), k(k_), l(l_) {
         shift.resize(B);
         ph.resize(B); vh.resize(B);
@@ -650,12 +672,12 @@ struct BatchedInfluence {
         std::vector<int> bad(B, 0);
 #pragma omp parallel
         {
-            std::vector<double> c(Ny), work(Ny), zero(Ny, 0.0), dw(Ny), a(Ny), b(Ny), x(Ny), y(Ny), q0(Ny), r0(Ny);
+            std::vector<real> c(Ny), work(Ny), zero(Ny, 0.0), dw(Ny), a(Ny), b(Ny), x(Ny), y(Ny), q0(Ny), r0(Ny);
 #pragma omp for schedule(static)
             for (long s = 0; s < (long)B; ++s) {
-                double kap2 = k[s] * k[s] + l[s] * l[s];
+                real kap2 = k[s] * k[s] + l[s] * l[s];
                 shift[s] = lambda + nu * kap2;
-                ph[s].setup(P, 1.0, kap2 == 0.0 ? 1.0 : kap2, false);  // batchedinfluence.jl:60-62
+                ph[s].setup(P, 1.0, kap2 == 0.0 ? (real)1.0 : kap2, false);  // batchedinfluence.jl:60-62
                 vh[s].setup(P, nu, shift[s], false);
                 // responses p+, v+, p-, v- (64-74)
                 ph[s].solve(a.data(), zero.data(), 1.0, 0.0);
@@ -668,11 +690,11 @@ struct BatchedInfluence {
                     pp[s + B * n] = a[n]; vp[s + B * n] = b[n]; pm[s + B * n] = x[n]; vm[s + B * n] = y[n];
                 }
                 // influence inverse (76-88)
-                double A = diff_right(b.data(), Ny), Bq = diff_right(y.data(), Ny);
-                double C = diff_left(b.data(), Ny), D = diff_left(y.data(), Ny);
-                double scale = std::max(std::max(std::fabs(A), std::fabs(Bq)), std::max(std::fabs(C), std::fabs(D)));
+                real A = diff_right(b.data(), Ny), Bq = diff_right(y.data(), Ny);
+                real C = diff_left(b.data(), Ny), D = diff_left(y.data(), Ny);
+                real scale = std::max(std::max(std::fabs(A), std::fabs(Bq)), std::max(std::fabs(C), std::fabs(D)));
                 A /= scale; Bq /= scale; C /= scale; D /= scale;
-                double det = A * D - Bq * C;
+                real det = A * D - Bq * C;
                 if (!(std::isfinite(det) && std::fabs(det) > 2.220446049250313e-16 * std::max(std::fabs(A * D), std::fabs(Bq * C))))
                     bad[s] = 1;
                 ia[s] = D / (det * scale); ib[s] = -Bq / (det * scale); ic[s] = -C / (det * scale); id[s] = A / (det * scale);
@@ -684,13 +706,13 @@ struct BatchedInfluence {
                 diff1(dw.data(), q0.data(), Ny);
                 vh[s].solve(r0.data(), dw.data(), 0.0, 0.0);
                 // _batchinfluence!(solver, p0, v0) on this row
-                double upper = 0.0, lower = 0.0;
+                real upper = 0.0, lower = 0.0;
                 for (int j = 1; j < Ny; ++j) {
-                    double term = (double)j * j * r0[j];
+                    real term = (real)j * j * r0[j];
                     upper -= term;
-                    lower -= ((j % 2 == 1) ? 1.0 : -1.0) * term;
+                    lower -= (real)((j % 2 == 1) ? 1.0 : -1.0) * term;
                 }
-                double dp = ia[s] * upper + ib[s] * lower, dm = ic[s] * upper + id[s] * lower;
+                real dp = ia[s] * upper + ib[s] * lower, dm = ic[s] * upper + id[s] * lower;
                 for (int n = 0; n < Ny; ++n) {
                     q0[n] += dp * a[n] + dm * x[n];
                     r0[n] += dp * b[n] + dm * y[n];
@@ -723,9 +745,9 @@ struct BatchedInfluence {
                 // influence
                 cplx upper = 0.0, lower = 0.0;
                 for (int j = 1; j < Ny; ++j) {
-                    cplx term = (double)j * j * vc[j];
+                    cplx term = (real)j * j * vc[j];
                     upper -= term;
-                    lower -= ((j % 2 == 1) ? 1.0 : -1.0) * term;
+                    lower -= (real)((j % 2 == 1) ? 1.0 : -1.0) * term;
                 }
                 cplx dp = ia[s] * upper + ib[s] * lower, dm = ic[s] * upper + id[s] * lower;
                 for (int n = 0; n < Ny; ++n) {
@@ -734,8 +756,8 @@ struct BatchedInfluence {
                 }
                 diff1(q.data(), pc.data(), Ny);
                 // tau
-                cplx s1 = (shift[s] * vc[Ny - 2] + q[Ny - 2] - ry[Ny - 2]) / (1.0 - sig1[s]);
-                cplx s2 = (shift[s] * vc[Ny - 1] + q[Ny - 1] - ry[Ny - 1]) / (1.0 - sig2[s]);
+                cplx s1 = (shift[s] * vc[Ny - 2] + q[Ny - 2] - ry[Ny - 2]) / ((real)1.0 - sig1[s]);
+                cplx s2 = (shift[s] * vc[Ny - 1] + q[Ny - 1] - ry[Ny - 1]) / ((real)1.0 - sig2[s]);
                 for (int n = 0; n < Ny; ++n) {
                     pc[n] += ((n % 2 == 0) ? s1 : s2) * p0[s + B * n];
                     vc[n] += ((n % 2 == 0) ? s2 : s1) * v0[s + B * n];
@@ -757,15 +779,15 @@ struct BatchedInfluence {
 // ---------------------------------------------------------------------------
 struct MeanMode {
     int Ny;
-    double nu, lambda;
+    real nu, lambda;
     Helmholtz vel;
-    std::vector<double> response;
-    double response_mean = 0;
+    std::vector<real> response;
+    real response_mean = 0;
     bool ok = true;
-    MeanMode(int Ny_, double nu_, double lam_) : Ny(Ny_), nu(nu_), lambda(lam_) {
+    MeanMode(int Ny_, real nu_, real lam_) : Ny(Ny_), nu(nu_), lambda(lam_) {
         vel.setup(Ny - 1, nu, lambda, false);
         response.assign(Ny, 0.0);
-        std::vector<double> work(Ny, 0.0);
+        std::vector<real> work(Ny, 0.0);
         work[0] = 1.0;
         vel.solve(response.data(), work.data(), 0.0, 0.0);
         response_mean = bulkmean(response.data(), Ny);
@@ -773,17 +795,17 @@ struct MeanMode {
     }
     // kind 0: pressure gradient (g0,g1); kind 1: bulk velocity target (g0,g1)
     void solve(cplx* u, cplx* v, cplx* w, cplx* p, const cplx* Rx, const cplx* Ry, const cplx* Rz, int kind,
-               double c0, double c1, double* gout) const {
+               real c0, real c1, real* gout) const {
         int P = Ny - 1;
         for (int n = 1; n <= P; ++n) {  // meanmode.jl:121-126 (1-based n+1 -> index n)
-            cplx lower = (n == 1) ? Ry[0] : Ry[n - 1] / 2.0;
-            cplx upper = (n + 1 < P) ? Ry[n + 1] / 2.0 : cplx(0.0);
-            p[n] = (lower - upper) / (double)n;
+            cplx lower = (n == 1) ? Ry[0] : Ry[n - 1] / (real)2.0;
+            cplx upper = (n + 1 < P) ? Ry[n + 1] / (real)2.0 : cplx(0.0);
+            p[n] = (lower - upper) / (real)n;
         }
         p[0] = 0.0;
         p[0] = -bulkmean(p, Ny);
         for (int n = 0; n < Ny; ++n) v[n] = 0.0;
-        std::vector<double> work(Ny), sol(Ny);
+        std::vector<real> work(Ny), sol(Ny);
         for (int i = 0; i < 2; ++i) {
             cplx* out = i == 0 ? u : w;
             const cplx* src = i == 0 ? Rx : Rz;
@@ -795,8 +817,8 @@ struct MeanMode {
                     else out[n] += cplx(0, 1) * sol[n];
                 }
             }
-            double target = i == 0 ? c0 : c1;
-            double gradient = (kind == 0) ? target : (target - bulkmean(out, Ny).real()) / response_mean;
+            real target = i == 0 ? c0 : c1;
+            real gradient = (kind == 0) ? target : (target - bulkmean(out, Ny).real()) / response_mean;
             for (int n = 0; n < Ny; ++n) out[n] += gradient * response[n];
             gout[i] = gradient;
         }
@@ -817,9 +839,9 @@ struct Stokes {
     Grid g;
     std::unique_ptr<MeanMode> mean;
     std::unique_ptr<BatchedInfluence> modes;
-    Stokes(const Grid& g_, double nu, double lambda) : g(g_) {
+    Stokes(const Grid& g_, real nu, real lambda) : g(g_) {
         mean.reset(new MeanMode(g.Ny, nu, lambda));
-        std::vector<double> k(g.B()), l(g.B());
+        std::vector<real> k(g.B()), l(g.B());
         for (int iz = 0; iz < g.Nz; ++iz)
             for (int ix = 0; ix < g.Nxh; ++ix) {
                 k[ix + (size_t)g.Nxh * iz] = g.alpha() * ix;
@@ -828,9 +850,9 @@ struct Stokes {
         modes.reset(new BatchedInfluence(g.Ny, k, l, nu, lambda));
     }
     // kind: 0 = pressuregradient (c0,c1), 1 = bulkvelocity total (c0,c1)
-    void solve(cplx* const U[3], cplx* P, const cplx* const R[3], int kind, double c0, double c1,
-               const double* basecoeffs, double* gout) const {
-        double t0 = c0, t1 = c1;
+    void solve(cplx* const U[3], cplx* P, const cplx* const R[3], int kind, real c0, real c1,
+               const real* basecoeffs, real* gout) const {
+        real t0 = c0, t1 = c1;
         if (kind == 1 && basecoeffs) t0 = c0 - bulkmean(basecoeffs, g.Ny);
         modes->solve(U[0], U[1], U[2], P, R[0], R[1], R[2]);
         size_t B = g.B();
@@ -857,9 +879,9 @@ enum Form { ROTATING = 0, CONVECTIVE = 1, DIVERGENCE = 2 };
 struct NonLinear {
     Grid g;
     Form form;
-    std::vector<double> base;  // Chebyshev coeffs of Ub
+    std::vector<real> base;  // Chebyshev coeffs of Ub
     Transforms tr;
-    NonLinear(const Grid& g_, Form f, const std::vector<double>& b) : g(g_), form(f), base(b), tr(g_) {}
+    NonLinear(const Grid& g_, Form f, const std::vector<real>& b) : g(g_), form(f), base(b), tr(g_) {}
 
     // (t, U, N, add) -> N (= or +=)
     void eval(const cplx* const U[3], cplx* const N[3], bool add) {
@@ -878,7 +900,7 @@ struct NonLinear {
             cplx* O[3] = {Om[0].data(), Om[1].data(), Om[2].data()};
             const cplx* T[3] = {TMP[0].data(), TMP[1].data(), TMP[2].data()};
             curl(g, O, T);
-            std::vector<double> u[3], w[3], nn(Pn);
+            std::vector<real> u[3], w[3], nn(Pn);
             for (int i = 0; i < 3; ++i) {
                 u[i].resize(Pn); w[i].resize(Pn);
                 tr.inverse(T[i], u[i].data());
@@ -891,7 +913,7 @@ struct NonLinear {
             }
             sign = 1;
         } else if (form == CONVECTIVE) {  // nonlinear.jl:113-140
-            std::vector<double> u[3], gr(Pn), nn[3];
+            std::vector<real> u[3], gr(Pn), nn[3];
             for (int i = 0; i < 3; ++i) {
                 u[i].resize(Pn);
                 tr.inverse(TMP[i].data(), u[i].data());
@@ -910,7 +932,7 @@ struct NonLinear {
             for (int i = 0; i < 3; ++i) tr.forward(nn[i].data(), out[i].data());
             sign = -1;
         } else {  // DIVERGENCE nonlinear.jl:146-167
-            std::vector<double> u[3], uu(Pn);
+            std::vector<real> u[3], uu(Pn);
             for (int i = 0; i < 3; ++i) { u[i].resize(Pn); tr.inverse(TMP[i].data(), u[i].data()); }
             std::vector<cplx> UU(S);
             for (int i = 0; i < 3; ++i) {
@@ -926,41 +948,41 @@ struct NonLinear {
             sign = -1;
         }
         for (int i = 0; i < 3; ++i)
-            for (size_t q = 0; q < S; ++q) N[i][q] = add ? N[i][q] + (double)sign * out[i][q] : (double)sign * out[i][q];
+            for (size_t q = 0; q < S; ++q) N[i][q] = add ? N[i][q] + (real)sign * out[i][q] : (real)sign * out[i][q];
     }
 };
 
 // ---------------------------------------------------------------------------
 // CNRK2 (src/timesteppers/cnrk2.jl)
 // ---------------------------------------------------------------------------
-static const double CNRK2_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
-static const double CNRK2_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
-static const double CNRK2_C[3] = {1.0 / 6.0, 5.0 / 24.0, 1.0 / 8.0};
+static const real CNRK2_A[3] = {0.0, (real)-5.0 / 9.0, (real)-153.0 / 128.0};
+static const real CNRK2_B[3] = {(real)1.0 / 3.0, (real)15.0 / 16.0, (real)8.0 / 15.0};
+static const real CNRK2_C[3] = {(real)1.0 / 6.0, (real)5.0 / 24.0, (real)1.0 / 8.0};
 
 struct Problem {
     Grid g;
-    double nu, dt;
+    real nu, dt;
     int kind;  // 0 pressuregradient, 1 bulkvelocity
-    double c0, c1;
-    std::vector<double> base, curv;
+    real c0, c1;
+    std::vector<real> base, curv;
     std::unique_ptr<Stokes> solvers[3];
     std::unique_ptr<NonLinear> nl;
     std::vector<cplx> Q[3], N[3], R[3];
-    double last_grad[2] = {0, 0};
+    real last_grad[2] = {0, 0};
     // forcing(tstage, U, N) (cnrk2.jl:173-175, forcing.jl:3-11) of the form
     // F(t) = F0 + sin(omega t) F1 + cos(omega t) F2, added into N at the stage times;
     // t advances by dt per step
     bool forced = false;
     std::vector<cplx> F0[3], F1[3], F2[3];
-    double omega = 0.0, t = 0.0;
-    Problem(const Grid& g_, const double* basevals, double nu_, double dt_, int form, int kind_, double c0_, double c1_)
+    real omega = 0.0, t = 0.0;
+    Problem(const Grid& g_, const real* basevals, real nu_, real dt_, int form, int kind_, real c0_, real c1_)
         : g(g_), nu(nu_), dt(dt_), kind(kind_), c0(c0_), c1(c1_) {
         base.resize(g.Ny); curv.resize(g.Ny);
         chebcoeffs(g.Ny, basevals, base.data());  // cnrk2.jl:76
-        std::vector<double> d1(g.Ny);
+        std::vector<real> d1(g.Ny);
         diff1(d1.data(), base.data(), g.Ny);      // diff2! (77)
         diff1(curv.data(), d1.data(), g.Ny);
-        for (int j = 0; j < 3; ++j) solvers[j].reset(new Stokes(g, nu, 1.0 / (CNRK2_C[j] * dt)));
+        for (int j = 0; j < 3; ++j) solvers[j].reset(new Stokes(g, nu, (real)1.0 / (CNRK2_C[j] * dt)));
         nl.reset(new NonLinear(g, (Form)form, base));
         for (int i = 0; i < 3; ++i) { Q[i].assign(g.S(), 0.0); N[i].assign(g.S(), 0.0); R[i].assign(g.S(), 0.0); }
     }
@@ -973,7 +995,7 @@ struct Problem {
     // (0 = none, 1 = drop the shear term, 2 = drop the base term, 3 = drop
     // B[N], 4 = flip the shear sign); it is never set outside those tests.
     static int bulkgrad_mutation;
-    void bulkgrad(const cplx* const U[3], double out[2]) {
+    void bulkgrad(const cplx* const U[3], real out[2]) {
         int Ny = g.Ny;
         size_t B = g.B();
         std::vector<cplx> u(Ny), n(Ny);
@@ -981,8 +1003,8 @@ struct Problem {
             int c = i == 0 ? 0 : 2;
             for (int m = 0; m < Ny; ++m) { u[m] = U[c][B * m]; n[m] = N[c][B * m]; }
             cplx shear = diff_right(u.data(), Ny) - diff_left(u.data(), Ny);
-            double bs = i == 0 ? nu * bulkmean(curv.data(), Ny) : 0.0;
-            double sh = nu * shear.real() / 2.0, bn = bulkmean(n.data(), Ny).real();
+            real bs = i == 0 ? nu * bulkmean(curv.data(), Ny) : (real)0.0;
+            real sh = nu * shear.real() / (real)2.0, bn = bulkmean(n.data(), Ny).real();
             if (bulkgrad_mutation == 1) sh = 0.0;
             if (bulkgrad_mutation == 2) bs = 0.0;
             if (bulkgrad_mutation == 3) bn = 0.0;
@@ -993,27 +1015,27 @@ struct Problem {
     // step! cnrk2.jl:149-207 (forcing = NoForcing)
     void step(cplx* const U[3], cplx* P) {
         size_t S = g.S(), B = g.B();
-        double grads[2] = {kind == 0 ? c0 : 0.0, kind == 0 ? c1 : 0.0};
+        real grads[2] = {kind == 0 ? c0 : (real)0.0, kind == 0 ? c1 : (real)0.0};
         cplx* Np[3] = {N[0].data(), N[1].data(), N[2].data()};
         cplx* Rp[3] = {R[0].data(), R[1].data(), R[2].data()};
-        static const double STAGE_T[3] = {0.0, 1.0 / 3.0, 3.0 / 4.0};  // cnrk2.jl:170
+        static const real STAGE_T[3] = {0.0, (real)1.0 / 3.0, (real)3.0 / 4.0};  // cnrk2.jl:170
         for (int j = 0; j < 3; ++j) {
             const cplx* Uc[3] = {U[0], U[1], U[2]};
             nl->eval(Uc, Np, false);
             if (forced) {
-                const double sw = std::sin(omega * (t + STAGE_T[j] * dt));
-                const double cw = std::cos(omega * (t + STAGE_T[j] * dt));
+                const real sw = std::sin(omega * (t + STAGE_T[j] * dt));
+                const real cw = std::cos(omega * (t + STAGE_T[j] * dt));
                 for (int i = 0; i < 3; ++i)
                     for (size_t q = 0; q < S; ++q) N[i][q] += F0[i][q] + sw * F1[i][q] + cw * F2[i][q];
             }
-            double old[2] = {grads[0], grads[1]};
+            real old[2] = {grads[0], grads[1]};
             if (kind == 1) bulkgrad(Uc, old);
             for (int i = 0; i < 3; ++i) {
                 if (j == 0) std::copy(N[i].begin(), N[i].end(), Q[i].begin());
                 else for (size_t q = 0; q < S; ++q) Q[i][q] = CNRK2_A[j] * Q[i][q] + N[i][q];
             }
-            double lambda = 1.0 / (CNRK2_C[j] * dt);
-            double weight = CNRK2_B[j] / CNRK2_C[j];
+            real lambda = (real)1.0 / (CNRK2_C[j] * dt);
+            real weight = CNRK2_B[j] / CNRK2_C[j];
             for (int i = 0; i < 3; ++i) {
                 laplacian(g, R[i].data(), U[i]);
                 if (i == 0) ddx1(g, N[i].data(), P, false);
@@ -1021,11 +1043,11 @@ struct Problem {
                 else ddx3(g, N[i].data(), P, false);
                 for (size_t q = 0; q < S; ++q) R[i][q] = lambda * U[i][q] + nu * R[i][q] - N[i][q] + weight * Q[i][q];
             }
-            for (int n = 0; n < g.Ny; ++n) R[0][B * n] += 2.0 * nu * curv[n];
+            for (int n = 0; n < g.Ny; ++n) R[0][B * n] += (real)2.0 * nu * curv[n];
             R[0][0] -= old[0];
             R[2][0] -= old[1];
             const cplx* Rc[3] = {Rp[0], Rp[1], Rp[2]};
-            double gout[2];
+            real gout[2];
             solvers[j]->solve(U, P, Rc, kind, kind == 0 ? c0 : c0, kind == 0 ? c1 : c1, base.data(), gout);
             if (kind == 0) { grads[0] = c0; grads[1] = c1; }
             last_grad[0] = gout[0]; last_grad[1] = gout[1];
@@ -1037,7 +1059,7 @@ struct Problem {
 // ---------------------------------------------------------------------------
 // pressure reconstruction (src/pressure.jl:176-285) and projection (117-135)
 // ---------------------------------------------------------------------------
-static void pressure_poisson(const Grid& g, const cplx* const A[3], const cplx* v, double nu, cplx* P) {
+static void pressure_poisson(const Grid& g, const cplx* const A[3], const cplx* v, real nu, cplx* P) {
     // div!(P, A) (operators.jl:270-276)
     ddx1(g, P, A[0], false);
     ddx2(g, P, A[1], true);
@@ -1046,7 +1068,7 @@ static void pressure_poisson(const Grid& g, const cplx* const A[3], const cplx* 
     size_t B = g.B();
 #pragma omp parallel
     {
-        std::vector<double> work(Ny), sol(Ny);
+        std::vector<real> work(Ny), sol(Ny);
         std::vector<cplx> rhs(Ny), normal(Ny);
         Helmholtz h;
 #pragma omp for collapse(2) schedule(static)
@@ -1054,14 +1076,14 @@ static void pressure_poisson(const Grid& g, const cplx* const A[3], const cplx* 
             for (long ix = 0; ix < g.Nxh; ++ix) {
                 if (ix == 0 && iz == 0) continue;
                 if (g.xnyq((int)ix) || g.znyq((int)iz)) continue;
-                double k = g.alpha() * ix, l = g.beta() * g.lmode((int)iz);
+                real k = g.alpha() * ix, l = g.beta() * g.lmode((int)iz);
                 h.setup(Ny - 1, 1.0, k * k + l * l, true);
                 size_t s = ix + (size_t)g.Nxh * iz;
                 for (int n = 0; n < Ny; ++n) { rhs[n] = P[s + B * n]; normal[n] = A[1][s + B * n]; }
                 cplx upper = 0.0, lower = 0.0;
-                for (int j = 0; j < Ny; ++j) { upper += normal[j]; lower += ((j % 2) ? -1.0 : 1.0) * normal[j]; }
+                for (int j = 0; j < Ny; ++j) { upper += normal[j]; lower += (real)((j % 2) ? -1.0 : 1.0) * normal[j]; }
                 for (int n = 2; n <= Ny - 1; ++n) {
-                    cplx wall = nu * ((double)n * n * ((double)n * n - 1.0) / 3.0) * v[s + B * n];
+                    cplx wall = nu * ((real)n * n * ((real)n * n - (real)1.0) / (real)3.0) * v[s + B * n];
                     upper += wall;
                     lower += (n % 2 == 0) ? wall : -wall;
                 }
@@ -1077,9 +1099,9 @@ static void pressure_poisson(const Grid& g, const cplx* const A[3], const cplx* 
     std::vector<cplx> normal(Ny), mean(Ny);
     for (int n = 0; n < Ny; ++n) normal[n] = A[1][B * n];
     for (int n = 1; n <= Ny - 1; ++n) {
-        cplx lower = n == 1 ? normal[0] : normal[n - 1] / 2.0;
-        cplx upper = (n + 1 < Ny - 1) ? normal[n + 1] / 2.0 : cplx(0.0);
-        mean[n] = (lower - upper) / (double)n;
+        cplx lower = n == 1 ? normal[0] : normal[n - 1] / (real)2.0;
+        cplx upper = (n + 1 < Ny - 1) ? normal[n + 1] / (real)2.0 : cplx(0.0);
+        mean[n] = (lower - upper) / (real)n;
     }
     mean[0] = 0.0;
     mean[0] = -bulkmean(mean.data(), Ny);
@@ -1090,14 +1112,14 @@ static void pressure_poisson(const Grid& g, const cplx* const A[3], const cplx* 
 // ---------------------------------------------------------------------------
 // Diagnostics (src/postprocessing.jl:21-128)
 // ---------------------------------------------------------------------------
-static double spectral_dot(const Grid& g, const cplx* U, const cplx* V) {
+static real spectral_dot(const Grid& g, const cplx* U, const cplx* V) {
     int Ny = g.Ny;
-    auto moment = [](int n) { return (n % 2 == 0) ? 1.0 / (1.0 - (double)n * n) : 0.0; };
-    std::vector<double> M((size_t)Ny * Ny);
+    auto moment = [](int n) { return (n % 2 == 0) ? (real)1.0 / ((real)1.0 - (real)n * n) : (real)0.0; };
+    std::vector<real> M((size_t)Ny * Ny);
     for (int m = 0; m < Ny; ++m)
-        for (int n = 0; n < Ny; ++n) M[m + (size_t)Ny * n] = (moment(m + n) + moment(std::abs(m - n))) / 2.0;
+        for (int n = 0; n < Ny; ++n) M[m + (size_t)Ny * n] = (moment(m + n) + moment(std::abs(m - n))) / (real)2.0;
     size_t B = g.B();
-    double result = 0.0;
+    real result = 0.0;
     std::vector<cplx> work(Ny), vv(Ny);
     for (int iz = 0; iz < g.Nz; ++iz)
         for (int ix = 0; ix < g.Nxh; ++ix) {
@@ -1111,10 +1133,49 @@ static double spectral_dot(const Grid& g, const cplx* U, const cplx* V) {
             }
             cplx d = 0.0;
             for (int n = 0; n < Ny; ++n) d += std::conj(U[s + B * n]) * work[n];
-            result += (ix == 0 ? 1.0 : 2.0) * d.real();
+            result += (real)(ix == 0 ? 1.0 : 2.0) * d.real();
         }
     return result;
 }
+
+// ---------------------------------------------------------------------------
+// FP64 arrays at the C ABI.  In the FP64 build these are the caller's arrays
+// themselves; in the extended build, converted copies (Out writes back,
+// rounding to FP64, when it goes out of scope).  T = real or cplx.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct In {
+    std::vector<T> buf;
+    const T* p = nullptr;
+    In(const double* d, size_t count) {
+        if (!d) return;
+        if (std::is_same<real, double>::value) { p = reinterpret_cast<const T*>(d); return; }
+        buf.resize(count);
+        real* r = reinterpret_cast<real*>(buf.data());
+        for (size_t i = 0; i < count * (sizeof(T) / sizeof(real)); ++i) r[i] = (real)d[i];
+        p = buf.data();
+    }
+};
+template <typename T>
+struct Out {
+    std::vector<T> buf;
+    T* p = nullptr;
+    double* d;
+    size_t count;
+    Out(double* d_, size_t count_) : d(d_), count(count_) {
+        if (!d) return;
+        if (std::is_same<real, double>::value) { p = reinterpret_cast<T*>(d); return; }
+        buf.resize(count);
+        real* r = reinterpret_cast<real*>(buf.data());
+        for (size_t i = 0; i < count * (sizeof(T) / sizeof(real)); ++i) r[i] = (real)d[i];
+        p = buf.data();
+    }
+    ~Out() {
+        if (!d || std::is_same<real, double>::value) return;
+        const real* r = reinterpret_cast<const real*>(buf.data());
+        for (size_t i = 0; i < count * (sizeof(T) / sizeof(real)); ++i) d[i] = (double)r[i];
+    }
+};
 
 // ===========================================================================
 // C ABI for the Python test harness (tests/, smoke(), bench cpu_baseline).
@@ -1125,6 +1186,9 @@ int Problem::bulkgrad_mutation = 0;
 extern "C" {
 
 const char* orc_last_error() { return g_error.c_str(); }
+
+// 1 in the extended-precision build, 0 in the FP64 build
+int orc_extended() { return std::is_same<real, double>::value ? 0 : 1; }
 
 int orc_num_threads() {
 #ifdef _OPENMP
@@ -1145,31 +1209,40 @@ void orc_set_threads(int n) {
 int orc_redft00(int B, int Ny, const double* src, double* dst) {
     Grid g(1, Ny, 1, 1.0, 1.0);
     Transforms t(g);
-    t.cheb((const cplx*)src, (cplx*)dst, false, (size_t)B);
+    In<cplx> s(src, (size_t)B * Ny);
+    Out<cplx> o(dst, (size_t)B * Ny);
+    t.cheb(s.p, o.p, false, (size_t)B);
     return 0;
 }
 int orc_cheb(int B, int Ny, int inverse, const double* src, double* dst) {
     Grid g(1, Ny, 1, 1.0, 1.0);
     Transforms t(g);
-    t.cheb((const cplx*)src, (cplx*)dst, inverse != 0, (size_t)B);
+    In<cplx> s(src, (size_t)B * Ny);
+    Out<cplx> o(dst, (size_t)B * Ny);
+    t.cheb(s.p, o.p, inverse != 0, (size_t)B);
     return 0;
 }
 int orc_forward(int Nx, int Ny, int Nz, double Lx, double Lz, const double* u, double* U) {
     Grid g(Nx, Ny, Nz, Lx, Lz);
     Transforms t(g);
-    t.forward(u, (cplx*)U);
+    In<real> s(u, g.phys());
+    Out<cplx> o(U, g.S());
+    t.forward(s.p, o.p);
     return 0;
 }
 int orc_inverse(int Nx, int Ny, int Nz, double Lx, double Lz, const double* U, double* u) {
     Grid g(Nx, Ny, Nz, Lx, Lz);
     Transforms t(g);
-    t.inverse((const cplx*)U, u);
+    In<cplx> s(U, g.S());
+    Out<real> o(u, g.phys());
+    t.inverse(s.p, o.p);
     return 0;
 }
 int orc_normalize_forward(int Nx, int Ny, int Nz, double norm, double* U) {
     Grid g(Nx, Ny, Nz, 1.0, 1.0);
     Transforms t(g);
-    t.normalize_forward((cplx*)U, norm);
+    Out<cplx> o(U, g.S());
+    t.normalize_forward(o.p, norm);
     return 0;
 }
 
@@ -1177,13 +1250,13 @@ int orc_normalize_forward(int Nx, int Ny, int Nz, double norm, double* U) {
 // op: 1 ddx1, 2 ddx2, 3 ddx3, 4 laplacian
 int orc_operator(int op, int Nx, int Ny, int Nz, double Lx, double Lz, const double* U, double* out, int add) {
     Grid g(Nx, Ny, Nz, Lx, Lz);
-    const cplx* a = (const cplx*)U;
-    cplx* o = (cplx*)out;
+    In<cplx> a(U, g.S());
+    Out<cplx> o(out, g.S());
     switch (op) {
-        case 1: ddx1(g, o, a, add != 0); break;
-        case 2: ddx2(g, o, a, add != 0); break;
-        case 3: ddx3(g, o, a, add != 0); break;
-        case 4: laplacian(g, o, a); break;
+        case 1: ddx1(g, o.p, a.p, add != 0); break;
+        case 2: ddx2(g, o.p, a.p, add != 0); break;
+        case 3: ddx3(g, o.p, a.p, add != 0); break;
+        case 4: laplacian(g, o.p, a.p); break;
         default: g_error = "unknown operator"; return 1;
     }
     return 0;
@@ -1191,42 +1264,60 @@ int orc_operator(int op, int Nx, int Ny, int Nz, double Lx, double Lz, const dou
 int orc_curl(int Nx, int Ny, int Nz, double Lx, double Lz, const double* U3, double* O3) {
     Grid g(Nx, Ny, Nz, Lx, Lz);
     size_t S = g.S();
-    const cplx* U[3] = {(const cplx*)U3, (const cplx*)U3 + S, (const cplx*)U3 + 2 * S};
-    cplx* O[3] = {(cplx*)O3, (cplx*)O3 + S, (cplx*)O3 + 2 * S};
+    In<cplx> a(U3, 3 * S);
+    Out<cplx> o(O3, 3 * S);
+    const cplx* U[3] = {a.p, a.p + S, a.p + 2 * S};
+    cplx* O[3] = {o.p, o.p + S, o.p + 2 * S};
     curl(g, O, U);
     return 0;
 }
 
 // -- CHS stand-ins ---------------------------------------------------------------
-int orc_chebcoeffs(int Ny, const double* values, double* coeffs) { chebcoeffs(Ny, values, coeffs); return 0; }
-int orc_diff(int Ny, const double* a, double* out) { diff1(out, a, Ny); return 0; }
+int orc_chebcoeffs(int Ny, const double* values, double* coeffs) {
+    In<real> v(values, Ny);
+    Out<real> c(coeffs, Ny);
+    chebcoeffs(Ny, v.p, c.p);
+    return 0;
+}
+int orc_diff(int Ny, const double* a, double* out) {
+    In<real> v(a, Ny);
+    Out<real> o(out, Ny);
+    diff1(o.p, v.p, Ny);
+    return 0;
+}
 int orc_helmholtz(int P, double t0, double t1, int neumann, const double* f, double* out, double gp, double gm) {
     Helmholtz h(P, t0, t1, neumann != 0);
-    h.solve(out, f, gp, gm);
+    In<real> ff(f, P + 1);
+    Out<real> o(out, P + 1);
+    h.solve(o.p, ff.p, gp, gm);
     return 0;
 }
 // complex variant (f, out interleaved complex; gp, gm complex)
 int orc_helmholtz_c(int P, double t0, double t1, int neumann, const double* f, double* out, const double* gp,
                     const double* gm) {
     Helmholtz h(P, t0, t1, neumann != 0);
-    h.solve((cplx*)out, (const cplx*)f, cplx(gp[0], gp[1]), cplx(gm[0], gm[1]));
+    In<cplx> ff(f, P + 1);
+    Out<cplx> o(out, P + 1);
+    h.solve(o.p, ff.p, cplx(gp[0], gp[1]), cplx(gm[0], gm[1]));
     return 0;
 }
 
 // -- modal solvers -----------------------------------------------------------------
 // Scalar InfluenceModeSolver: u,v,w,p,Rx,Ry,Rz complex Ny vectors.
-// info (out, 18 doubles): Ainv[4], sigma0[2], shift, ok, then
-//   pp[0], pm[0]... (not used)
+// responses (out, optional, 6 x Ny): pp, pm, vp, vm, p0, v0.
 int orc_influence_mode(int Ny, double k, double l, double nu, double lambda, const double* Rx, const double* Ry,
                        const double* Rz, double* u, double* v, double* w, double* p, double* responses) {
     if (!(Ny >= 5 && Ny % 2 == 1)) { g_error = "Gibson's tau correction requires odd Ny >= 5"; return 2; }
     if (!(k * k + l * l > 0)) { g_error = "InfluenceModeSolver requires kappa2 > 0"; return 2; }
     InfluenceMode m(Ny, k, l, nu, lambda);
     if (!m.ok) { g_error = "the influence matrix is numerically singular"; return 2; }
-    m.solve((cplx*)u, (cplx*)v, (cplx*)w, (cplx*)p, (const cplx*)Rx, (const cplx*)Ry, (const cplx*)Rz);
+    In<cplx> rx(Rx, Ny), ry(Ry, Ny), rz(Rz, Ny);
+    Out<cplx> uo(u, Ny), vo(v, Ny), wo(w, Ny), po(p, Ny);
+    m.solve(uo.p, vo.p, wo.p, po.p, rx.p, ry.p, rz.p);
     if (responses) {
-        const std::vector<double>* r[6] = {&m.pp, &m.pm, &m.vp, &m.vm, &m.p0, &m.v0};
-        for (int i = 0; i < 6; ++i) std::copy(r[i]->begin(), r[i]->end(), responses + (size_t)i * Ny);
+        const std::vector<real>* r[6] = {&m.pp, &m.pm, &m.vp, &m.vm, &m.p0, &m.v0};
+        for (int i = 0; i < 6; ++i)
+            for (int n = 0; n < Ny; ++n) responses[(size_t)i * Ny + n] = (double)(*r[i])[n];
     }
     return 0;
 }
@@ -1235,10 +1326,13 @@ int orc_batched_influence(int Ny, int B, const double* k, const double* l, doubl
                           const double* Rx, const double* Ry, const double* Rz, double* u, double* v, double* w,
                           double* p) {
     if (!(Ny >= 5 && Ny % 2 == 1)) { g_error = "tau correction requires odd Ny >= 5"; return 2; }
-    std::vector<double> kk(k, k + B), ll(l, l + B);
+    std::vector<real> kk(k, k + B), ll(l, l + B);
     BatchedInfluence h(Ny, kk, ll, nu, lambda);
     if (!h.ok) { g_error = h.err; return 2; }
-    h.solve((cplx*)u, (cplx*)v, (cplx*)w, (cplx*)p, (const cplx*)Rx, (const cplx*)Ry, (const cplx*)Rz);
+    size_t n = (size_t)B * Ny;
+    In<cplx> rx(Rx, n), ry(Ry, n), rz(Rz, n);
+    Out<cplx> uo(u, n), vo(v, n), wo(w, n), po(p, n);
+    h.solve(uo.p, vo.p, wo.p, po.p, rx.p, ry.p, rz.p);
     return 0;
 }
 // MeanModeSolver. kind 0 pressure gradient, 1 bulk velocity; grads[2] out.
@@ -1250,9 +1344,14 @@ int orc_mean_mode(int Ny, double nu, double lambda, const double* Rx, const doub
     if (!(std::isfinite(lambda) && lambda >= 0)) { g_error = "lambda must be finite and nonnegative"; return 2; }
     MeanMode m(Ny, nu, lambda);
     if (!m.ok) { g_error = "the uniform-gradient response has no finite nonzero bulk mean"; return 2; }
-    m.solve((cplx*)u, (cplx*)v, (cplx*)w, (cplx*)p, (const cplx*)Rx, (const cplx*)Ry, (const cplx*)Rz, kind, c0, c1,
-            grads);
-    if (response) std::copy(m.response.begin(), m.response.end(), response);
+    In<cplx> rx(Rx, Ny), ry(Ry, Ny), rz(Rz, Ny);
+    Out<cplx> uo(u, Ny), vo(v, Ny), wo(w, Ny), po(p, Ny);
+    real g2[2];
+    m.solve(uo.p, vo.p, wo.p, po.p, rx.p, ry.p, rz.p, kind, c0, c1, g2);
+    grads[0] = (double)g2[0];
+    grads[1] = (double)g2[1];
+    if (response)
+        for (int n = 0; n < Ny; ++n) response[n] = (double)m.response[n];
     return 0;
 }
 static int check_grid(int Nx, int Ny, int Nz, double Lx, double Lz) {
@@ -1271,9 +1370,15 @@ int orc_stokes(int Nx, int Ny, int Nz, double Lx, double Lz, double nu, double l
     Stokes st(g, nu, lambda);
     if (!st.modes->ok) { g_error = st.modes->err; return 2; }
     size_t S = g.S();
-    cplx* U[3] = {(cplx*)U3, (cplx*)U3 + S, (cplx*)U3 + 2 * S};
-    const cplx* R[3] = {(const cplx*)R3, (const cplx*)R3 + S, (const cplx*)R3 + 2 * S};
-    st.solve(U, (cplx*)P, R, kind, c0, c1, basecoeffs, grads);
+    In<cplx> r(R3, 3 * S);
+    In<real> bc(basecoeffs, Ny);
+    Out<cplx> uo(U3, 3 * S), po(P, S);
+    cplx* U[3] = {uo.p, uo.p + S, uo.p + 2 * S};
+    const cplx* R[3] = {r.p, r.p + S, r.p + 2 * S};
+    real g2[2];
+    st.solve(U, po.p, R, kind, c0, c1, bc.p, g2);
+    grads[0] = (double)g2[0];
+    grads[1] = (double)g2[1];
     return 0;
 }
 
@@ -1284,7 +1389,8 @@ void* orc_problem_create(int Nx, int Ny, int Nz, double Lx, double Lz, double nu
         if (check_grid(Nx, Ny, Nz, Lx, Lz)) return nullptr;
         if (!(std::isfinite(dt) && dt > 0)) { g_error = "dt must be finite and positive"; return nullptr; }
         Grid g(Nx, Ny, Nz, Lx, Lz);
-        Problem* p = new Problem(g, basevals, nu, dt, form, kind, c0, c1);
+        In<real> bv(basevals, Ny);
+        Problem* p = new Problem(g, bv.p, nu, dt, form, kind, c0, c1);
         if (!p->ok()) { g_error = "singular influence matrix"; delete p; return nullptr; }
         return p;
     } catch (const std::exception& e) {
@@ -1295,16 +1401,17 @@ void* orc_problem_create(int Nx, int Ny, int Nz, double Lx, double Lz, double nu
 void orc_problem_destroy(void* h) { delete (Problem*)h; }
 int orc_base_coeffs(void* h, double* base, double* curv) {
     Problem* p = (Problem*)h;
-    std::copy(p->base.begin(), p->base.end(), base);
-    std::copy(p->curv.begin(), p->curv.end(), curv);
+    for (int n = 0; n < p->g.Ny; ++n) { base[n] = (double)p->base[n]; curv[n] = (double)p->curv[n]; }
     return 0;
 }
 // N = nonlinear(U) (+= if add)
 int orc_nonlinear(void* h, const double* U3, double* N3, int add) {
     Problem* p = (Problem*)h;
     size_t S = p->g.S();
-    const cplx* U[3] = {(const cplx*)U3, (const cplx*)U3 + S, (const cplx*)U3 + 2 * S};
-    cplx* N[3] = {(cplx*)N3, (cplx*)N3 + S, (cplx*)N3 + 2 * S};
+    In<cplx> u(U3, 3 * S);
+    Out<cplx> o(N3, 3 * S);
+    const cplx* U[3] = {u.p, u.p + S, u.p + 2 * S};
+    cplx* N[3] = {o.p, o.p + S, o.p + 2 * S};
     p->nl->eval(U, N, add != 0);
     return 0;
 }
@@ -1312,8 +1419,9 @@ int orc_nonlinear(void* h, const double* U3, double* N3, int add) {
 int orc_step(void* h, double* U3, double* P, int nsteps) {
     Problem* p = (Problem*)h;
     size_t S = p->g.S();
-    cplx* U[3] = {(cplx*)U3, (cplx*)U3 + S, (cplx*)U3 + 2 * S};
-    for (int s = 0; s < nsteps; ++s) p->step(U, (cplx*)P);
+    Out<cplx> uo(U3, 3 * S), po(P, S);
+    cplx* U[3] = {uo.p, uo.p + S, uo.p + 2 * S};
+    for (int s = 0; s < nsteps; ++s) p->step(U, po.p);
     return 0;
 }
 // forcing F(t) = F0 + sin(omega t) F1 + cos(omega t) F2 (3 spectral fields
@@ -1325,12 +1433,13 @@ int orc_set_forcing(void* h, const double* F0, const double* F1, const double* F
     p->forced = F0 != nullptr;
     p->omega = omega;
     p->t = t0;
+    In<cplx> f0(F0, 3 * S), f1(F1, 3 * S), f2(F2, 3 * S);
     for (int i = 0; i < 3; ++i) {
         if (!F0) { p->F0[i].clear(); p->F1[i].clear(); p->F2[i].clear(); continue; }
-        p->F0[i].assign((const cplx*)F0 + i * S, (const cplx*)F0 + (i + 1) * S);
-        if (F1) p->F1[i].assign((const cplx*)F1 + i * S, (const cplx*)F1 + (i + 1) * S);
+        p->F0[i].assign(f0.p + i * S, f0.p + (i + 1) * S);
+        if (F1) p->F1[i].assign(f1.p + i * S, f1.p + (i + 1) * S);
         else p->F1[i].assign(S, 0.0);
-        if (F2) p->F2[i].assign((const cplx*)F2 + i * S, (const cplx*)F2 + (i + 1) * S);
+        if (F2) p->F2[i].assign(f2.p + i * S, f2.p + (i + 1) * S);
         else p->F2[i].assign(S, 0.0);
     }
     return 0;
@@ -1342,7 +1451,7 @@ int orc_set_bulkgrad_mutation(int m) {
 }
 int orc_last_gradient(void* h, double* g2) {
     Problem* p = (Problem*)h;
-    g2[0] = p->last_grad[0]; g2[1] = p->last_grad[1];
+    g2[0] = (double)p->last_grad[0]; g2[1] = (double)p->last_grad[1];
     return 0;
 }
 // project! (pressure.jl:117-135) in place on U3
@@ -1353,14 +1462,15 @@ int orc_project(void* h, double* U3) {
     Stokes st(g, 1.0, 0.0);
     if (!st.modes->ok) { g_error = st.modes->err; return 2; }
     std::vector<cplx> R[3], phi(S);
-    cplx* U[3] = {(cplx*)U3, (cplx*)U3 + S, (cplx*)U3 + 2 * S};
+    Out<cplx> uo(U3, 3 * S);
+    cplx* U[3] = {uo.p, uo.p + S, uo.p + 2 * S};
     for (int i = 0; i < 3; ++i) {
         R[i].resize(S);
         laplacian(g, R[i].data(), U[i]);
         for (auto& x : R[i]) x = -x;
     }
     const cplx* Rc[3] = {R[0].data(), R[1].data(), R[2].data()};
-    double grads[2];
+    real grads[2];
     if (p->kind == 1) st.solve(U, phi.data(), Rc, 1, p->c0, p->c1, p->base.data(), grads);
     else st.solve(U, phi.data(), Rc, 0, 0.0, 0.0, p->base.data(), grads);
     return 0;
@@ -1372,11 +1482,13 @@ int orc_pressure(void* h, const double* U3, double* P) {
     size_t S = g.S();
     std::vector<cplx> A[3];
     for (int i = 0; i < 3; ++i) A[i].resize(S);
-    const cplx* U[3] = {(const cplx*)U3, (const cplx*)U3 + S, (const cplx*)U3 + 2 * S};
+    In<cplx> u(U3, 3 * S);
+    Out<cplx> po(P, S);
+    const cplx* U[3] = {u.p, u.p + S, u.p + 2 * S};
     cplx* Ap[3] = {A[0].data(), A[1].data(), A[2].data()};
     p->nl->eval(U, Ap, false);
     const cplx* Ac[3] = {A[0].data(), A[1].data(), A[2].data()};
-    pressure_poisson(g, Ac, U[1], p->nu, (cplx*)P);
+    pressure_poisson(g, Ac, U[1], p->nu, po.p);
     return 0;
 }
 // _pressure_poisson(A, v, nu) directly
@@ -1384,14 +1496,17 @@ int orc_pressure_poisson(int Nx, int Ny, int Nz, double Lx, double Lz, double nu
                          double* P) {
     Grid g(Nx, Ny, Nz, Lx, Lz);
     size_t S = g.S();
-    const cplx* A[3] = {(const cplx*)A3, (const cplx*)A3 + S, (const cplx*)A3 + 2 * S};
-    pressure_poisson(g, A, (const cplx*)v, nu, (cplx*)P);
+    In<cplx> a(A3, 3 * S), vv(v, S);
+    Out<cplx> po(P, S);
+    const cplx* A[3] = {a.p, a.p + S, a.p + 2 * S};
+    pressure_poisson(g, A, vv.p, nu, po.p);
     return 0;
 }
 // spectral dot (postprocessing.jl:31-55) of scalar fields
 double orc_dot(int Nx, int Ny, int Nz, const double* U, const double* V) {
     Grid g(Nx, Ny, Nz, 1.0, 1.0);
-    return spectral_dot(g, (const cplx*)U, (const cplx*)V);
+    In<cplx> u(U, g.S()), v(V, g.S());
+    return (double)spectral_dot(g, u.p, v.p);
 }
 
 }  // extern "C"

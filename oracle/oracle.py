@@ -14,19 +14,26 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libcforacle.so")
+# the same source built with long double arithmetic: the accuracy reference
+# for FP64 results (every function below takes extended=True to use it)
+LIB_X = os.path.join(HERE, "libcforacle_x.so")
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(extended=False):
+    key = bool(extended)
+    if key not in _libs:
+        path = LIB_X if key else LIB
+        if not os.path.exists(path):
             from oracle import build as _b
             _b.build()
-        _lib = C.CDLL(LIB)
-        _declare(_lib)
-    return _lib
+        L = C.CDLL(path)
+        _declare(L)
+        if L.orc_extended() != int(key):
+            raise OracleError(f"{path} is not the {'extended' if key else 'FP64'} build")
+        _libs[key] = L
+    return _libs[key]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -37,6 +44,7 @@ _i = C.c_int
 
 def _declare(L):
     L.orc_last_error.restype = C.c_char_p
+    L.orc_extended.restype = _i
     L.orc_num_threads.restype = _i
     L.orc_set_threads.argtypes = [_i]
     L.orc_cheb.argtypes = [_i, _i, _i, _cp, _cp]
@@ -73,9 +81,9 @@ class OracleError(RuntimeError):
     pass
 
 
-def _check(rc):
+def _check(rc, extended=False):
     if rc != 0:
-        raise OracleError(lib().orc_last_error().decode())
+        raise OracleError(lib(extended).orc_last_error().decode())
 
 
 def set_bulkgrad_mutation(m):
@@ -142,15 +150,15 @@ def _r(a):
 # ---------------------------------------------------------------------------
 # Transforms and operators
 # ---------------------------------------------------------------------------
-def forward(g, u):
+def forward(g, u, extended=False):
     U = g.zeros_spec()
-    _check(lib().orc_forward(*g.args(), _r(u), U))
+    _check(lib(extended).orc_forward(*g.args(), _r(u), U), extended)
     return U
 
 
-def inverse(g, U):
+def inverse(g, U, extended=False):
     u = np.zeros(g.phys_shape)
-    _check(lib().orc_inverse(*g.args(), _c(U), u))
+    _check(lib(extended).orc_inverse(*g.args(), _c(U), u), extended)
     return u
 
 
@@ -199,50 +207,51 @@ def diff(a):
     return out
 
 
-def helmholtz(P, t0, t1, f, gp, gm, neumann=False):
+def helmholtz(P, t0, t1, f, gp, gm, neumann=False, extended=False):
     """Chebyshev-tau solve of rows 0..P-2 of (t0 D^2 - t1) a = f with wall data."""
+    L = lib(extended)
     if np.iscomplexobj(f) or np.iscomplexobj(gp) or np.iscomplexobj(gm):
         f = _c(f)
         out = np.zeros(P + 1, dtype=np.complex128)
-        _check(lib().orc_helmholtz_c(P, t0, t1, 1 if neumann else 0, f, out,
-                                     np.array([complex(gp)]), np.array([complex(gm)])))
+        _check(L.orc_helmholtz_c(P, t0, t1, 1 if neumann else 0, f, out,
+                                 np.array([complex(gp)]), np.array([complex(gm)])), extended)
         return out
     f = _r(f)
     out = np.zeros(P + 1)
-    _check(lib().orc_helmholtz(P, t0, t1, 1 if neumann else 0, f, out, float(gp), float(gm)))
+    _check(L.orc_helmholtz(P, t0, t1, 1 if neumann else 0, f, out, float(gp), float(gm)), extended)
     return out
 
 
-def influence_mode(Ny, k, l, nu, lam, Rx, Ry, Rz, responses=False):
+def influence_mode(Ny, k, l, nu, lam, Rx, Ry, Rz, responses=False, extended=False):
     outs = [np.zeros(Ny, dtype=np.complex128) for _ in range(4)]
     resp = np.zeros((6, Ny)) if responses else None
-    _check(lib().orc_influence_mode(Ny, k, l, nu, lam, _c(Rx), _c(Ry), _c(Rz), *outs,
-                                    resp.ctypes.data if responses else None))
+    _check(lib(extended).orc_influence_mode(Ny, k, l, nu, lam, _c(Rx), _c(Ry), _c(Rz), *outs,
+                                            resp.ctypes.data if responses else None), extended)
     return (outs, resp) if responses else outs
 
 
-def batched_influence(Ny, k, l, nu, lam, Rx, Ry, Rz):
+def batched_influence(Ny, k, l, nu, lam, Rx, Ry, Rz, extended=False):
     """Matrices given as numpy (Ny, B) C-order == (B, Ny) column-major."""
     k, l = _r(k), _r(l)
     B = len(k)
     outs = [np.zeros((Ny, B), dtype=np.complex128) for _ in range(4)]
-    _check(lib().orc_batched_influence(Ny, B, k, l, nu, lam, _c(Rx), _c(Ry), _c(Rz), *outs))
+    _check(lib(extended).orc_batched_influence(Ny, B, k, l, nu, lam, _c(Rx), _c(Ry), _c(Rz), *outs), extended)
     return outs
 
 
-def mean_mode(Ny, nu, lam, Rx, Ry, Rz, bulkvelocity=None, pressuregradient=None):
+def mean_mode(Ny, nu, lam, Rx, Ry, Rz, bulkvelocity=None, pressuregradient=None, extended=False):
     if bulkvelocity is not None and pressuregradient is not None:
         raise ValueError("specify either pressuregradient or bulkvelocity")
     kind, c = (1, bulkvelocity) if bulkvelocity is not None else (0, pressuregradient or (0.0, 0.0))
     outs = [np.zeros(Ny, dtype=np.complex128) for _ in range(4)]
     grads = np.zeros(2)
     resp = np.zeros(Ny)
-    _check(lib().orc_mean_mode(Ny, nu, lam, _c(Rx), _c(Ry), _c(Rz), *outs, kind, float(c[0]), float(c[1]),
-                               grads, resp))
+    _check(lib(extended).orc_mean_mode(Ny, nu, lam, _c(Rx), _c(Ry), _c(Rz), *outs, kind, float(c[0]),
+                                       float(c[1]), grads, resp), extended)
     return outs, tuple(grads), resp
 
 
-def stokes(g, nu, lam, R3, bulkvelocity=None, pressuregradient=None, baseflow=None):
+def stokes(g, nu, lam, R3, bulkvelocity=None, pressuregradient=None, baseflow=None, extended=False):
     """StokesSolver solve!: returns (U3, P, gradients)."""
     if bulkvelocity is not None and pressuregradient is not None:
         raise ValueError("specify either pressuregradient or bulkvelocity")
@@ -251,19 +260,19 @@ def stokes(g, nu, lam, R3, bulkvelocity=None, pressuregradient=None, baseflow=No
     P = g.zeros_spec()
     grads = np.zeros(2)
     base = None if baseflow is None else _r(baseflow)
-    _check(lib().orc_stokes(*g.args(), nu, lam, _c(R3), U3, P, kind, float(c[0]), float(c[1]),
-                            None if base is None else base.ctypes.data, grads))
+    _check(lib(extended).orc_stokes(*g.args(), nu, lam, _c(R3), U3, P, kind, float(c[0]), float(c[1]),
+                                    None if base is None else base.ctypes.data, grads), extended)
     return U3, P, tuple(grads)
 
 
-def pressure_poisson(g, A3, v, nu):
+def pressure_poisson(g, A3, v, nu, extended=False):
     P = g.zeros_spec()
-    _check(lib().orc_pressure_poisson(*g.args(), nu, _c(A3), _c(v), P))
+    _check(lib(extended).orc_pressure_poisson(*g.args(), nu, _c(A3), _c(v), P), extended)
     return P
 
 
-def dot(g, U, V):
-    return lib().orc_dot(g.Nx, g.Ny, g.Nz, _c(U), _c(V))
+def dot(g, U, V, extended=False):
+    return lib(extended).orc_dot(g.Nx, g.Ny, g.Nz, _c(U), _c(V))
 
 
 # ---------------------------------------------------------------------------
@@ -273,7 +282,13 @@ FORMS = {"rotating": 0, "convective": 1, "divergence": 2}
 
 
 class Problem:
-    def __init__(self, g, profile, nu, dt, form="rotating", pressuregradient=None, bulkvelocity=None):
+    """ChannelFlowProblem + CNRK2 + NonLinearTerm of the oracle.  extended=True
+    runs the same algorithm in long double (the accuracy reference); its
+    fields still cross as complex128 and stay extended between the steps of
+    one step() call."""
+
+    def __init__(self, g, profile, nu, dt, form="rotating", pressuregradient=None, bulkvelocity=None,
+                 extended=False):
         if pressuregradient is not None and bulkvelocity is not None:
             raise ValueError("specify either pressuregradient or bulkvelocity")
         self.g = g
@@ -286,68 +301,74 @@ class Problem:
         vals = profile(g.y) if callable(profile) else np.asarray(profile, dtype=float)
         self.baseflow_values = _r(np.broadcast_to(vals, (g.Ny,)))
         self.form = form
-        h = lib().orc_problem_create(*g.args(), self.nu, self.dt, self.baseflow_values, FORMS[form], self.kind,
+        self.extended = bool(extended)
+        self._L = lib(extended)
+        h = self._L.orc_problem_create(*g.args(), self.nu, self.dt, self.baseflow_values, FORMS[form], self.kind,
                                      self.c[0], self.c[1])
         if not h:
-            raise OracleError(lib().orc_last_error().decode())
+            raise OracleError(self._L.orc_last_error().decode())
         self.h = h
         self.base = np.zeros(g.Ny)
         self.curv = np.zeros(g.Ny)
-        lib().orc_base_coeffs(self.h, self.base, self.curv)
+        self._L.orc_base_coeffs(self.h, self.base, self.curv)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise OracleError(self._L.orc_last_error().decode())
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_problem_destroy(self.h)
+            self._L.orc_problem_destroy(self.h)
             self.h = None
 
     def nonlinear(self, U3, N3=None, add=False):
         N3 = self.g.zeros_spec(3) if N3 is None else _c(N3).copy()
-        _check(lib().orc_nonlinear(self.h, _c(U3), N3, 1 if add else 0))
+        self._chk(self._L.orc_nonlinear(self.h, _c(U3), N3, 1 if add else 0))
         return N3
 
     def step(self, U3, P, nsteps=1):
         """Advance copies of (U3, P) by nsteps CNRK2 steps; return them."""
         U3 = _c(U3).copy()
         P = _c(P).copy()
-        _check(lib().orc_step(self.h, U3, P, int(nsteps)))
+        self._chk(self._L.orc_step(self.h, U3, P, int(nsteps)))
         return U3, P
 
     def step_inplace(self, U3, P, nsteps=1):
-        _check(lib().orc_step(self.h, U3, P, int(nsteps)))
+        self._chk(self._L.orc_step(self.h, U3, P, int(nsteps)))
 
     def set_forcing(self, F0=None, F1=None, omega=0.0, t0=0.0, F2=None):
         """forcing(t, U, N) += F0 + sin(omega t) F1 + cos(omega t) F2 at the
         stage times (cnrk2.jl:173-175); F0=None removes it.  t0: time of the
         next step."""
         if F0 is None:
-            _check(lib().orc_set_forcing(self.h, None, None, None, 0.0, float(t0)))
+            self._chk(self._L.orc_set_forcing(self.h, None, None, None, 0.0, float(t0)))
             self._forcing = None
             return
         F0 = _c(np.asarray(F0)).copy()
         F1 = None if F1 is None else _c(np.asarray(F1)).copy()
         F2 = None if F2 is None else _c(np.asarray(F2)).copy()
         self._forcing = (F0, F1, F2)  # keep alive
-        _check(lib().orc_set_forcing(self.h, F0.ctypes.data, None if F1 is None else F1.ctypes.data,
+        self._chk(self._L.orc_set_forcing(self.h, F0.ctypes.data, None if F1 is None else F1.ctypes.data,
                                      None if F2 is None else F2.ctypes.data, float(omega), float(t0)))
 
     def last_gradient(self):
         g2 = np.zeros(2)
-        lib().orc_last_gradient(self.h, g2)
+        self._L.orc_last_gradient(self.h, g2)
         return tuple(g2)
 
     def project(self, U3):
         U3 = _c(U3).copy()
-        _check(lib().orc_project(self.h, U3))
+        self._chk(self._L.orc_project(self.h, U3))
         return U3
 
     def pressure(self, U3):
         P = self.g.zeros_spec()
-        _check(lib().orc_pressure(self.h, _c(U3), P))
+        self._chk(self._L.orc_pressure(self.h, _c(U3), P))
         return P
 
     def random_state(self, noise3):
         """random_state (initialization.jl:32-51) from supplied Gaussian samples
         already scaled by epsilon: shape (3,) + padded physical shape."""
-        U3 = np.stack([forward(self.g, noise3[i]) for i in range(3)])
+        U3 = np.stack([forward(self.g, noise3[i], self.extended) for i in range(3)])
         U3 = self.project(U3)
         return U3, self.pressure(U3)

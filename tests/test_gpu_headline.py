@@ -6,14 +6,20 @@ at N = 32, 64, 128 after one step (test/cuda/runtests.jl:152-175); here the
 transforms and the nonlinear term (FFT lengths 192 in z and a 96-point
 half-length x transform) are held to 1e-13 / 1e-12, and one and five steps
 of U, P and the bulk gradient to the oracle within its roundoff envelope
-(8 perturbation trials).  One-step velocity is also held to a flat 1e-12."""
+(8 perturbation trials).  At this shape the oracle's own output moves by
+~1e-10 in U and ~1e-9 in P when its inputs are perturbed at 1e-16
+(lambda/nu = 3.4e6 in the first stage), so a flat 1e-12 is out of reach of
+any FP64 implementation; after one step the device must instead be at least
+as accurate as the FP64 oracle (up to 2x) against the extended-precision
+oracle, the same algorithm run in long double on the same inputs."""
 import math
 
 import numpy as np
 import pytest
 
+from accuracy import report
 from oracle import oracle as O
-from test_gpu_parity import ENVELOPE_FACTOR, TOL, _split, assert_parity, rel, roundoff_envelope, say
+from test_gpu_parity import ENVELOPE_FACTOR, _split, assert_parity, rel, roundoff_envelope, say
 
 pytestmark = pytest.mark.gpu
 
@@ -85,9 +91,11 @@ def test_headline_steps(headline, nsteps):
     Uh, Ph = U.cpu().numpy(), P.cpu().numpy()
     assert_parity(f"headline {nsteps} steps", _split((Uh, Ph)), _split((U1, P1)), env)
     if nsteps == 1:
-        e = rel(Uh, U1)
-        say(f"headline 1 step: velocity rel err {e:.3e} (flat bound {TOL:.0e})")
-        assert e <= TOL
+        ext = O.Problem(go, base, NU, DT, bulkvelocity=(1.0, 0.0), extended=True)
+        XU, XP = ext.step(U0, P0, 1)
+        ehU, eoU = report(say, "headline U after 1", Uh, U1, XU, go)
+        ehP, eoP = report(say, "headline P after 1", Ph, P1, XP, go)
+        assert ehU <= 2.0 * eoU and ehP <= 2.0 * eoP, (ehU, eoU, ehP, eoP)
     dg = max(abs(a - b) / abs(b) if b else abs(a) for a, b in zip(grad, gref))
     say(f"headline {nsteps} steps: gradient {grad} oracle {gref} rel {dg:.3e}")
     assert dg <= max(1e-10, ENVELOPE_FACTOR * env[0]), dg

@@ -300,6 +300,28 @@ def test_flow_shortened_final_step(gpu, span_steps):
                   _split((U1, P1)), env)
 
 
+def test_flow_remainder_banks_bounded(gpu):
+    """Each distinct shortened final step needs stage solvers of its own
+    duration; a caller varying the span continuously (a Newton search over
+    T) must not grow device memory without bound.  16 spans with distinct
+    remainders at 64x97x64 (about 10 MB of solver banks per remainder)."""
+    import torch
+    cf = gpu
+    dims, dt = (64, 97, 64), 0.01
+    gc, go, prob = make(cf, dims, L=(2 * np.pi / 1.14, 2 * np.pi / 2.5), dt=dt)
+    U = torch.zeros((3,) + gc.spec_shape, dtype=torch.complex128, device="cuda")
+    P = torch.zeros(gc.spec_shape, dtype=torch.complex128, device="cuda")
+    st = cf.State(U, P)
+    free = []
+    for i in range(16):
+        cf.flow(prob)(st, (0.0, dt * (1.0 + 0.05 + 0.03 * i)))
+        torch.cuda.synchronize()
+        free.append(torch.cuda.mem_get_info()[0])
+    growth = free[3] - free[-1]
+    say(f"device memory after 4 and 16 distinct remainders: growth {growth / 2**20:.1f} MiB")
+    assert growth < 24 * 2**20, growth
+
+
 @pytest.mark.parametrize("dims", [(8, 17, 8), (9, 17, 9), (16, 33, 12), (10, 97, 12)])
 def test_vector_transforms_equal_single_field(gpu, dims):
     """cf_forward3 / cf_inverse3 (one launch per pass for three fields) give
