@@ -17,7 +17,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcfhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels_y.hip", "kernels_xz.hip", "kernels_rng.hip", "kernels_closure.hip", "setup.cpp", "api.cpp"]
+SOURCES = ["kernels_y.hip", "kernels_xz.hip", "kernels_rng.hip", "kernels_closure.hip", "scratch.hip", "setup.cpp",
+           "api.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -40,7 +40,7 @@ _lib = None
 SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_problem_sizes", "cf_step", "cf_advance",
            "cf_step_timed",
            "cf_last_gradient", "cf_forward", "cf_inverse", "cf_nonlinear", "cf_stokes_solve", "cf_project",
-           "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input", "cf_plane_statistics", "cf_advance_to", "cf_gaussian_noise", "cf_random_state",
+           "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input", "cf_plane_statistics", "cf_advance_to", "cf_advance_to_noalloc", "cf_gaussian_noise", "cf_random_state",
            "cf_ensemble_create", "cf_ensemble_advance", "cf_ensemble_info", "cf_ensemble_destroy", "cf_set_forcing", "cf_set_nonlinear",
            "cf_velocity_gradient", "cf_box_filter", "cf_mlp_features", "cf_mlp_infer", "cf_mlp_work_floats",
            "cf_mlp_grad", "cf_mlp_adam", "cf_problem_tune", "cf_problem_tiles", "cf_forward3", "cf_inverse3"]
@@ -78,6 +78,7 @@ def load(path=LIB_PATH):
     L.cf_power_input.argtypes = [vp, vp, vp, d, dp, dp, vp]
     L.cf_plane_statistics.argtypes = [vp, vp, vp, vp, dp, vp]
     L.cf_advance_to.argtypes = [vp, vp, vp, vp, vp, d, d, i, dp, dp, vp]
+    L.cf_advance_to_noalloc.argtypes = [vp, vp, vp, vp, vp, d, d, i, dp, dp, vp]
     L.cf_gaussian_noise.argtypes = [vp, d, C.c_ulonglong, vp, vp]
     L.cf_random_state.argtypes = [vp, d, C.c_ulonglong, vp, vp, vp, vp, vp]
     L.cf_ensemble_create.argtypes = [C.POINTER(vp), C.POINTER(vp), i, i, C.POINTER(vp)]

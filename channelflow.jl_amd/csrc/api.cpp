@@ -733,8 +733,18 @@ extern "C" int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, dou
 // Flows.step! adapter semantics (ext/ChannelFlowFlowsExt.jl:36-52): nominal
 // steps of dt from t while they fit in T, then one shortened step of the
 // remaining duration with stage solvers built for it (cached per duration)
-extern "C" int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T,
-                             int constraint, const double* values, double* t_out, void* stream) {
+// doubles of one stage bank laid out in the fixed scratch: facV (B Ny),
+// mconst (B MC_COUNT), resp (Ny), each segment 256-byte aligned
+static long long scratch_bank_doubles(const GridDims& gd) {
+    auto up = [](long long n) { return (n + 31) / 32 * 32; };
+    return up(gd.B * (long long)gd.Ny) + up(gd.B * (long long)MC_COUNT) + up(gd.Ny);
+}
+
+// cf_advance_to; noalloc: the shortened step's three stage banks are built on
+// the host and copied into the fixed device scratch instead of the handle's
+// bank cache (no device allocation)
+static int advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T, int constraint,
+                      const double* values, double* t_out, void* stream, bool noalloc) {
     if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
     const void* f[4] = {U0, U1, U2, P};
     if (int rc = check_fields(f, 4)) return rc;
@@ -749,15 +759,65 @@ extern "C" int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, 
     double rem = span - (double)n * h->dt;
     if (rem <= tol) rem = 0.0;
     if (n > (1ll << 30)) return fail(CF_ERR_ARGUMENT, "too many steps");
+    // everything that can fail is checked before the first step is enqueued
+    const long long per_bank = scratch_bank_doubles(h->gd);
+    HostBank hb[3];
+    double* scratch = nullptr;
+    if (rem > 0.0 && noalloc) {
+        if (3 * per_bank > SCRATCH_DOUBLES)
+            return fail(CF_ERR_UNSUPPORTED, "the shortened step's solver banks (" + std::to_string(3 * per_bank * 8 >> 20) +
+                                                " MiB) exceed the fixed device scratch of " +
+                                                std::to_string(SCRATCH_BYTES >> 20) + " MiB; use cf_advance_to");
+        for (int j = 0; j < 3; ++j) {
+            std::string err;
+            if (!build_bank(h->gd, h->nu, 1.0 / (CNRK2_C[j] * rem), hb[j], err)) return fail(CF_ERR_ARGUMENT, err);
+        }
+        HIPTRY(scratch_address(h->device, &scratch));
+    }
     if (int rc = enqueue_steps(h, U, (double2*)P, (int)n, c, st, nullptr, 0.0, nullptr, t)) return rc;
     if (rem > 0.0) {
         DevBank* b[3];
-        for (int j = 0; j < 3; ++j)
-            if (int rc = get_bank(h, h->nu, 1.0 / (CNRK2_C[j] * rem), &b[j])) return rc;
+        DevBank sb[3];
+        if (noalloc) {
+            // stream order: the copies follow every earlier use of the scratch on
+            // this stream; the wait lets the host banks go out of scope
+            for (int j = 0; j < 3; ++j) {
+                double* base = scratch + j * per_bank;
+                const long long o1 = (h->gd.B * (long long)h->gd.Ny + 31) / 32 * 32;
+                const long long o2 = o1 + (h->gd.B * (long long)MC_COUNT + 31) / 32 * 32;
+                HIPTRY(hipMemcpyAsync(base, hb[j].facV.data(), hb[j].facV.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, st));
+                HIPTRY(hipMemcpyAsync(base + o1, hb[j].mconst.data(), hb[j].mconst.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, st));
+                HIPTRY(hipMemcpyAsync(base + o2, hb[j].resp.data(), hb[j].resp.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, st));
+                sb[j].facV = base;
+                sb[j].mconst = base + o1;
+                sb[j].resp = base + o2;
+                sb[j].resp_mean = hb[j].resp_mean;
+                sb[j].lambda = hb[j].lambda;
+                sb[j].nu = hb[j].nu;
+                b[j] = &sb[j];
+            }
+            HIPTRY(hipStreamSynchronize(st));
+        } else {
+            for (int j = 0; j < 3; ++j)
+                if (int rc = get_bank(h, h->nu, 1.0 / (CNRK2_C[j] * rem), &b[j])) return rc;
+        }
         if (int rc = enqueue_steps(h, U, (double2*)P, 1, c, st, nullptr, rem, b, t + (double)n * h->dt)) return rc;
     }
     if (t_out) *t_out = T;
     return CF_OK;
+}
+
+extern "C" int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T,
+                             int constraint, const double* values, double* t_out, void* stream) {
+    return advance_to(h, U0, U1, U2, P, t, T, constraint, values, t_out, stream, false);
+}
+
+extern "C" int cf_advance_to_noalloc(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T,
+                                     int constraint, const double* values, double* t_out, void* stream) {
+    return advance_to(h, U0, U1, U2, P, t, T, constraint, values, t_out, stream, true);
 }
 
 // Velocity-gradient tensor of the total velocity V = U + Ub e_x on the padded

@@ -176,4 +176,11 @@ hipError_t launch_xpass_batch(int mode, int ppt, int waves, const XArgs* tab, co
 unsigned z_blocks(const ZArgs& a);  // grid of one instance's z pass
 unsigned x_blocks(const XArgs& a);  // grid of one instance's even-Nxp x pass
 
+// Fixed device scratch of the entry points that allocate nothing (32 MiB on
+// every GPU the library runs on, held until the process ends; scratch.hip).
+constexpr long long SCRATCH_BYTES = 32ll << 20;
+constexpr long long SCRATCH_DOUBLES = SCRATCH_BYTES / 8;
+// its device address on `device` (looked up once per device, then cached)
+hipError_t scratch_address(int device, double** out);
+
 }  // namespace cf

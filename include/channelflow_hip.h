@@ -98,6 +98,21 @@ int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, double* P, dou
 int cf_advance_to(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T, int constraint,
                   const double* values, double* t_out, void* stream);
 
+/* cf_advance_to that allocates no device memory (no counterpart in the
+ * reference).  For callers that cannot call hipMalloc: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation, e.g. a
+ * Newton or periodic-orbit search that varies T on every call.  Same
+ * arguments and results (bit for bit) as cf_advance_to.  The shortened final
+ * step's three stage solver banks are built on the host and copied into a
+ * fixed 32 MiB device array that every GPU the library runs on holds until
+ * the process ends; the call then waits for `stream` once (the copies).
+ * Grids whose three banks exceed it (3 x 8 x (B (Ny + 16) + Ny) bytes,
+ * B = (Nx/2+1) Nz; e.g. 128x97x128 needs 22.6 MiB) return CF_ERR_UNSUPPORTED
+ * before anything is enqueued.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (other streams, other handles) race. */
+int cf_advance_to_noalloc(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, double T,
+                          int constraint, const double* values, double* t_out, void* stream);
+
 /* Measurement variant of cf_step (no counterpart in the reference): one step
  * with HIP events recorded on `stream` around every launch; waits for the
  * step, then ms[0..4] = device milliseconds spent in the prologue (y), the
