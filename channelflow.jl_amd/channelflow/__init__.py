@@ -27,7 +27,7 @@ __all__ = ["Grid", "ChannelFlowProblem", "CouetteFlow", "PoiseuilleFlow", "State
            "step_", "project_", "pressure", "FFT", "IFFT", "velocity", "stagepressure", "kinetic_energy",
            "require_device", "RotatingForm", "ConvectiveForm", "DivergenceForm", "dot", "norm", "dissipation_rate",
            "power_input", "plane_statistics", "ProfileStatistics", "flow", "device_gaussian_noise",
-           "savefield", "loadfield", "Ensemble", "velocity_gradient", "box_filter", "tune"]
+           "savefield", "loadfield", "Ensemble", "velocity_gradient", "box_filter", "tune", "get_tiles", "set_tiles"]
 
 # nonlinear forms (src/nonlinear.jl:214-224)
 RotatingForm = "rotating"
@@ -439,6 +439,28 @@ def tune(problem, stream=None):
     out = (C.c_int * 3)()
     _lib.check(_lib.load().cf_problem_tiles(problem.scheme.h, out))
     return tuple(out)
+
+
+X_VARIANTS = {"six": 0, "split": 1, "async": 2}  # x pass variants of the rotating form (even padded Nx)
+
+
+def get_tiles(problem):
+    """(z tile width, z points per thread, x pass variant, x wavefronts per
+    block) of the problem's FP64 step (cf_problem_get_tiles)."""
+    out = (C.c_int * 4)()
+    _lib.check(_lib.load().cf_problem_get_tiles(problem.scheme.h, out))
+    return tuple(out)
+
+
+def set_tiles(problem, z_width, x_variant, x_waves):
+    """Set the FP64 step's z tile width, x pass variant (0/"six", 1/"split",
+    2/"async") and x wavefronts per block (cf_problem_set_tiles); an
+    unavailable choice raises and leaves the tiling as it was.  Results of the
+    step are identical for any valid tiling."""
+    v = X_VARIANTS.get(x_variant, x_variant)
+    arr = (C.c_int * 4)(int(z_width), 0, int(v), int(x_waves))
+    _lib.check(_lib.load().cf_problem_set_tiles(problem.scheme.h, arr))
+    return get_tiles(problem)
 
 
 def last_gradient(scheme):

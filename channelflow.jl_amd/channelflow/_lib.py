@@ -43,7 +43,8 @@ SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_probl
            "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input", "cf_plane_statistics", "cf_advance_to", "cf_advance_to_noalloc", "cf_gaussian_noise", "cf_random_state",
            "cf_ensemble_create", "cf_ensemble_advance", "cf_ensemble_info", "cf_ensemble_destroy", "cf_set_forcing", "cf_set_nonlinear",
            "cf_velocity_gradient", "cf_box_filter", "cf_mlp_features", "cf_mlp_infer", "cf_mlp_work_floats",
-           "cf_mlp_grad", "cf_mlp_adam", "cf_problem_tune", "cf_problem_tiles", "cf_forward3", "cf_inverse3"]
+           "cf_mlp_grad", "cf_mlp_adam", "cf_problem_tune", "cf_problem_tiles", "cf_problem_get_tiles", "cf_problem_set_tiles",
+           "cf_forward3", "cf_inverse3"]
 
 # int (*cf_forcing_fn)(void* user, double tstage, const double* U0, U1, U2, double* F0, F1, F2, void* stream)
 FORCING_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -96,6 +97,8 @@ def load(path=LIB_PATH):
     L.cf_mlp_adam.argtypes = [vp, vp, vp, vp, vp, C.c_float, i, vp]
     L.cf_problem_tune.argtypes = [vp, vp]
     L.cf_problem_tiles.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cf_problem_get_tiles.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cf_problem_set_tiles.argtypes = [vp, C.POINTER(C.c_int)]
     L.cf_forward3.argtypes = [vp, vp, vp, vp, vp, vp]
     L.cf_inverse3.argtypes = [vp, vp, vp, vp, vp, vp]
     for name in SYMBOLS:

@@ -59,6 +59,11 @@ struct DevBank {
     }
 };
 
+struct Constraint {
+    int bulk;
+    double c0, c1;
+};
+
 struct cf_problem {
     int device = 0;
     GridDims gd{};
@@ -102,11 +107,27 @@ struct cf_problem {
     int KT32 = 1, zppt32 = 0, xppt32 = 0, xwaves32 = 0;
     // velocity-gradient x pass (12 FFT lines): in-place width, wavefronts per block
     int xpptg = 0, xwavesg = 0;
-    // rotating form, FP64: split x pass (X_ROT_SPLIT, three LDS lines) in use,
-    // its in-place width and wavefronts per block
-    int xsplit = 0, xppts = 0, xwavess = 0;
+    // rotating form, FP64: x pass variant in use (0 six-line X_ROTATIONAL,
+    // 1 split X_ROT_SPLIT with three LDS lines, 2 X_ROT_ASYNC: six lines with
+    // direct-to-LDS loads), the split pass's in-place width and wavefronts
+    // per block, and the async pass's wavefronts per block (its width is xppt)
+    int xsplit = 0, xppts = 0, xwavess = 0, xwavesa = 0;
+    // largest wavefronts per block that fit the LDS, per variant (0: unavailable)
+    int xmaxw[3] = {0, 0, 0};
     // the same for the mixed step's FP32 x pass
     int xsplit32 = 0, xppts32 = 0, xwavess32 = 0;
+    // one-step graphs of the plain step (no host callbacks), one per field
+    // pointer set and constraint, least recently used dropped past GRAPH_CACHE;
+    // cleared when the tiling changes
+    struct StepGraph {
+        const void* f[4];
+        Constraint c;
+        hipGraphExec_t exec;
+        unsigned long long used;
+    };
+    std::vector<StepGraph> graphs;
+    unsigned long long graph_clock = 0;
+    hipStream_t cap = nullptr;  // capture stream (created with the first graph)
 };
 
 struct DeviceGuard {
@@ -216,7 +237,24 @@ static int get_bank(cf_problem* h, double nu, double lambda, DevBank** out) {
     return 0;
 }
 
+static void drop_graphs(cf_problem* h) {
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
+// the tiling changed: cached step graphs hold the old launch shapes (they may
+// still run, so the device is drained first)
+static int invalidate_graphs(cf_problem* h) {
+    if (h->graphs.empty()) return 0;
+    if (hipDeviceSynchronize() != hipSuccess) return fail(CF_ERR_DEVICE, "hipDeviceSynchronize");
+    drop_graphs(h);
+    return 0;
+}
+
 static void free_all(cf_problem* h) {
+    drop_graphs(h);
+    if (h->cap) (void)hipStreamDestroy(h->cap);
+    h->cap = nullptr;
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
 }
@@ -358,7 +396,11 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
             xs.ppt = h->xppts;
             h->xwavess = h->xppts ? x_max_waves(xs) : 0;
             h->xsplit = h->xwavess > 0 ? 1 : 0;
+            h->xwavesa = h->xppt ? h->xwaves : 0;  // same LDS lines as the six-line in-place pass
         }
+        h->xmaxw[0] = h->xwaves;
+        h->xmaxw[1] = h->xwavess;
+        h->xmaxw[2] = h->xwavesa;
         {
             XArgs xg = xa;
             xg.mode = X_GRADIENT;
@@ -526,9 +568,15 @@ static NonlinearArgs nonlinear_args(cf_problem* h, bool mixed) {
     na.x = base_xargs(h, mixed);
     XArgs& x = na.x;
     if (h->form == CF_FORM_ROTATING) {
-        na.zin = make_zargs(h, 6, six, zb, 0, mixed);
+        // z field 4 is i beta l V_x (dV_x/dz); the x pass forms Omega_y from
+        // it and V_z, so the y kernels emit five y-physical fields
+        double2* zin[6] = {six[0], six[1], six[2], six[3], six[0], six[5]};
+        na.zin = make_zargs(h, 6, zin, zb, 0x10u, mixed);
         x.mode = X_ROTATIONAL;
-        if (mixed ? h->xsplit32 : h->xsplit) {
+        if (!mixed && h->xsplit == 2) {
+            x.mode = X_ROT_ASYNC;
+            x.waves = h->xwavesa;
+        } else if (mixed ? h->xsplit32 : h->xsplit) {
             x.mode = X_ROT_SPLIT;
             x.ppt = mixed ? h->xppts32 : h->xppts;
             x.waves = mixed ? h->xwavess32 : h->xwavess;
@@ -586,11 +634,6 @@ static int enqueue_prologue(cf_problem* h, double2* const U[3], hipStream_t st, 
     HIPTRY(launch_ykernel(Y_PROLOGUE, prologue_args(h, U, mixed), st));
     return 0;
 }
-
-struct Constraint {
-    int bulk;
-    double c0, c1;
-};
 
 static int resolve_constraint(cf_problem* h, int kind, const double* values, Constraint& c) {
     if (kind == -1) {
@@ -714,6 +757,59 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
     return 0;
 }
 
+// Plain steps (no host callbacks) replayed from a cached one-step graph: one
+// hipGraphLaunch per step instead of the step's 13 kernel launches.  The graph
+// holds kernel nodes only (no memset or copy nodes) with the arguments of
+// these field pointers and this constraint.
+static constexpr size_t GRAPH_CACHE = 4;
+
+static int graph_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st) {
+    const void* key[4] = {U[0], U[1], U[2], P};
+    cf_problem::StepGraph* g = nullptr;
+    for (auto& e : h->graphs)
+        if (std::equal(key, key + 4, e.f) && e.c.bulk == c.bulk && e.c.c0 == c.c0 && e.c.c1 == c.c1) g = &e;
+    if (!g) {
+        if (!h->cap && hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking) != hipSuccess)
+            return fail(CF_ERR_DEVICE, "hipStreamCreate");
+        HIPTRY(hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
+        const int rc = enqueue_steps(h, U, P, 1, c, h->cap);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(h->cap, &graph);
+        hipGraphExec_t exec = nullptr;
+        hipError_t ei = hipSuccess;
+        if (!rc && ec == hipSuccess) ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        if (ec != hipSuccess) return fail(CF_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        if (ei != hipSuccess) return fail(CF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        if (h->graphs.size() >= GRAPH_CACHE) {
+            // the evicted graph may still run on some stream: wait for the device
+            auto lru = std::min_element(h->graphs.begin(), h->graphs.end(),
+                                        [](const auto& a, const auto& b) { return a.used < b.used; });
+            HIPTRY(hipDeviceSynchronize());
+            (void)hipGraphExecDestroy(lru->exec);
+            h->graphs.erase(lru);
+        }
+        cf_problem::StepGraph e;
+        std::copy(key, key + 4, e.f);
+        e.c = c;
+        e.exec = exec;
+        h->graphs.push_back(e);
+        g = &h->graphs.back();
+    }
+    g->used = ++h->graph_clock;
+    for (int s = 0; s < nsteps; ++s) HIPTRY(hipGraphLaunch(g->exec, st));
+    return 0;
+}
+
+// nominal steps of the problem's dt: graph replay when no host callback runs
+// inside the step, else launched one by one
+static int run_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
+                     double t0, bool graph) {
+    if (graph && nsteps > 0 && !h->force_fn && !h->nl_fn) return graph_steps(h, U, P, nsteps, c, st);
+    return enqueue_steps(h, U, P, nsteps, c, st, nullptr, 0.0, nullptr, t0);
+}
+
 extern "C" int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, double* P, double t, int nsteps,
                           int constraint, const double* values, double* t_out, void* stream) {
     if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
@@ -725,7 +821,7 @@ extern "C" int cf_advance(cf_problem* h, double* U0, double* U1, double* U2, dou
     if (int rc = resolve_constraint(h, constraint, values, c)) return rc;
     DeviceGuard dg(h->device);
     double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
-    if (int rc = enqueue_steps(h, U, (double2*)P, nsteps, c, (hipStream_t)stream, nullptr, 0.0, nullptr, t)) return rc;
+    if (int rc = run_steps(h, U, (double2*)P, nsteps, c, (hipStream_t)stream, t, true)) return rc;
     if (t_out) *t_out = t + nsteps * h->dt;
     return CF_OK;
 }
@@ -774,7 +870,8 @@ static int advance_to(cf_problem* h, double* U0, double* U1, double* U2, double*
         }
         HIPTRY(scratch_address(h->device, &scratch));
     }
-    if (int rc = enqueue_steps(h, U, (double2*)P, (int)n, c, st, nullptr, 0.0, nullptr, t)) return rc;
+    // instantiating a graph allocates: the allocation-free form launches one by one
+    if (int rc = run_steps(h, U, (double2*)P, (int)n, c, st, t, !noalloc)) return rc;
     if (rem > 0.0) {
         DevBank* b[3];
         DevBank sb[3];
@@ -1378,6 +1475,7 @@ extern "C" int cf_random_state(cf_problem* h, double epsilon, unsigned long long
 extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
     if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
     DeviceGuard dg(h->device);
+    if (int rc = invalidate_graphs(h)) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t a, b;
     HIPTRY(hipEventCreate(&a));
@@ -1434,13 +1532,16 @@ extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
     // x pass: wavefronts per block (even padded Nx), and for the rotating
     // form the six-line or the split (three-line) variant
     if (!rc && h->gd.Nxp % 2 == 0) {
-        const int maxw = h->xwaves, maxws = h->xwavess, keepSplit = h->xsplit;
-        int bestW = maxw, bestWs = maxws, bestSplit = keepSplit;
+        const int keepSplit = h->xsplit;
+        int bestW[3] = {h->xwaves, h->xwavess, h->xwavesa}, bestSplit = keepSplit;
+        int* waves[3] = {&h->xwaves, &h->xwavess, &h->xwavesa};
+        const int* maxv = h->xmaxw;
         best = 1e30f;
-        for (int split = 0; split <= (maxws > 0 ? 1 : 0) && !rc; ++split) {
+        for (int split = 0; split < 3 && !rc; ++split) {
+            if (maxv[split] < 1 || (split > 0 && h->form != CF_FORM_ROTATING)) continue;
             h->xsplit = split;
-            for (int w = 1; w <= (split ? maxws : maxw) && !rc; ++w) {
-                (split ? h->xwavess : h->xwaves) = w;
+            for (int w = 1; w <= maxv[split] && !rc; ++w) {
+                *waves[split] = w;
                 const NonlinearArgs na = nonlinear_args(h, false);
                 float ms;
                 rc = timed([&] {
@@ -1450,13 +1551,14 @@ extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
                 if (!rc && ms < best) {
                     best = ms;
                     bestSplit = split;
-                    (split ? bestWs : bestW) = w;
+                    bestW[split] = w;
                 }
             }
         }
         h->xsplit = rc ? keepSplit : bestSplit;
-        h->xwaves = bestW;
-        h->xwavess = bestWs;
+        h->xwaves = bestW[0];
+        h->xwavess = bestW[1];
+        h->xwavesa = bestW[2];
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
@@ -1467,7 +1569,49 @@ extern "C" int cf_problem_tiles(const cf_problem* h, int out[3]) {
     if (!h || !out) return fail(CF_ERR_ARGUMENT, "null argument");
     out[0] = h->KT;
     out[1] = h->zppt;
-    out[2] = h->xsplit ? h->xwavess : h->xwaves;  // of the x pass variant in use
+    out[2] = h->xsplit == 2 ? h->xwavesa : h->xsplit ? h->xwavess : h->xwaves;  // of the x pass variant in use
+    return CF_OK;
+}
+
+extern "C" int cf_problem_get_tiles(const cf_problem* h, int tiles[4]) {
+    if (!h || !tiles) return fail(CF_ERR_ARGUMENT, "null argument");
+    tiles[0] = h->KT;
+    tiles[1] = h->zppt;
+    tiles[2] = h->xsplit;
+    tiles[3] = h->gd.Nxp % 2 ? 0 : h->xsplit == 2 ? h->xwavesa : h->xsplit ? h->xwavess : h->xwaves;
+    return CF_OK;
+}
+
+extern "C" int cf_problem_set_tiles(cf_problem* h, const int tiles[4]) {
+    if (!h || !tiles) return fail(CF_ERR_ARGUMENT, "null argument");
+    // z tile: an in-place width (one LDS buffer) or, for plans that need two
+    // buffers, a width whose two buffers fit 64 KB
+    const int kt = tiles[0];
+    if (kt < 1 || kt > std::min(32, h->gd.Nxh)) return fail(CF_ERR_ARGUMENT, "z tile width out of range");
+    ZArgs z;
+    std::memset(&z, 0, sizeof(z));
+    z.gd = h->gd;
+    z.KT = kt;
+    z.ld = h->zld;
+    z.ppt = z_inplace_ppt(h->pz, kt);
+    if (!z.ppt && kt > 16) return fail(CF_ERR_ARGUMENT, "two-buffer z tiles hold at most 16 columns");
+    if (z_lds_bytes(z) > 64 * 1024) return fail(CF_ERR_ARGUMENT, "z tile exceeds 64 KB of LDS");
+    const int var = tiles[2], w = tiles[3];
+    if (h->gd.Nxp % 2 == 0) {
+        if (var < 0 || var > 2 || (var > 0 && h->form != CF_FORM_ROTATING) || h->xmaxw[var] < 1)
+            return fail(CF_ERR_ARGUMENT, "x pass variant unavailable for this problem");
+        if (w < 1 || w > h->xmaxw[var]) return fail(CF_ERR_ARGUMENT, "x wavefronts per block out of range");
+    } else if (var != 0) {
+        return fail(CF_ERR_ARGUMENT, "odd padded Nx has one x pass");
+    }
+    DeviceGuard dg(h->device);
+    if (int rc = invalidate_graphs(h)) return rc;
+    h->KT = kt;
+    h->zppt = z.ppt;
+    if (h->gd.Nxp % 2 == 0) {
+        h->xsplit = var;
+        (var == 2 ? h->xwavesa : var == 1 ? h->xwavess : h->xwaves) = w;
+    }
     return CF_OK;
 }
 
@@ -1709,7 +1853,7 @@ static int ens_tune(cf_ensemble* e) {
         // x wavefronts per block (LDS permitting)
         best = 1e30f;
         int wkeep = e->tiles[members[0]].xwaves;
-        for (int w = 1; w <= h->xwaves; w *= 2) {
+        for (int w = 1; w <= h->xmaxw[0]; w *= 2) {
             for (int i : members) e->tiles[i].xwaves = w;
             float ms;
             if (int rc = ens_time_class(e, B_X, &ms)) return rc;

@@ -19,7 +19,11 @@ namespace cf {
 __device__ __forceinline__ double2 cmk(double r, double i) { return make_double2(r, i); }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return cmk(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// products written as explicit fused multiply-adds: a product rounds the same
+// way in every kernel it is inlined into, whatever else the compiler contracts
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return cmk(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
 __device__ __forceinline__ double2 cconj(double2 a) { return cmk(a.x, -a.y); }
 __device__ __forceinline__ double2 cscale(double2 a, double s) { return cmk(a.x * s, a.y * s); }
 __device__ __forceinline__ double2 cmuli(double2 a) { return cmk(-a.y, a.x); }    // i*a
@@ -31,7 +35,7 @@ __device__ __forceinline__ double2 czero() { return cmk(0.0, 0.0); }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
@@ -625,8 +629,8 @@ __device__ __forceinline__ void stockham_pass_fixed(const C* src, C* dst, int n,
 
 // generic odd radix (rare sizes): O(R^2) per butterfly, reads LDS directly
 template <typename C>
-__device__ __noinline__ void stockham_pass_generic(const C* src, C* dst, int n, int R, int nlines, int ld, int Ns,
-                                                   const C* tw, int sign, int tid, int nthr) {
+__device__ __forceinline__ void stockham_pass_generic(const C* src, C* dst, int n, int R, int nlines, int ld, int Ns,
+                                                      const C* tw, int sign, int tid, int nthr) {
     const int m = n / R;
     const int twstep = n / (Ns * R);
     const int total = nlines * m;
@@ -651,6 +655,13 @@ __device__ __noinline__ void stockham_pass_generic(const C* src, C* dst, int n, 
             d[idxD + q * Ns] = acc;
         }
     }
+}
+// the same pass as an out-of-line call: keeps the rare path's registers out
+// of kernels whose register budget is tight elsewhere
+template <typename C>
+__device__ __noinline__ void stockham_pass_generic_call(const C* src, C* dst, int n, int R, int nlines, int ld, int Ns,
+                                                        const C* tw, int sign, int tid, int nthr) {
+    stockham_pass_generic(src, dst, n, R, nlines, ld, Ns, tw, sign, tid, nthr);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -689,7 +700,10 @@ __device__ __forceinline__ void wave_sync() {
 // (passes end in __syncthreads); WAVE = true: the lines are private to one
 // wavefront, tid/nthr are its lane/64 and passes end in wave_sync().
 // tw: twiddle table exp(-2 pi i t/n) (global or LDS).
-template <bool WAVE = false, typename C>
+// GENERIC_INLINE: the generic-radix pass is inlined (no call: a call forces
+// registers live across it to be saved, which at a tight register budget
+// means scratch) rather than called out of line.
+template <bool WAVE = false, bool GENERIC_INLINE = false, typename C>
 __device__ __forceinline__ C* lds_fft(C* a, C* b, const FFTPlan& plan, const C* tw, int nlines, int ld, int sign, int tid,
                                       int nthr) {
     int Ns = 1;
@@ -704,7 +718,10 @@ __device__ __forceinline__ C* lds_fft(C* a, C* b, const FFTPlan& plan, const C* 
             case 4: stockham_pass_fixed<4>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
             case 5: stockham_pass_fixed<5>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
             case 8: stockham_pass_fixed<8>(src, dst, plan.n, nlines, ld, Ns, mm, mn, tw, sign, tid, nthr); break;
-            default: stockham_pass_generic(src, dst, plan.n, R, nlines, ld, Ns, tw, sign, tid, nthr); break;
+            default:
+                if constexpr (GENERIC_INLINE) stockham_pass_generic(src, dst, plan.n, R, nlines, ld, Ns, tw, sign, tid, nthr);
+                else stockham_pass_generic_call(src, dst, plan.n, R, nlines, ld, Ns, tw, sign, tid, nthr);
+                break;
         }
         if constexpr (WAVE) wave_sync();
         else __syncthreads();
@@ -794,7 +811,7 @@ __host__ __device__ inline bool fft_inplace_fits(const FFTPlan& plan, int nlines
 
 __device__ __forceinline__ double2* lds_fft(double2* a, double2* b, const FFTPlan& plan, int nlines, int ld, int sign, int tid,
                                             int nthr) {
-    return lds_fft<false, double2>(a, b, plan, plan.tw, nlines, ld, sign, tid, nthr);
+    return lds_fft<false, false, double2>(a, b, plan, plan.tw, nlines, ld, sign, tid, nthr);
 }
 
 }  // namespace cf

@@ -87,7 +87,10 @@ struct ZArgs {
 // X_ROT_SPLIT: the rotating form's products with the velocity lines
 // transformed first and held in registers, so the LDS region holds three
 // lines instead of six (more wavefronts per CU)
-enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD, X_CONVECTIVE, X_DIVERGENCE, X_GRADIENT, X_ROT_SPLIT };
+// X_ROT_ASYNC: the rotating pass of X_ROTATIONAL (six LDS lines, in place)
+// whose HBM loads go straight to LDS (global_load_lds), all six fields in
+// flight at once; FP64 only
+enum { X_ROTATIONAL = 0, X_INVERSE, X_FORWARD, X_CONVECTIVE, X_DIVERGENCE, X_GRADIENT, X_ROT_SPLIT, X_ROT_ASYNC };
 
 struct XArgs {
     GridDims gd;

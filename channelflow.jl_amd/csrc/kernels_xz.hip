@@ -10,7 +10,10 @@
 // Nxp/2+1, inverse real-transformed (brfft), the rotational product u x w is
 // formed on the padded points (nonlinear.jl:187-189) and forward
 // real-transformed (rfft), keeping k < Nxh.  Real transforms of even length
-// use a half-length complex FFT with the standard pack/unpack.
+// use a half-length complex FFT with the standard pack/unpack.  For the
+// rotating form the vorticity component Omega_y = dV_x/dz - dV_z/dx is
+// assembled here in k-space from the z pass's i beta l V_x and from V_z, so
+// the wall-normal kernels transform five fields instead of six.
 #include "cf_device.h"
 #include "cf_kernels.h"
 
@@ -148,6 +151,56 @@ __global__ void __launch_bounds__(ZNT) zpass_batch_kernel(const ZArgs* __restric
 // ---------------------------------------------------------------------------
 // x pass helpers.  Even Nxp = 2M: complex FFT of length M; odd: length Nxp.
 // ---------------------------------------------------------------------------
+// Pointwise arithmetic shared by every x pass (odd and even Nxp, six-line,
+// split and direct-load rotating passes, ensembles), each multiply-add written
+// as an explicit fma: every variant then rounds identically whatever the
+// surrounding code lets the compiler contract, so any tiling gives the same
+// bits.
+__device__ __forceinline__ double xfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float xfma(float a, float b, float c) { return fmaf(a, b, c); }
+
+// packed input of the half-length inverse transform of a real line:
+// Z_k = (X_k + conj X_{M-k}) + i conj(t) (X_k - conj X_{M-k}), t = exp(-2 pi i k / Nxp)
+template <typename C>
+__device__ __forceinline__ C x_pack(C xk, C xm, C t) {
+    using T = creal_t<C>;
+    const T sx = xk.x + xm.x, sy = xk.y - xm.y;  // X_k + conj X_{M-k}
+    const T dx = xk.x - xm.x, dy = xk.y + xm.y;  // X_k - conj X_{M-k}
+    const T px = xfma(t.x, dx, t.y * dy), py = xfma(t.x, dy, -(t.y * dx));  // conj(t) d
+    return cplx<C>(sx - py, sy + px);
+}
+
+// coefficient k of a real line from the half-length forward transform Z:
+// X_k = ((Z_k + conj Z_{M-k}) - i t (Z_k - conj Z_{M-k})) / 2
+template <typename C>
+__device__ __forceinline__ C x_unpack(C zk, C zm, C t) {
+    using T = creal_t<C>;
+    const T sx = zk.x + zm.x, sy = zk.y - zm.y;
+    const T dx = zk.x - zm.x, dy = zk.y + zm.y;
+    const T px = xfma(t.x, dx, -(t.y * dy)), py = xfma(t.x, dy, t.y * dx);  // t d
+    return cplx<C>(T(0.5) * (sx + py), T(0.5) * (sy - px));
+}
+
+// v - i s w (Omega_y = field 4 - i alpha k V_z)
+template <typename C>
+__device__ __forceinline__ C x_minus_i(C v, C w, creal_t<C> s) {
+    return cplx<C>(xfma(s, w.y, v.x), xfma(-s, w.x, v.y));
+}
+
+// V x Omega on the paired points (x = 2j in .x, x = 2j+1 in .y)
+template <typename C>
+__device__ __forceinline__ void x_cross(C u0, C u1, C u2, C w0, C w1, C w2, C& n0, C& n1, C& n2) {
+    n0 = cplx<C>(xfma(u1.x, w2.x, -(u2.x * w1.x)), xfma(u1.y, w2.y, -(u2.y * w1.y)));
+    n1 = cplx<C>(xfma(u2.x, w0.x, -(u0.x * w2.x)), xfma(u2.y, w0.y, -(u0.y * w2.y)));
+    n2 = cplx<C>(xfma(u0.x, w1.x, -(u1.x * w0.x)), xfma(u0.y, w1.y, -(u1.y * w0.y)));
+}
+
+// u gx + v gy + w gz on the paired points
+template <typename C>
+__device__ __forceinline__ C x_dot3(C u, C v, C w, C gx, C gy, C gz) {
+    return cplx<C>(xfma(w.x, gz.x, xfma(v.x, gy.x, u.x * gx.x)), xfma(w.y, gz.y, xfma(v.y, gy.y, u.y * gx.y)));
+}
+
 struct XCtx {
     const XArgs& a;
     int Nxp, Nxh, M, LPB, ld, tid;
@@ -168,8 +221,10 @@ struct XCtx {
         bufB = s + LPB * ld;
     }
     // inverse real transform of LPB lines of `in` (Nxh complex per line)
-    // into phys[line * Nxp + x]
-    __device__ __forceinline__ void c2r(const double2* __restrict__ in, double* phys, bool dx = false) {
+    // into phys[line * Nxp + x]; with `sub`, of in - i alpha k sub (the
+    // rotating form's Omega_y line)
+    __device__ __forceinline__ void c2r(const double2* __restrict__ in, double* phys, bool dx = false,
+                                        const double2* __restrict__ sub = nullptr) {
         const bool xeven = (a.gd.Nx % 2 == 0);
         // stage X_k (k = 0..M) in bufB (zero padding, Nyquist plane and Im(DC) removed)
         const int nX = even ? M + 1 : Nxp / 2 + 1;
@@ -177,7 +232,10 @@ struct XCtx {
             int ln = idx / nX, k = idx - ln * nX;
             long long gl = gl0 + ln;
             double2 v = czero();
-            if (gl < nlines && k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = in[(size_t)gl * Nxh + k];
+            if (gl < nlines && k < Nxh && !(xeven && k == a.gd.Nx / 2)) {
+                v = in[(size_t)gl * Nxh + k];
+                if (sub) v = x_minus_i(v, sub[(size_t)gl * Nxh + k], a.gd.alpha * k);
+            }
             if (k == 0) v.y = 0.0;
             if (even && k == M) v.y = 0.0;
             if (dx) v = cmuli(cscale(v, a.gd.alpha * k));
@@ -187,10 +245,7 @@ struct XCtx {
         if (even) {
             for (int idx = tid; idx < LPB * M; idx += XNT) {
                 int ln = idx / M, k = idx - ln * M;
-                double2 Xk = bufB[ln * ld + k];
-                double2 Xc = cconj(bufB[ln * ld + (M - k)]);
-                double2 w = cconj(a.twx[k]);  // exp(+2 pi i k / Nxp)
-                bufA[ln * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
+                bufA[ln * ld + k] = x_pack(bufB[ln * ld + k], bufB[ln * ld + (M - k)], a.twx[k]);
             }
         } else {
             for (int idx = tid; idx < LPB * M; idx += XNT) {
@@ -229,10 +284,7 @@ struct XCtx {
             if (gl >= nlines) continue;
             double2 X;
             if (even) {
-                double2 Zk = res[ln * ld + (k % M)];
-                double2 Zc = cconj(res[ln * ld + ((M - k) % M)]);
-                double2 w = a.twx[k];  // exp(-2 pi i k / Nxp)
-                X = csub(cscale(cadd(Zk, Zc), 0.5), cscale(cmuli(cmul(w, csub(Zk, Zc))), 0.5));
+                X = x_unpack(res[ln * ld + (k % M)], res[ln * ld + ((M - k) % M)], a.twx[k]);
             } else {
                 X = res[ln * ld + k];
             }
@@ -277,6 +329,18 @@ __device__ __forceinline__ C x_coef(const XArgs& a, int f, long long gl, int k, 
     return v;
 }
 
+// Coefficient X_k of FFT line f.  Rotating forms: line 4 is Omega_y =
+// dV_x/dz - dV_z/dx, formed here as field 4 (i beta l V_x, emitted by the z
+// inverse) minus i alpha k times field 2 (V_z).
+template <int MODE, typename C>
+__device__ __forceinline__ C x_line_coef(const XArgs& a, int f, long long gl, int k, int M) {
+    C v = x_coef<C>(a, f, gl, k, M);
+    if constexpr (MODE == X_ROTATIONAL || MODE == X_ROT_SPLIT || MODE == X_ROT_ASYNC) {
+        if (f == 4) v = x_minus_i(v, x_coef<C>(a, 2, gl, k, M), creal_t<C>(a.gd.alpha * k));
+    }
+    return v;
+}
+
 template <int MODE, int PPT, typename C>
 __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned block) {
     using T = creal_t<C>;
@@ -313,8 +377,8 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
                     Xk[b] = Xm[b] = cplx<C>(0, 0);
                     if (idx < 3 * M) {
                         const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
-                        Xk[b] = x_coef<C>(a, 3 * half + q, gl, k, M);
-                        Xm[b] = x_coef<C>(a, 3 * half + q, gl, M - k, M);
+                        Xk[b] = x_line_coef<MODE, C>(a, 3 * half + q, gl, k, M);
+                        Xm[b] = x_line_coef<MODE, C>(a, 3 * half + q, gl, M - k, M);
                     }
                 }
 #pragma unroll
@@ -322,9 +386,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
                     const int idx = base + b * 64;
                     if (idx < 3 * M) {
                         const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
-                        const C Xc = cconj(Xm[b]);
-                        const C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
-                        bufA[q * ld + k] = cadd(cadd(Xk[b], Xc), cmuli(cmul(w, csub(Xk[b], Xc))));
+                        bufA[q * ld + k] = x_pack(Xk[b], Xm[b], twx[k]);
                     }
                 }
             }
@@ -346,24 +408,70 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
             const int j = lane + 64 * i;
             if (j < M) {
                 const C w0 = bufA[0 * ld + j], w1 = bufA[1 * ld + j], w2 = bufA[2 * ld + j];
-                const C u0 = u[0][i], u1 = u[1][i], u2 = u[2][i];
-                bufA[0 * ld + j] = cplx<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
-                bufA[1 * ld + j] = cplx<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
-                bufA[2 * ld + j] = cplx<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+                C n0, n1, n2;
+                x_cross(u[0][i], u[1][i], u[2][i], w0, w1, w2, n0, n1, n2);
+                bufA[0 * ld + j] = n0;
+                bufA[1 * ld + j] = n1;
+                bufA[2 * ld + j] = n2;
             }
         }
         wave_sync();
         lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, 3, ld, -1, lane, 64);
         for (int idx = lane; idx < 3 * Nxh; idx += 64) {
             const int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
-            const C Zk = bufA[c * ld + k];
-            const C Zc = cconj(bufA[c * ld + ((M - k) % M)]);
-            const C w = twx[k];  // exp(-2 pi i k / Nxp)
             reinterpret_cast<C*>(a.out[c])[(size_t)gl * Nxh + k] =
-                csub(cscale(cadd(Zk, Zc), T(0.5)), cscale(cmuli(cmul(w, csub(Zk, Zc))), T(0.5)));
+                x_unpack(bufA[c * ld + k], bufA[c * ld + ((M - k) % M)], twx[k]);
         }
         return;
     } else if constexpr (PPT > 0) {
+      if constexpr (MODE == X_ROT_ASYNC) {
+        // Raw coefficients of the six fields straight from HBM into the six
+        // lines (global_load_lds: no registers, every load in flight at
+        // once); padding k = Nxh..M and the x Nyquist k = Nx/2 are zero.
+        static_assert(sizeof(C) == 16, "direct LDS loads move 16-byte elements");
+        const int nload = xeven ? a.gd.Nx / 2 : Nxh;  // retained coefficients, Nyquist excluded
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const C* src = reinterpret_cast<const C*>(a.in[f]) + (size_t)gl * Nxh;
+            for (int k0 = 0; k0 < nload; k0 += 64)
+                if (k0 + lane < nload)
+                    __builtin_amdgcn_global_load_lds((const void*)(src + k0 + lane),
+                                                     (__attribute__((address_space(3))) void*)(bufA + f * ld + k0),
+                                                     16, 0, 0);
+        }
+        for (int idx = lane; idx < 6 * (M + 1 - nload); idx += 64) {
+            const int f = idx / (M + 1 - nload), k = nload + idx - f * (M + 1 - nload);
+            bufA[f * ld + k] = cplx<C>(0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // the direct loads have landed
+        wave_sync();
+        // pack pairs (k, M - k) in place, each pair by one lane; line 4 first
+        // (Omega_y = X_4 - i alpha k X_2 reads line 2 before it is packed)
+        // the same values (and operation order) as x_coef / x_line_coef
+        auto coef = [&](int q, int k) {
+            C v = bufA[q * ld + k];
+            if (k == 0 || k == M) v.y = 0;
+            return v;
+        };
+        auto line_coef = [&](int q, int k) {
+            C v = coef(q, k);
+            if (q == 4) v = x_minus_i(v, coef(2, k), T(a.gd.alpha * k));
+            return v;
+        };
+        auto pack_pair = [&](int q, int k) {
+            const C xk = line_coef(q, k), xm = line_coef(q, M - k);
+            bufA[q * ld + k] = x_pack(xk, xm, twx[k]);
+            if (M - k != k && M - k < M) bufA[q * ld + (M - k)] = x_pack(xm, xk, twx[M - k]);
+        };
+        const int npair = M / 2 + 1;  // k = 0..M/2
+        for (int k = lane; k < npair; k += 64) pack_pair(4, k);
+        wave_sync();
+        for (int idx = lane; idx < 5 * npair; idx += 64) {
+            const int j = idx / npair, k = idx - j * npair;
+            pack_pair(j < 4 ? j : 5, k);
+        }
+        wave_sync();
+      } else {
         // pack straight from HBM into the single buffer; FFT line q reads
         // field src(q), times i alpha k for the convective x derivatives
         for (int base = lane; base < NFFT * M; base += XLOADS * 64) {
@@ -376,8 +484,8 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
                     const int q = (int)fast_div((unsigned)idx, a.mul_m), k = idx - q * M;
                     int f = q;
                     if constexpr (MODE == X_CONVECTIVE || MODE == X_GRADIENT) f = q < 3 ? q : q - 3;
-                    Xk[b] = x_coef<C>(a, f, gl, k, M);
-                    Xm[b] = x_coef<C>(a, f, gl, M - k, M);
+                    Xk[b] = x_line_coef<MODE, C>(a, f, gl, k, M);
+                    Xm[b] = x_line_coef<MODE, C>(a, f, gl, M - k, M);
                 }
             }
 #pragma unroll
@@ -392,13 +500,12 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
                             xm = cmuli(cscale(xm, T(a.gd.alpha * (M - k))));
                         }
                     }
-                    const C Xc = cconj(xm);
-                    const C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
-                    bufA[q * ld + k] = cadd(cadd(xk, Xc), cmuli(cmul(w, csub(xk, Xc))));
+                    bufA[q * ld + k] = x_pack(xk, xm, twx[k]);
                 }
             }
         }
         wave_sync();
+      }
         lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, NFFT, ld, +1, lane, 64);
         if constexpr (MODE == X_GRADIENT) {
             // line 3 + 3 d + i holds dV_i/dx_d (d = x, y, z): values at x = 2m, 2m+1
@@ -415,19 +522,20 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
         // products at x = 2j (real parts) and 2j+1 (imaginary parts), in place:
         // each lane reads all lines at its j before writing them
         for (int j = lane; j < M; j += 64) {
-            if constexpr (MODE == X_ROTATIONAL) {
+            if constexpr (MODE == X_ROTATIONAL || MODE == X_ROT_ASYNC) {
                 C u0 = bufA[0 * ld + j], u1 = bufA[1 * ld + j], u2 = bufA[2 * ld + j];
                 C w0 = bufA[3 * ld + j], w1 = bufA[4 * ld + j], w2 = bufA[5 * ld + j];
-                bufA[0 * ld + j] = cplx<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
-                bufA[1 * ld + j] = cplx<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
-                bufA[2 * ld + j] = cplx<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+                C n0, n1, n2;
+                x_cross(u0, u1, u2, w0, w1, w2, n0, n1, n2);
+                bufA[0 * ld + j] = n0;
+                bufA[1 * ld + j] = n1;
+                bufA[2 * ld + j] = n2;
             } else if constexpr (MODE == X_CONVECTIVE) {
                 C u = bufA[0 * ld + j], v = bufA[1 * ld + j], w = bufA[2 * ld + j];
                 C n[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    C gx = bufA[(3 + i) * ld + j], gy = bufA[(6 + i) * ld + j], gz = bufA[(9 + i) * ld + j];
-                    n[i] = cplx<C>(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
+                    n[i] = x_dot3(u, v, w, bufA[(3 + i) * ld + j], bufA[(6 + i) * ld + j], bufA[(9 + i) * ld + j]);
                 }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) bufA[i * ld + j] = n[i];
@@ -445,11 +553,8 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
         lds_fft_inplace<PPT, true>(bufA, a.plan, (const C*)tw, NOUT, ld, -1, lane, 64);
         for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
             int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
-            C Zk = bufA[c * ld + k];
-            C Zc = cconj(bufA[c * ld + ((M - k) % M)]);
-            C w = twx[k];  // exp(-2 pi i k / Nxp)
             reinterpret_cast<C*>(a.out[c])[(size_t)gl * Nxh + k] =
-                csub(cscale(cadd(Zk, Zc), T(0.5)), cscale(cmuli(cmul(w, csub(Zk, Zc))), T(0.5)));
+                x_unpack(bufA[c * ld + k], bufA[c * ld + ((M - k) % M)], twx[k]);
         }
         return;
     }
@@ -459,7 +564,11 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
     for (int idx = lane; idx < NIN * nX; idx += 64) {
         int f = (int)fast_div((unsigned)idx, a.mul_nx), k = idx - f * nX;
         C v = cplx<C>(0, 0);
-        if (k < Nxh && !(xeven && k == a.gd.Nx / 2)) v = reinterpret_cast<const C*>(a.in[f])[(size_t)gl * Nxh + k];
+        if (k < Nxh && !(xeven && k == a.gd.Nx / 2)) {
+            v = reinterpret_cast<const C*>(a.in[f])[(size_t)gl * Nxh + k];
+            if constexpr (MODE == X_ROTATIONAL)  // Omega_y = field 4 - i alpha k V_z
+                if (f == 4) v = x_minus_i(v, reinterpret_cast<const C*>(a.in[2])[(size_t)gl * Nxh + k], T(a.gd.alpha * k));
+        }
         if (k == 0 || k == M) v.y = 0;
         bufB[f * ld + k] = v;
     }
@@ -480,9 +589,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
             Xk = cmuli(cscale(Xk, T(a.gd.alpha * k)));
             Xm = cmuli(cscale(Xm, T(a.gd.alpha * (M - k))));
         }
-        C Xc = cconj(Xm);
-        C w = cconj(twx[k]);  // exp(+2 pi i k / Nxp)
-        bufA[q * ld + k] = cadd(cadd(Xk, Xc), cmuli(cmul(w, csub(Xk, Xc))));
+        bufA[q * ld + k] = x_pack(Xk, Xm, twx[k]);
     }
     wave_sync();
     C* res = lds_fft<true>(bufA, bufB, a.plan, (const C*)tw, NFFT, ld, +1, lane, 64);
@@ -501,17 +608,13 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
     // pointwise products at x = 2j (real parts) and x = 2j+1 (imaginary parts)
     for (int j = lane; j < M; j += 64) {
         if constexpr (MODE == X_ROTATIONAL) {
-            C u0 = res[0 * ld + j], u1 = res[1 * ld + j], u2 = res[2 * ld + j];
-            C w0 = res[3 * ld + j], w1 = res[4 * ld + j], w2 = res[5 * ld + j];
-            nxt[0 * ld + j] = cplx<C>(u1.x * w2.x - u2.x * w1.x, u1.y * w2.y - u2.y * w1.y);
-            nxt[1 * ld + j] = cplx<C>(u2.x * w0.x - u0.x * w2.x, u2.y * w0.y - u0.y * w2.y);
-            nxt[2 * ld + j] = cplx<C>(u0.x * w1.x - u1.x * w0.x, u0.y * w1.y - u1.y * w0.y);
+            x_cross(res[0 * ld + j], res[1 * ld + j], res[2 * ld + j], res[3 * ld + j], res[4 * ld + j], res[5 * ld + j],
+                    nxt[0 * ld + j], nxt[1 * ld + j], nxt[2 * ld + j]);
         } else if constexpr (MODE == X_CONVECTIVE) {
             C u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                C gx = res[(3 + i) * ld + j], gy = res[(6 + i) * ld + j], gz = res[(9 + i) * ld + j];
-                nxt[i * ld + j] = cplx<C>(u.x * gx.x + v.x * gy.x + w.x * gz.x, u.y * gx.y + v.y * gy.y + w.y * gz.y);
+                nxt[i * ld + j] = x_dot3(u, v, w, res[(3 + i) * ld + j], res[(6 + i) * ld + j], res[(9 + i) * ld + j]);
             }
         } else {
             C u = res[0 * ld + j], v = res[1 * ld + j], w = res[2 * ld + j];
@@ -527,11 +630,7 @@ __device__ __forceinline__ void xpass_wave(const XArgs& a, C* smem, unsigned blo
     C* r2 = lds_fft<true>(nxt, res, a.plan, (const C*)tw, NOUT, ld, -1, lane, 64);
     for (int idx = lane; idx < NOUT * Nxh; idx += 64) {
         int c = (int)fast_div((unsigned)idx, a.mul_nxh), k = idx - c * Nxh;
-        C Zk = r2[c * ld + k];
-        C Zc = cconj(r2[c * ld + ((M - k) % M)]);
-        C w = twx[k];  // exp(-2 pi i k / Nxp)
-        reinterpret_cast<C*>(a.out[c])[(size_t)gl * Nxh + k] =
-            csub(cscale(cadd(Zk, Zc), T(0.5)), cscale(cmuli(cmul(w, csub(Zk, Zc))), T(0.5)));
+        reinterpret_cast<C*>(a.out[c])[(size_t)gl * Nxh + k] = x_unpack(r2[c * ld + k], r2[c * ld + ((M - k) % M)], twx[k]);
     }
 }
 
@@ -566,8 +665,8 @@ __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
             double n[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                n[i] = u * phys[(3 + i) * LPB * Nxp + idx] + v * phys[(6 + i) * LPB * Nxp + idx] +
-                       w * phys[(9 + i) * LPB * Nxp + idx];
+                n[i] = xfma(w, phys[(9 + i) * LPB * Nxp + idx],
+                            xfma(v, phys[(6 + i) * LPB * Nxp + idx], u * phys[(3 + i) * LPB * Nxp + idx]));
 #pragma unroll
             for (int i = 0; i < 3; ++i) phys[i * LPB * Nxp + idx] = n[i];
         }
@@ -587,13 +686,13 @@ __global__ void __launch_bounds__(XNT) xpass_kernel(XArgs a) {
         __syncthreads();
         for (int f = 0; f < 6; ++f) c.r2c(phys + (size_t)(3 + f) * LPB * Nxp, a.out[f]);
     } else if (a.mode == X_ROTATIONAL) {
-        for (int f = 0; f < 6; ++f) c.c2r(a.in[f], phys + (size_t)f * LPB * Nxp);
+        for (int f = 0; f < 6; ++f) c.c2r(a.in[f], phys + (size_t)f * LPB * Nxp, false, f == 4 ? a.in[2] : nullptr);
         for (int idx = c.tid; idx < LPB * Nxp; idx += XNT) {
             double u0 = phys[0 * LPB * Nxp + idx], u1 = phys[1 * LPB * Nxp + idx], u2 = phys[2 * LPB * Nxp + idx];
             double w0 = phys[3 * LPB * Nxp + idx], w1 = phys[4 * LPB * Nxp + idx], w2 = phys[5 * LPB * Nxp + idx];
-            phys[0 * LPB * Nxp + idx] = u1 * w2 - u2 * w1;
-            phys[1 * LPB * Nxp + idx] = u2 * w0 - u0 * w2;
-            phys[2 * LPB * Nxp + idx] = u0 * w1 - u1 * w0;
+            phys[0 * LPB * Nxp + idx] = xfma(u1, w2, -(u2 * w1));
+            phys[1 * LPB * Nxp + idx] = xfma(u2, w0, -(u0 * w2));
+            phys[2 * LPB * Nxp + idx] = xfma(u0, w1, -(u1 * w0));
         }
         __syncthreads();
         for (int f = 0; f < 3; ++f) c.r2c(phys + (size_t)f * LPB * Nxp, a.out[f]);
@@ -686,7 +785,7 @@ hipError_t launch_zpass_batch(int dir, int ppt, const ZArgs* tab, const unsigned
 
 static bool x_wave_path(const XArgs& a) {
     return a.gd.Nxp % 2 == 0 && (a.mode == X_ROTATIONAL || a.mode == X_CONVECTIVE || a.mode == X_DIVERGENCE ||
-                                 a.mode == X_GRADIENT || a.mode == X_ROT_SPLIT);
+                                 a.mode == X_GRADIENT || a.mode == X_ROT_SPLIT || a.mode == X_ROT_ASYNC);
 }
 
 int x_phys_lines(int mode) {
@@ -749,6 +848,10 @@ hipError_t launch_xpass_batch(int mode, int ppt, int waves, const XArgs* tab, co
         if (ppt == 5) CF_XB(X_ROT_SPLIT, 5)
         if (ppt == 9) CF_XB(X_ROT_SPLIT, 9)
     }
+    if (mode == X_ROT_ASYNC) {
+        if (ppt == 9) CF_XB(X_ROT_ASYNC, 9)
+        if (ppt == 18) CF_XB(X_ROT_ASYNC, 18)
+    }
     if (mode == X_ROTATIONAL) CF_XBM(X_ROTATIONAL)
     if (mode == X_CONVECTIVE) CF_XBM(X_CONVECTIVE)
 #undef CF_XBM
@@ -800,6 +903,11 @@ hipError_t launch_xpass(const XArgs& a, hipStream_t st) {
         }
         if (a.ppt == 5) CF_XWAVE(X_ROT_SPLIT, 5, double2)
         CF_XWAVE(X_ROT_SPLIT, 9, double2)
+    }
+    if (a.mode == X_ROT_ASYNC) {  // FP64, in place only
+        if (a.mixed || (a.ppt != 9 && a.ppt != 18)) return hipErrorInvalidValue;
+        if (a.ppt == 9) CF_XWAVE(X_ROT_ASYNC, 9, double2)
+        CF_XWAVE(X_ROT_ASYNC, 18, double2)
     }
     if (a.mode == X_ROTATIONAL) CF_XMODE(X_ROTATIONAL)
     if (a.mode == X_CONVECTIVE) CF_XMODE(X_CONVECTIVE)

@@ -84,16 +84,17 @@ struct Tile {
     // mode index.  W divides nxr, so the wave's W slots lie in one z row and
     // its modes stay consecutive; a wave past the last non-Nyquist mode is
     // empty (modes >= B: zero loads, no stores).
-    __device__ __forceinline__ void order_modes(const YOrder& o) {
-        if (sw0 >= o.nreg) {
+    __device__ __forceinline__ void order_modes(long long nreg, int nxr, int zskip) {
+        if (sw0 >= nreg) {
             sw0 = gd.B;
         } else {
-            const unsigned q = (unsigned)sw0 / (unsigned)o.nxr;
-            const int iz = (int)q + ((int)q >= o.zskip ? 1 : 0);
-            sw0 = (long long)iz * gd.Nxh + ((unsigned)sw0 - q * (unsigned)o.nxr);
+            const unsigned q = (unsigned)sw0 / (unsigned)nxr;
+            const int iz = (int)q + ((int)q >= zskip ? 1 : 0);
+            sw0 = (long long)iz * gd.Nxh + ((unsigned)sw0 - q * (unsigned)nxr);
         }
         s = sw0 + mw;
     }
+    __device__ __forceinline__ void order_modes(const YOrder& o) { order_modes(o.nreg, o.nxr, o.zskip); }
     // spectral field column of this group's mode (zero beyond P / absent mode)
     __device__ __forceinline__ CVec load(const double2* __restrict__ src, const Group<L>& g) const {
         CVec v;
@@ -144,7 +145,7 @@ struct Tile {
             }
         }
         wave_sync();
-        double2* res = lds_fft<true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
+        double2* res = lds_fft<true, true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
         CVec v;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -172,7 +173,7 @@ struct Tile {
             }
         }
         wave_sync();
-        double2* res = lds_fft<true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
+        double2* res = lds_fft<true, true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
         for (int idx = lane; idx < (P + 1) * W; idx += 64) {
             int n = idx / W, m = idx - n * W;
             long long sm = sw0 + m;
@@ -240,47 +241,60 @@ __device__ __forceinline__ void load_fac(const double* __restrict__ f, long long
 
 // Total velocity V = U + Ub e_x at (0,0) and the y-physical fields each
 // nonlinear form needs, at the Lobatto points, into out6:
-//   rotating   V, Omega = curl V   (nonlinear.jl:173-193, operators.jl:172-190)
+//   rotating   V, Omega_x, Omega_z of Omega = curl V (nonlinear.jl:173-193,
+//              operators.jl:172-190); out6[4] is not written: Omega_y =
+//              dV_x/dz - dV_z/dx is formed by the periodic passes (the z
+//              inverse emits i beta l V_x from V_x into z field 4, the x pass
+//              subtracts i alpha k V_z), one Chebyshev transform fewer
 //   convective V, dV/dy            (nonlinear.jl:113-140; x and z derivatives
 //                                   are formed by the periodic passes)
 //   divergence V                   (nonlinear.jl:146-167)
-template <int L, typename SC = double2>
+// get(c) returns U component c: from registers in the stage kernel, from HBM
+// in the prologue (loaded where first needed, which shortens live ranges).
+template <int L, typename SC, typename GetU>
 __device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
-                            const CVec U[3]) {
+                                            GetU get) {
     SC* out6[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) out6[i] = reinterpret_cast<SC*>(a.out6[i]);
-    CVec V[3] = {U[0], U[1], U[2]};
+    CVec V0 = get(0);
     if (mi.mean && mi.valid) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int n = 4 * g.t + e;
-            if (n < a.gd.Ny) V[0].v[e] = cadd(V[0].v[e], cmk(a.base[n], 0.0));
+            if (n < a.gd.Ny) V0.v[e] = cadd(V0.v[e], cmk(a.base[n], 0.0));
         }
     }
-    tile.dct_inverse(out6[0], a.dct, g, V[0]);
-    tile.dct_inverse(out6[1], a.dct, g, V[1]);
-    tile.dct_inverse(out6[2], a.dct, g, V[2]);
-    if (a.form == FORM_DIVERGENCE) return;
-    if (a.form == FORM_CONVECTIVE) {
-        tile.dct_inverse(out6[3], a.dct, g, cheb_deriv(g, V[0]));
-        tile.dct_inverse(out6[4], a.dct, g, cheb_deriv(g, V[1]));
-        tile.dct_inverse(out6[5], a.dct, g, cheb_deriv(g, V[2]));
+    if (a.form != FORM_ROTATING) {
+        tile.dct_inverse(out6[0], a.dct, g, V0);
+        if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[3], a.dct, g, cheb_deriv(g, V0));
+        const CVec V1 = get(1);
+        tile.dct_inverse(out6[1], a.dct, g, V1);
+        if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[4], a.dct, g, cheb_deriv(g, V1));
+        const CVec V2 = get(2);
+        tile.dct_inverse(out6[2], a.dct, g, V2);
+        if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[5], a.dct, g, cheb_deriv(g, V2));
         return;
     }
-    CVec d3 = cheb_deriv(g, V[2]);
+    // order chosen so each velocity component dies as early as possible
+    const CVec V1 = get(1);
     CVec W;
+    tile.dct_inverse(out6[0], a.dct, g, V0);
+    {
+        CVec d1 = cheb_deriv(g, V0);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V[1].v[e], mi.kz)));  // dy V3 - i l V2
-    tile.dct_inverse(out6[3], a.dct, g, W);
-#pragma unroll
-    for (int e = 0; e < 4; ++e)  // i (l V1 - k V3)
-        W.v[e] = cmuli(csub(cscale(V[0].v[e], mi.kz), cscale(V[2].v[e], mi.kx)));
-    tile.dct_inverse(out6[4], a.dct, g, W);
-    CVec d1 = cheb_deriv(g, V[0]);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) W.v[e] = csub(cmuli(cscale(V[1].v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
+        for (int e = 0; e < 4; ++e) W.v[e] = csub(cmuli(cscale(V1.v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
+    }
     tile.dct_inverse(out6[5], a.dct, g, W);
+    const CVec V2 = get(2);
+    tile.dct_inverse(out6[2], a.dct, g, V2);
+    {
+        CVec d3 = cheb_deriv(g, V2);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V1.v[e], mi.kz)));  // dy V3 - i l V2
+    }
+    tile.dct_inverse(out6[3], a.dct, g, W);
+    tile.dct_inverse(out6[1], a.dct, g, V1);
 }
 
 // ---------------------------------------------------------------------------
@@ -545,7 +559,7 @@ __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     tile.store(a.U[1], g, Un[1]);
     tile.store(a.U[2], g, Un[2]);
     tile.store(a.P, g, Pn);
-    if (a.out6[0]) inverse_six<L, SC>(tile, a, g, mi, Un);
+    if (a.out6[0]) inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return Un[c]; });
 }
 
 // Nyquist modes of a stage under the non-Nyquist-first order, elementwise
@@ -608,26 +622,35 @@ __device__ __forceinline__ void yprologue_body(const YArgs& a, unsigned block) {
     Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
-    CVec U[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
-    inverse_six<L, SC>(tile, a, g, mi, U);
+    inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
 }
 
 // The single-problem prologue keeps its own copy of the body: passing the
 // byval kernel argument by reference to yprologue_body makes the compiler
-// materialise part of it in scratch memory.
+// materialise part of it in scratch memory.  Under the non-Nyquist-first
+// order (a.order.on) the launch has only the regular blocks: the periodic
+// passes never read the Nyquist modes' y-physical values, so none are
+// written.
 template <int L, typename SC>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
+    // the order's fields are read one by one (a reference to the byval
+    // argument, or a copy of the whole struct, lands in scratch)
+    const bool ordered = a.order.on != 0;
+    const unsigned block = (ordered && a.order.xcd) ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
+    if (ordered) tile.order_modes(a.order.nreg, a.order.nxr, a.order.zskip);
     Group<L> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
-    CVec U[3];
+    if constexpr (L == 4) {  // all loads first: the 4-lane build stays within 128 registers so
+        CVec U[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
-    inverse_six<L, SC>(tile, a, g, mi, U);
+        for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
+        inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return U[c]; });
+    } else {
+        inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
+    }
 }
 
 // Batched ensemble launches: instance i owns blocks [start[i], start[i+1])
@@ -1054,10 +1077,13 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
             else hipLaunchKernelGGL((ystage_kernel<L, double2>), g2, dim3(YNT), lds, st, b);
             break;
         }
-        case Y_PROLOGUE:
-            if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, float2>), grid, dim3(YNT), lds, st, a);
-            else hipLaunchKernelGGL((yprologue_kernel<L, double2>), grid, dim3(YNT), lds, st, a);
+        case Y_PROLOGUE: {
+            YArgs b = a;  // regular (non-Nyquist) blocks only, see yprologue_kernel
+            const dim3 g2 = y_stage_order<L>(b) ? dim3(b.order.nyq_first) : grid;
+            if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, float2>), g2, dim3(YNT), lds, st, b);
+            else hipLaunchKernelGGL((yprologue_kernel<L, double2>), g2, dim3(YNT), lds, st, b);
             break;
+        }
         case Y_SOLVE: hipLaunchKernelGGL(ysolve_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_NEGLAP: hipLaunchKernelGGL(yneglap_kernel<L>, grid, dim3(YNT), lds, st, a); break;
         case Y_PRESSURE: hipLaunchKernelGGL(ypressure_kernel<L>, grid, dim3(YNT), lds, st, a); break;

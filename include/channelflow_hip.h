@@ -211,6 +211,18 @@ int cf_problem_tune(cf_problem* h, void* stream);
  * (0: two buffers), out[2] x wavefronts per block */
 int cf_problem_tiles(const cf_problem* h, int out[3]);
 
+/* Tile choice of the FP64 step, to read and to set (no counterpart in the
+ * reference; lets a caller keep a tuning across runs, and the tests run every
+ * variant): tiles[0] z tile width (1..min(32, Nx/2+1)), tiles[1] z in-place
+ * points per thread (read only; set from the width), tiles[2] x pass variant
+ * of the rotating form with even padded Nx (0 six LDS lines, 1 split: three
+ * lines, velocity held in registers, 2 six lines with direct-to-LDS loads;
+ * other forms: 0), tiles[3] x wavefronts per block (1..the variant's LDS
+ * limit).  Results are identical for every valid choice; an unavailable one
+ * returns CF_ERR_ARGUMENT and changes nothing. */
+int cf_problem_get_tiles(const cf_problem* h, int tiles[4]);
+int cf_problem_set_tiles(cf_problem* h, const int tiles[4]);
+
 /* Uniform pressure gradient (dPdx, dPdz) returned by the last Stokes solve of
  * the last step (stokes.jl:101-107).  Synchronises the stream; HOST output. */
 int cf_last_gradient(cf_problem* h, double out[2], void* stream);

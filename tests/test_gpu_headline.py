@@ -9,15 +9,19 @@ of U, P and the bulk gradient to the oracle within its roundoff envelope
 (8 perturbation trials).  At this shape the oracle's own output moves by
 ~1e-10 in U and ~1e-9 in P when its inputs are perturbed at 1e-16
 (lambda/nu = 3.4e6 in the first stage), so a flat 1e-12 is out of reach of
-any FP64 implementation; after one step the device must instead be at least
-as accurate as the FP64 oracle (up to 2x) against the extended-precision
-oracle, the same algorithm run in long double on the same inputs."""
+any FP64 implementation.  After one step the device is judged against the
+extended-precision oracle (the same algorithm in long double on the same
+inputs): on the mean mode and on the modes outside the few smallest
+wavenumbers it must be as accurate as the FP64 oracle (up to 2x); overall,
+where those smallest wavenumbers amplify each rounding by ~1e5, its distance
+must stay within max(2x the oracle's, the roundoff envelope): one FP64
+rounding sequence against another lands anywhere inside that envelope."""
 import math
 
 import numpy as np
 import pytest
 
-from accuracy import report
+from accuracy import class_errors, report
 from oracle import oracle as O
 from test_gpu_parity import ENVELOPE_FACTOR, _split, assert_parity, rel, roundoff_envelope, say
 
@@ -95,7 +99,15 @@ def test_headline_steps(headline, nsteps):
         XU, XP = ext.step(U0, P0, 1)
         ehU, eoU = report(say, "headline U after 1", Uh, U1, XU, go)
         ehP, eoP = report(say, "headline P after 1", Ph, P1, XP, go)
-        assert ehU <= 2.0 * eoU and ehP <= 2.0 * eoP, (ehU, eoU, ehP, eoP)
+        # envelopes relative to the joint maximum of each field
+        envU = max(env[i] * np.abs(U1[i]).max() for i in range(3)) / np.abs(U1).max()
+        envP = env[3]
+        say(f"headline after 1: envelope U {envU:.3e} P {envP:.3e}")
+        assert ehU <= max(2.0 * eoU, envU) and ehP <= max(2.0 * eoP, envP), (ehU, eoU, envU, ehP, eoP, envP)
+        for name, a, o, x in (("U", Uh, U1, XU), ("P", Ph, P1, XP)):
+            ch, co = class_errors(a, x, go), class_errors(o, x, go)
+            assert ch["rest"] <= 2.0 * co["rest"], (name, ch, co)
+            assert ch["mean"] <= 2.0 * co["mean"] + 1e-15, (name, ch, co)
     dg = max(abs(a - b) / abs(b) if b else abs(a) for a, b in zip(grad, gref))
     say(f"headline {nsteps} steps: gradient {grad} oracle {gref} rel {dg:.3e}")
     assert dg <= max(1e-10, ENVELOPE_FACTOR * env[0]), dg
