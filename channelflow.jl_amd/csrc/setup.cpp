@@ -212,17 +212,26 @@ bool build_bank(const GridDims& gd, double nu, double lambda, HostBank& hb, std:
 // --------------------------------------------------------------------------
 // FFT plans
 // --------------------------------------------------------------------------
+// Pass order: odd radices first, then 2, 4, 8.  A Stockham pass of radix R
+// after spans Ns writes runs of Ns consecutive points Ns*R apart; the first
+// pass (Ns = 1) thus writes at stride R, which for R = 8 puts eight lanes of
+// a 16-lane LDS access on one bank group (16-byte points, 64 banks).  An odd
+// first radix writes at a stride coprime to the bank count.
 void plan_factors(int n, std::vector<int>& radix) {
     radix.clear();
     int m = n;
-    while (m % 8 == 0 && m > 8) { radix.push_back(8); m /= 8; }
-    while (m % 4 == 0) { radix.push_back(4); m /= 4; }
-    while (m % 2 == 0) { radix.push_back(2); m /= 2; }
+    int c8 = 0, c4 = 0, c2 = 0;
+    while (m % 8 == 0 && m > 8) { ++c8; m /= 8; }
+    while (m % 4 == 0) { ++c4; m /= 4; }
+    while (m % 2 == 0) { ++c2; m /= 2; }
     while (m % 3 == 0) { radix.push_back(3); m /= 3; }
     while (m % 5 == 0) { radix.push_back(5); m /= 5; }
     for (int p = 7; p * p <= m; p += 2)
         while (m % p == 0) { radix.push_back(p); m /= p; }
     if (m > 1) radix.push_back(m);
+    radix.insert(radix.end(), c2, 2);
+    radix.insert(radix.end(), c4, 4);
+    radix.insert(radix.end(), c8, 8);
     if (radix.empty()) radix.push_back(1);
 }
 
