@@ -114,8 +114,10 @@ struct cf_problem {
     int xsplit = 0, xppts = 0, xwavess = 0, xwavesa = 0;
     // largest wavefronts per block that fit the LDS, per variant (0: unavailable)
     int xmaxw[3] = {0, 0, 0};
-    // the same for the mixed step's FP32 x pass
+    // the same for the mixed step's FP32 x pass (largest wavefronts per
+    // block of the six-line and the split variant in xmaxw32)
     int xsplit32 = 0, xppts32 = 0, xwavess32 = 0;
+    int xmaxw32[2] = {0, 0};
     // one-step graphs of the plain step (no host callbacks), one per field
     // pointer set and constraint, least recently used dropped past GRAPH_CACHE;
     // cleared when the tiling changes
@@ -281,7 +283,7 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         return fail(CF_ERR_ARGUMENT, "unknown precision");
     if (cfg->precision == CF_PRECISION_MIXED && (cfg->form == CF_FORM_DIVERGENCE || ((3 * cfg->Nx + 1) / 2) % 2 != 0))
         return fail(CF_ERR_UNSUPPORTED, "mixed precision needs an even padded Nx and the rotating or convective form");
-    if (y_group_lanes(cfg->Ny) == 0) return fail(CF_ERR_UNSUPPORTED, "Ny above 255 is not supported");
+    if (y_group_lanes(cfg->Ny) == 0) return fail(CF_ERR_UNSUPPORTED, "Ny above 511 is not supported");
     // device indexing: mode counts and padded line counts in 32 bits
     if ((long long)(cfg->Nx / 2 + 1) * cfg->Nz * cfg->Ny >= (1ll << 31) ||
         (long long)((3 * cfg->Nx + 1) / 2) * ((3 * cfg->Nz + 1) / 2) * cfg->Ny >= (1ll << 31))
@@ -424,6 +426,8 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
                 h->xwavess32 = h->xppts32 ? x_max_waves(xs) : 0;
                 h->xsplit32 = h->xwavess32 > 0 ? 1 : 0;
             }
+            h->xmaxw32[0] = h->xwaves32;
+            h->xmaxw32[1] = h->xwavess32;
         }
     }
     if (cfg->precision == CF_PRECISION_MIXED) {
@@ -1560,6 +1564,34 @@ extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
         h->xwavess = bestW[1];
         h->xwavesa = bestW[2];
     }
+    // mixed step: its FP32 x pass (six-line or split, wavefronts per block)
+    if (!rc && h->mixed && h->gd.Nxp % 2 == 0) {
+        const int keepSplit = h->xsplit32;
+        int bestW[2] = {h->xwaves32, h->xwavess32}, bestSplit = keepSplit;
+        int* waves[2] = {&h->xwaves32, &h->xwavess32};
+        best = 1e30f;
+        for (int split = 0; split < 2 && !rc; ++split) {
+            if (h->xmaxw32[split] < 1 || (split > 0 && h->form != CF_FORM_ROTATING)) continue;
+            h->xsplit32 = split;
+            for (int w = 1; w <= h->xmaxw32[split] && !rc; ++w) {
+                *waves[split] = w;
+                const NonlinearArgs na = nonlinear_args(h, true);
+                float ms;
+                rc = timed([&] {
+                    HIPTRY(launch_xpass(na.x, st));
+                    return 0;
+                }, &ms);
+                if (!rc && ms < best) {
+                    best = ms;
+                    bestSplit = split;
+                    bestW[split] = w;
+                }
+            }
+        }
+        h->xsplit32 = rc ? keepSplit : bestSplit;
+        h->xwaves32 = bestW[0];
+        h->xwavess32 = bestW[1];
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return rc;
@@ -1884,6 +1916,7 @@ extern "C" int cf_ensemble_create(cf_problem* const* problems, double* const* fi
         if (h->nl_fn) return fail(CF_ERR_UNSUPPORTED, "a problem with a nonlinear callback cannot join an ensemble");
         if (h->form == CF_FORM_DIVERGENCE || h->gd.Nxp % 2 != 0)
             return fail(CF_ERR_UNSUPPORTED, "ensembles need the rotating or convective form and an even padded Nx");
+        if (y_group_k(h->gd.Ny) != 4) return fail(CF_ERR_UNSUPPORTED, "ensemble members need Ny <= 256");
         for (int f = 0; f < 4; ++f) all.push_back(fields[4 * i + f]);
     }
     if (int rc = check_fields(all.data(), (int)all.size())) return rc;

@@ -8,7 +8,7 @@ constexpr int MAX_FACTORS = 16;
 
 // per-degree constant table of the wall-normal solves (see build_cheb_table)
 constexpr int CHEB_TAB_ARRAYS = 4;
-constexpr int CHEB_TAB_MAX = 256;  // entries per array: 4L <= 256
+constexpr int CHEB_TAB_MAX = 512;  // entries per array: K L <= 512
 
 // FFT of length n factored into radices; tw[t] = exp(-2 pi i t/n) (device).
 // Per pass f (radix R, span Ns = product of earlier radices): reciprocal

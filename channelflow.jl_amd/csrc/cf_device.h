@@ -139,10 +139,14 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 // one row; L = 32 is a row pair, L = 64 the whole wave.  Every lane of a group
 // must call a group operation together (values of other groups are never used).
 //
-// tab: per-degree constants (see build_cheb_table), NT = 4L entries per array.
-template <int L>
+// Each lane holds K consecutive Chebyshev coefficients (K = 4 or 8, even:
+// element e of a lane has the parity of e), so a group covers degrees below
+// K L.  tab: per-degree constants (see build_cheb_table), NT = K L entries
+// per array.
+template <int L, int K = 4>
 struct Group {
-    static constexpr int NT = 4 * L;
+    static_assert(K % 2 == 0, "parity of a coefficient must follow its element index");
+    static constexpr int NT = K * L;
     int lane, t, r;
     const double* tab;
     __device__ __forceinline__ explicit Group(const double* tab_ = nullptr) : tab(tab_) {
@@ -237,86 +241,116 @@ struct Group {
     // ---- affine compositions x -> A + T x.  Suffix: lane t ends with
     // f_t o f_{t+1} o ... o f_{L-1}; prefix: f_t o f_{t-1} o ... o f_0.
     // Combining with a later/earlier composite (oT, oA): (T oT, A + T oA).
-    template <int D>
-    __device__ __forceinline__ void suffix_affine_step(double& T, double2& A) const {
+    // NR right-hand sides share the multiplicative part T (one operator):
+    // A[0..NR) are scanned together, each step moving T once.
+    template <int D, int NR>
+    __device__ __forceinline__ void suffix_affine_step(double& T, double2* A) const {
         if constexpr (D < L && D < 16) {
             if constexpr (L >= 16) {
                 double oT = dpp::mov1<dpp::ROW_SHL + D>(T);
-                double2 oA = dpp::mov0<dpp::ROW_SHL + D>(A);
-                A = cfma(T, oA, A);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    double2 oA = dpp::mov0<dpp::ROW_SHL + D>(A[r]);
+                    A[r] = cfma(T, oA, A[r]);
+                }
                 T *= oT;
             } else {
                 double oT = dpp::mov<dpp::ROW_SHL + D>(T);
-                double2 oA = dpp::mov<dpp::ROW_SHL + D>(A);
+                double2 oA[NR];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) oA[r] = dpp::mov<dpp::ROW_SHL + D>(A[r]);
                 if (suffix_ok<D>()) {
-                    A = cfma(T, oA, A);
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) A[r] = cfma(T, oA[r], A[r]);
                     T *= oT;
                 }
             }
         }
     }
-    __device__ __forceinline__ void suffix_affine(double& T, double2& A) const {
-        suffix_affine_step<1>(T, A);
-        suffix_affine_step<2>(T, A);
-        suffix_affine_step<4>(T, A);
-        suffix_affine_step<8>(T, A);
+    template <int NR>
+    __device__ __forceinline__ void suffix_affine_n(double& T, double2* A) const {
+        suffix_affine_step<1, NR>(T, A);
+        suffix_affine_step<2, NR>(T, A);
+        suffix_affine_step<4, NR>(T, A);
+        suffix_affine_step<8, NR>(T, A);
         if constexpr (L >= 32) {
             double oT = dpp::mov<dpp::ROW_NEWBCAST0>(dpp::odd_row(T));
-            double2 oA = dpp::mov<dpp::ROW_NEWBCAST0>(dpp::odd_row(A));
+            double2 oA[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) oA[r] = dpp::mov<dpp::ROW_NEWBCAST0>(dpp::odd_row(A[r]));
             if ((lane & 16) == 0) {
-                A = cfma(T, oA, A);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) A[r] = cfma(T, oA[r], A[r]);
                 T *= oT;
             }
         }
         if constexpr (L == 64) {
             double oT = dpp::readlane(T, 32);
-            double2 oA = dpp::readlane(A, 32);
+            double2 oA[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) oA[r] = dpp::readlane(A[r], 32);
             if (lane < 32) {
-                A = cfma(T, oA, A);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) A[r] = cfma(T, oA[r], A[r]);
                 T *= oT;
             }
         }
     }
-    template <int D>
-    __device__ __forceinline__ void prefix_affine_step(double& T, double2& A) const {
+    __device__ __forceinline__ void suffix_affine(double& T, double2& A) const { suffix_affine_n<1>(T, &A); }
+    template <int D, int NR>
+    __device__ __forceinline__ void prefix_affine_step(double& T, double2* A) const {
         if constexpr (D < L && D < 16) {
             if constexpr (L >= 16) {
                 double oT = dpp::mov1<dpp::ROW_SHR + D>(T);
-                double2 oA = dpp::mov0<dpp::ROW_SHR + D>(A);
-                A = cfma(T, oA, A);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    double2 oA = dpp::mov0<dpp::ROW_SHR + D>(A[r]);
+                    A[r] = cfma(T, oA, A[r]);
+                }
                 T *= oT;
             } else {
                 double oT = dpp::mov<dpp::ROW_SHR + D>(T);
-                double2 oA = dpp::mov<dpp::ROW_SHR + D>(A);
+                double2 oA[NR];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) oA[r] = dpp::mov<dpp::ROW_SHR + D>(A[r]);
                 if (prefix_ok<D>()) {
-                    A = cfma(T, oA, A);
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) A[r] = cfma(T, oA[r], A[r]);
                     T *= oT;
                 }
             }
         }
     }
-    __device__ __forceinline__ void prefix_affine(double& T, double2& A) const {
-        prefix_affine_step<1>(T, A);
-        prefix_affine_step<2>(T, A);
-        prefix_affine_step<4>(T, A);
-        prefix_affine_step<8>(T, A);
+    template <int NR>
+    __device__ __forceinline__ void prefix_affine_n(double& T, double2* A) const {
+        prefix_affine_step<1, NR>(T, A);
+        prefix_affine_step<2, NR>(T, A);
+        prefix_affine_step<4, NR>(T, A);
+        prefix_affine_step<8, NR>(T, A);
         if constexpr (L >= 32) {  // odd rows take the previous row's total (its lane 15)
             double oT = dpp::mov<dpp::ROW_BCAST15, 0xA>(T);
-            double2 oA = dpp::mov<dpp::ROW_BCAST15, 0xA>(A);
+            double2 oA[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) oA[r] = dpp::mov<dpp::ROW_BCAST15, 0xA>(A[r]);
             if (lane & 16) {
-                A = cfma(T, oA, A);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) A[r] = cfma(T, oA[r], A[r]);
                 T *= oT;
             }
         }
         if constexpr (L == 64) {  // upper half takes the lower half's total (lane 31)
             double oT = dpp::mov<dpp::ROW_BCAST31, 0xC>(T);
-            double2 oA = dpp::mov<dpp::ROW_BCAST31, 0xC>(A);
+            double2 oA[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) oA[r] = dpp::mov<dpp::ROW_BCAST31, 0xC>(A[r]);
             if (lane & 32) {
-                A = cfma(T, oA, A);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) A[r] = cfma(T, oA[r], A[r]);
                 T *= oT;
             }
         }
     }
+    __device__ __forceinline__ void prefix_affine(double& T, double2& A) const { prefix_affine_n<1>(T, &A); }
 };
 
 // Per-degree constants of the Chebyshev tau rows and of the bulk mean,
@@ -331,62 +365,72 @@ struct Group {
 
 // ---------------------------------------------------------------------------
 // Chebyshev coefficient vectors in the blocked group layout: lane t holds
-// v[e] = a_{4t+e}.  Entries above degree P must be zero.
+// v[e] = a_{Kt+e}.  Entries above degree P must be zero.
 // ---------------------------------------------------------------------------
+template <int K = 4>
 struct CVec {
-    double2 v[4];
+    double2 v[K];
 };
 
 // d/dy (backward recurrence b_n = b_{n+2} + 2(n+1) a_{n+1}, b_0 halved;
-// _batchdiff!, batchedinfluence.jl:32-43) as two parity suffix sums of 2m a_m.
-template <int L>
-__device__ __forceinline__ CVec cheb_deriv(const Group<L>& g, const CVec& a) {
-    const int n0 = 4 * g.t;
-    double2 c[4];
+// _batchdiff!, batchedinfluence.jl:32-43) as two parity suffix sums of 2m a_m:
+// b_n is the sum of 2m a_m over m > n of the other parity.
+template <int L, int K>
+__device__ __forceinline__ CVec<K> cheb_deriv(const Group<L, K>& g, const CVec<K>& a) {
+    const int n0 = K * g.t;
+    double2 c[K];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) c[e] = cscale(a.v[e], 2.0 * (n0 + e));
-    double2 E = cadd(c[0], c[2]), O = cadd(c[1], c[3]);
-    // inclusive suffix sums over lanes of E and O
-    double2 sE = g.suffix_sum(E), sO = g.suffix_sum(O);
-    // exclusive
-    sE = csub(sE, E);
-    sO = csub(sO, O);
-    CVec b;
-    b.v[3] = sE;
-    b.v[2] = cadd(c[3], sO);
-    b.v[1] = cadd(c[2], sE);
-    b.v[0] = cadd(cadd(c[1], c[3]), sO);
+    for (int e = 0; e < K; ++e) c[e] = cscale(a.v[e], 2.0 * (n0 + e));
+    double2 E = c[0], O = c[1];
+#pragma unroll
+    for (int e = 2; e < K; e += 2) {
+        E = cadd(E, c[e]);
+        O = cadd(O, c[e + 1]);
+    }
+    // exclusive suffix sums over lanes of E and O
+    double2 accE = csub(g.suffix_sum(E), E), accO = csub(g.suffix_sum(O), O);
+    CVec<K> b;
+#pragma unroll
+    for (int e = K - 1; e >= 0; --e) {
+        if (e & 1) {
+            b.v[e] = accE;
+            accO = cadd(c[e], accO);
+        } else {
+            b.v[e] = accO;
+            accE = cadd(c[e], accE);
+        }
+    }
     if (g.t == 0) b.v[0] = cscale(b.v[0], 0.5);
     return b;
 }
 
 // Wall values / derivatives / bulk mean as group reductions.
-template <int L>
-__device__ __forceinline__ double2 wall_deriv_plus(const Group<L>& g, const CVec& a) {  // sum n^2 a_n
+template <int L, int K>
+__device__ __forceinline__ double2 wall_deriv_plus(const Group<L, K>& g, const CVec<K>& a) {  // sum n^2 a_n
     double2 s = czero();
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        double n = 4 * g.t + e;
+    for (int e = 0; e < K; ++e) {
+        double n = K * g.t + e;
         s = cfma(n * n, a.v[e], s);
     }
     return g.sum(s);
 }
-template <int L>
-__device__ __forceinline__ double2 wall_deriv_minus(const Group<L>& g, const CVec& a) {  // sum (-1)^(n+1) n^2 a_n
+template <int L, int K>
+__device__ __forceinline__ double2 wall_deriv_minus(const Group<L, K>& g, const CVec<K>& a) {  // sum (-1)^(n+1) n^2 a_n
     double2 s = czero();
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        double n = 4 * g.t + e;
+    for (int e = 0; e < K; ++e) {
+        double n = K * g.t + e;
         s = cfma(((e & 1) ? 1.0 : -1.0) * n * n, a.v[e], s);
     }
     return g.sum(s);
 }
-template <int L>
-__device__ __forceinline__ double2 bulk_mean(const Group<L>& g, const CVec& a) {  // a_0 + sum_{even m>=2} a_m/(1-m^2)
+template <int L, int K>
+__device__ __forceinline__ double2 bulk_mean(const Group<L, K>& g, const CVec<K>& a) {  // a_0 + sum_{even m>=2} a_m/(1-m^2)
     double2 s = czero();
-    const double* bw = g.tab + 3 * Group<L>::NT;
+    const double* bw = g.tab + 3 * Group<L, K>::NT;
 #pragma unroll
-    for (int e = 0; e < 4; e += 2) s = cfma(bw[4 * g.t + e], a.v[e], s);
+    for (int e = 0; e < K; e += 2) s = cfma(bw[K * g.t + e], a.v[e], s);
     return g.sum(s);
 }
 
@@ -403,125 +447,164 @@ struct HelmCoef {
 };
 __device__ __forceinline__ double cfac_(int m) { return m == 0 ? 2.0 : 1.0; }
 
-template <int L>
-__device__ __forceinline__ CVec helm_solve(const Group<L>& g, int P, double t1, const double fac[4], bool neumann,
-                                           const CVec& f, double2 data_even, double2 data_odd) {
-    const int n0 = 4 * g.t;
-    // neighbouring coefficients of f for the 3-term tau stencil
-    double2 fm2[4], fp2[4];
-    {
-        double2 a2 = g.up1(f.v[2]), a3 = g.up1(f.v[3]);
-        double2 b0 = g.down1(f.v[0]), b1 = g.down1(f.v[1]);
-        bool hasprev = g.t > 0, hasnext = g.t + 1 < L;
-        fm2[0] = hasprev ? a2 : czero();
-        fm2[1] = hasprev ? a3 : czero();
-        fm2[2] = f.v[0];
-        fm2[3] = f.v[1];
-        fp2[0] = f.v[2];
-        fp2[1] = f.v[3];
-        fp2[2] = hasnext ? b0 : czero();
-        fp2[3] = hasnext ? b1 : czero();
-    }
-    // per element: elimination coefficients; A = s*rhs, T = -up*s (backward), alpha = -lo*s
-    double T[4], alpha[4];
-    double2 A[4];
-    constexpr int NT = Group<L>::NT;
+// NR right-hand sides f[r] with their parity wall data (de[r], dd[r]) share
+// the operator: its elimination coefficients, the multiplicative parts of
+// the scans and the wall-row weights are computed once, and the NR
+// independent recurrences interleave (more independent work per scan step).
+template <int NR, int L, int K>
+__device__ __forceinline__ void helm_solve_n(const Group<L, K>& g, int P, double t1, const double fac[K], bool neumann,
+                                             const CVec<K>* f, const double2* de, const double2* dd, CVec<K>* out) {
+    const int n0 = K * g.t;
+    constexpr int NT = Group<L, K>::NT;
+    // per element: elimination coefficients; T = -up*s (backward), alpha = -lo*s
+    double T[K], alpha[K];
+    bool rows[K];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        int n = n0 + e;
-        bool row = (n >= 2) && (n <= P);
-        double s = row ? fac[e] : 0.0;
-        double en2 = (n + 2 <= P - 2) ? 1.0 : 0.0;
-        const double c1 = g.tab[n], c2 = g.tab[NT + n], c3 = g.tab[2 * NT + n];
-        double lo = -t1 * c1;
-        double up = -t1 * en2 * c3;
-        double2 Fm = (n - 2 >= 0 && n - 2 <= P - 2) ? fm2[e] : czero();
-        double2 F0 = (n <= P - 2) ? f.v[e] : czero();
-        double2 Fp = (n + 2 <= P - 2) ? fp2[e] : czero();
-        double2 rhs = czero();
-        if (row) {
-            rhs = cscale(Fm, c1);
-            rhs = cfma(c2, F0, rhs);
-            rhs = cfma(c3, Fp, rhs);
-        }
-        A[e] = cscale(rhs, s);
+    for (int e = 0; e < K; ++e) {
+        const int n = n0 + e;
+        const bool row = (n >= 2) && (n <= P);
+        rows[e] = row;
+        const double s = row ? fac[e] : 0.0;
+        const double en2 = (n + 2 <= P - 2) ? 1.0 : 0.0;
+        const double lo = -t1 * g.tab[n];
+        const double up = -t1 * en2 * g.tab[2 * NT + n];
         T[e] = row ? -up * s : 0.0;
         alpha[e] = row ? -lo * s : (n < 2 ? 1.0 : 0.0);  // j=0: identity (gamma_0 = 1)
     }
+    // A = s * rhs of the 3-term tau stencil in coefficients n - 2, n, n + 2
+    // (elements e -+ 2 of this lane, or of the previous / next lane)
+    double2 A[NR][K];
+    {
+        const bool hasprev = g.t > 0, hasnext = g.t + 1 < L;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            double2 fm2[K], fp2[K];
+            const double2 a2 = g.up1(f[r].v[K - 2]), a3 = g.up1(f[r].v[K - 1]);
+            const double2 b0 = g.down1(f[r].v[0]), b1 = g.down1(f[r].v[1]);
+            fm2[0] = hasprev ? a2 : czero();
+            fm2[1] = hasprev ? a3 : czero();
+            fp2[K - 2] = hasnext ? b0 : czero();
+            fp2[K - 1] = hasnext ? b1 : czero();
+#pragma unroll
+            for (int e = 2; e < K; ++e) {
+                fm2[e] = f[r].v[e - 2];
+                fp2[e - 2] = f[r].v[e];
+            }
+#pragma unroll
+            for (int e = 0; e < K; ++e) {
+                const int n = n0 + e;
+                const double c1 = g.tab[n], c2 = g.tab[NT + n], c3 = g.tab[2 * NT + n];
+                const double2 Fm = (n - 2 >= 0 && n - 2 <= P - 2) ? fm2[e] : czero();
+                const double2 F0 = (n <= P - 2) ? f[r].v[e] : czero();
+                const double2 Fp = (n + 2 <= P - 2) ? fp2[e] : czero();
+                double2 rhs = czero();
+                if (rows[e]) {
+                    rhs = cscale(Fm, c1);
+                    rhs = cfma(c2, F0, rhs);
+                    rhs = cfma(c3, Fp, rhs);
+                }
+                A[r][e] = cscale(rhs, rows[e] ? fac[e] : 0.0);
+            }
+        }
+    }
     // ---- backward sweep: beta_j = A_j + T_j beta_{j+1}, per parity ----
-    // lane composite over its two elements of each parity: j=2t (e=p), 2t+1 (e=p+2)
-    double cT[2];
-    double2 cA[2];
+    // lane composite over its K/2 elements of each parity p (e = p, p+2, ...):
+    // f_p o f_{p+2} o ... o f_{p+K-2}
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        cT[p] = T[p] * T[p + 2];
-        cA[p] = cfma(T[p], A[p + 2], A[p]);
-    }
-    // inclusive suffix scan of composites (f_t o f_{t+1} o ...)
+        double cT = T[K - 2 + p];
+        double2 cA[NR];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) g.suffix_affine(cT[p], cA[p]);
-    // carry = beta at first j of next lane = A-part of inclusive scan at lane t+1
-    double2 beta[4];
-    {
-        bool ok = g.t + 1 < L;
+        for (int r = 0; r < NR; ++r) cA[r] = A[r][K - 2 + p];
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            double2 carry = g.down1(cA[p]);
+        for (int e = K - 4 + p; e >= 0; e -= 2) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cA[r] = cfma(T[e], cA[r], A[r][e]);
+            cT = T[e] * cT;
+        }
+        // inclusive suffix scan of composites (f_t o f_{t+1} o ...); the carry
+        // is beta at the first j of the next lane: the A part at lane t+1
+        g.template suffix_affine_n<NR>(cT, cA);
+        const bool ok = g.t + 1 < L;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            double2 carry = g.down1(cA[r]);
             if (!ok) carry = czero();
-            beta[p + 2] = cfma(T[p + 2], carry, A[p + 2]);
-            beta[p] = cfma(T[p], beta[p + 2], A[p]);
+            A[r][K - 2 + p] = cfma(T[K - 2 + p], carry, A[r][K - 2 + p]);  // A now holds beta
+#pragma unroll
+            for (int e = K - 4 + p; e >= 0; e -= 2) A[r][e] = cfma(T[e], A[r][e + 2], A[r][e]);
         }
     }
     // ---- forward sweep: delta_j = beta_j + alpha_j delta_{j-1}, gamma_j = prod alpha ----
-    double fT[2];
-    double2 fA[2];
+    double gm[K];
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        // composite g_{2t+1} o g_{2t}: x -> beta1 + alpha1 (beta0 + alpha0 x)
-        fT[p] = alpha[p + 2] * alpha[p];
-        fA[p] = cfma(alpha[p + 2], beta[p], beta[p + 2]);
-    }
+        // lane composite g_{p+K-2} o ... o g_{p+2} o g_p: x -> beta + alpha x per element
+        double fT = alpha[p];
+        double2 fA[NR];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) g.prefix_affine(fT[p], fA[p]);  // (later) o (earlier)
-    double2 dl[4];
-    double gm[4];
-    {
-        bool ok = g.t >= 1;
+        for (int r = 0; r < NR; ++r) fA[r] = A[r][p];
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            double2 carryA = g.up1(fA[p]);
-            double carryT = g.up1(fT[p]);
-            if (!ok) { carryA = czero(); carryT = 1.0; }
-            dl[p] = cfma(alpha[p], carryA, beta[p]);
-            gm[p] = alpha[p] * carryT;
-            dl[p + 2] = cfma(alpha[p + 2], dl[p], beta[p + 2]);
-            gm[p + 2] = alpha[p + 2] * gm[p];
+        for (int e = p + 2; e < K; e += 2) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) fA[r] = cfma(alpha[e], fA[r], A[r][e]);
+            fT = alpha[e] * fT;
+        }
+        g.template prefix_affine_n<NR>(fT, fA);  // (later) o (earlier)
+        const bool ok = g.t >= 1;
+        double carryT = g.up1(fT);
+        if (!ok) carryT = 1.0;
+        gm[p] = alpha[p] * carryT;
+#pragma unroll
+        for (int e = p + 2; e < K; e += 2) gm[e] = alpha[e] * gm[e - 2];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            double2 carryA = g.up1(fA[r]);
+            if (!ok) carryA = czero();
+            A[r][p] = cfma(alpha[p], carryA, A[r][p]);  // A now holds delta
+#pragma unroll
+            for (int e = p + 2; e < K; e += 2) A[r][e] = cfma(alpha[e], A[r][e - 2], A[r][e]);
         }
     }
     // ---- wall row: x0 = (data - sum w delta) / sum w gamma ----
-    double2 swd[2] = {czero(), czero()};
     double swg[2] = {0.0, 0.0};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        int n = n0 + e;
-        double w = (n <= P) ? (neumann ? (double)n * n : 1.0) : 0.0;
-        swd[e & 1] = cfma(w, dl[e], swd[e & 1]);
+    for (int e = 0; e < K; ++e) {
+        const int n = n0 + e;
+        const double w = (n <= P) ? (neumann ? (double)n * n : 1.0) : 0.0;
         swg[e & 1] += w * gm[e];
     }
-    CVec out;
+    double inv[2];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        double2 s1 = g.sum(swd[p]);
-        double s2 = g.sum(swg[p]);
-        double2 dat = p == 0 ? data_even : data_odd;
-        double2 x0 = cscale(csub(dat, s1), 1.0 / s2);
+    for (int p = 0; p < 2; ++p) inv[p] = 1.0 / g.sum(swg[p]);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            int e = p + 2 * q;
-            int n = n0 + e;
-            out.v[e] = (n <= P) ? cfma(gm[e], x0, dl[e]) : czero();
+    for (int r = 0; r < NR; ++r) {
+        double2 swd[2] = {czero(), czero()};
+#pragma unroll
+        for (int e = 0; e < K; ++e) {
+            const int n = n0 + e;
+            const double w = (n <= P) ? (neumann ? (double)n * n : 1.0) : 0.0;
+            swd[e & 1] = cfma(w, A[r][e], swd[e & 1]);
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double2 s1 = g.sum(swd[p]);
+            const double2 dat = p == 0 ? de[r] : dd[r];
+            const double2 x0 = cscale(csub(dat, s1), inv[p]);
+#pragma unroll
+            for (int e = p; e < K; e += 2) {
+                const int n = n0 + e;
+                out[r].v[e] = (n <= P) ? cfma(gm[e], x0, A[r][e]) : czero();
+            }
         }
     }
+}
+
+template <int L, int K>
+__device__ __forceinline__ CVec<K> helm_solve(const Group<L, K>& g, int P, double t1, const double fac[K], bool neumann,
+                                              const CVec<K>& f, double2 data_even, double2 data_odd) {
+    CVec<K> out;
+    helm_solve_n<1>(g, P, t1, fac, neumann, &f, &data_even, &data_odd, &out);
     return out;
 }
 

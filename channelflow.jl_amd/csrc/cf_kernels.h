@@ -52,7 +52,7 @@ struct YArgs {
     const double* facP;  // (B, Ny) factors of D^2 - kappa^2 (Dirichlet; kappa^2=1 placeholder at (0,0))
     const double* facPN; // (B, Ny) factors of D^2 - kappa^2 with Neumann rows (pressure init)
     const double* cheb_tab;  // CHEB_TAB_ARRAYS x ntab per-degree constants (build_cheb_table layout)
-    int ntab;                // entries per array (= 4L of the launch)
+    int ntab;                // entries per array (>= K L of the launch)
     Bank bank;
     int form;                 // FORM_*: which y-physical fields the epilogue emits
     int mixed;                // Y_STAGE / Y_PROLOGUE: Nphys and out6 hold FP32 complex (float2)
@@ -155,7 +155,8 @@ hipError_t launch_reduce_columns(const double* part, int nblocks, int nq, double
 // epsilon * N(0,1) samples into out3[3][n] (Philox4x32-10 + Box-Muller, counter = element)
 hipError_t launch_gaussian(double* out3, long long n, double eps, unsigned long long seed, hipStream_t st);
 constexpr int PLANE_STATS_Q = 9;  // per y: means of u, v, w; covariances uu vv ww uv uw vw
-int y_group_lanes(int Ny);
+int y_group_lanes(int Ny);  // lanes L per mode of the wall-normal kernels (0: Ny unsupported)
+int y_group_k(int Ny);      // coefficients K per lane (4, or 8 above 256 points)
 size_t y_lds_bytes(int Ny, int ld);
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st);
 size_t z_lds_bytes(const ZArgs& a);

@@ -2,7 +2,8 @@
 //
 // A block owns M = NT/L consecutive Fourier modes s = ix + Nxh*iz of the
 // reference (k,l,n) layout; a group of L lanes owns one mode, lane t holding
-// Chebyshev coefficients n = 4t..4t+3 in registers.  HBM rows (fixed n, M
+// Chebyshev coefficients n = Kt..Kt+K-1 in registers (K = 4, or 8 when Ny
+// exceeds 256).  HBM rows (fixed n, M
 // consecutive modes) are moved through LDS, which also stages the DCT-I
 // (REDFT00) as a length-2P Stockham FFT of the even extension.
 //
@@ -40,19 +41,19 @@ __device__ __forceinline__ ModeInfo mode_info(const GridDims& gd, long long s) {
     return mi;
 }
 
-// LDS slots (double2) of the per-degree table: CHEB_TAB_ARRAYS arrays of 4L doubles
-template <int L>
-__host__ __device__ constexpr int y_table_slots() { return CHEB_TAB_ARRAYS * 4 * L / 2; }
+// LDS slots (double2) of the per-degree table: CHEB_TAB_ARRAYS arrays of K L doubles
+template <int L, int K>
+__host__ __device__ constexpr int y_table_slots() { return CHEB_TAB_ARRAYS * K * L / 2; }
 
 // ---------------------------------------------------------------------------
 // Per-wavefront tile.  Wave w of block b owns the W = 64/L consecutive modes
 // starting at b*M + w*W, and a private LDS region holding two FFT lines per
 // mode (the DCT-I staging) that also serves as the solve's stash.  Spectral
 // rows move between HBM and the group registers directly (lane t of a mode
-// reads/writes degrees 4t..4t+3; the W modes of one row are contiguous), so
+// reads/writes degrees Kt..Kt+K-1; the W modes of one row are contiguous), so
 // no block-level barrier is needed anywhere after the twiddle table load.
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, int K>
 struct Tile {
     static constexpr int M = YNT / L;  // modes per block
     static constexpr int W = 64 / L;   // modes per wavefront
@@ -76,9 +77,9 @@ struct Tile {
         s = sw0 + mw;
         tw = smem;
         tab = reinterpret_cast<const double*>(smem + ntw);
-        bufA = smem + ntw + y_table_slots<L>() + (size_t)wave * region;
+        bufA = smem + ntw + y_table_slots<L, K>() + (size_t)wave * region;
         bufB = bufA + W * ld;
-        stash = bufA + mw * 12 * L;
+        stash = bufA + mw * 3 * K * L;
     }
     // Non-Nyquist-first order (YOrder): the wave's first slot sw0 becomes a
     // mode index.  W divides nxr, so the wave's W slots lie in one z row and
@@ -96,21 +97,21 @@ struct Tile {
     }
     __device__ __forceinline__ void order_modes(const YOrder& o) { order_modes(o.nreg, o.nxr, o.zskip); }
     // spectral field column of this group's mode (zero beyond P / absent mode)
-    __device__ __forceinline__ CVec load(const double2* __restrict__ src, const Group<L>& g) const {
-        CVec v;
+    __device__ __forceinline__ CVec<K> load(const double2* __restrict__ src, const Group<L, K>& g) const {
+        CVec<K> v;
         const bool ok = s < gd.B;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             v.v[e] = (ok && n < gd.Ny) ? src[s + gd.B * n] : czero();
         }
         return v;
     }
-    __device__ __forceinline__ void store(double2* __restrict__ dst, const Group<L>& g, const CVec& v) const {
+    __device__ __forceinline__ void store(double2* __restrict__ dst, const Group<L, K>& g, const CVec<K>& v) const {
         const bool ok = s < gd.B;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             if (ok && n < gd.Ny) dst[s + gd.B * n] = v.v[e];
         }
     }
@@ -119,8 +120,8 @@ struct Tile {
     // SC: storage of the rows (double2, or float2 widened on load); the
     // transform itself runs in FP64.
     template <typename SC>
-    __device__ __forceinline__ CVec dct_forward(const SC* __restrict__ src, const FFTPlan& plan, double norm,
-                                                const Group<L>& g, bool zero_mode) {
+    __device__ __forceinline__ CVec<K> dct_forward(const SC* __restrict__ src, const FFTPlan& plan, double norm,
+                                                const Group<L, K>& g, bool zero_mode) {
         const int P = gd.P;
         // YLOADS rows' HBM loads in flight before their LDS stores
         constexpr int YLOADS = 4;
@@ -146,10 +147,10 @@ struct Tile {
         }
         wave_sync();
         double2* res = lds_fft<true, true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
-        CVec v;
+        CVec<K> v;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             double w = (n == 0 || n == P) ? 0.5 * norm : norm;
             v.v[e] = (n <= P && !zero_mode) ? cscale(res[mw * ld + n], w) : czero();
         }
@@ -160,12 +161,12 @@ struct Tile {
     // REDFT00, x1/2: InverseFFT!'s chebyshev + copy_to_padded! scale).
     // Computed in FP64; SC = float2 rounds the stored values.
     template <typename SC>
-    __device__ __forceinline__ void dct_inverse(SC* __restrict__ dst, const FFTPlan& plan, const Group<L>& g,
-                                                const CVec& a) {
+    __device__ __forceinline__ void dct_inverse(SC* __restrict__ dst, const FFTPlan& plan, const Group<L, K>& g,
+                                                const CVec<K>& a) {
         const int P = gd.P;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             if (n <= P) {
                 double2 x = a.v[e];
                 bufA[mw * ld + n] = (n == 0 || n == P) ? cscale(x, 2.0) : x;
@@ -184,19 +185,19 @@ struct Tile {
 };
 
 // LDS slots of one wave's region: two FFT lines per mode, or the solve's
-// stash of three coefficient vectors (4L slots each) per mode.
-template <int L>
+// stash of three coefficient vectors (K L slots each) per mode.
+template <int L, int K>
 __host__ __device__ constexpr int y_region_slots(int ld) {
-    return (2 * (64 / L) * ld > 12 * 64) ? 2 * (64 / L) * ld : 12 * 64;
+    return (2 * (64 / L) * ld > 3 * K * 64) ? 2 * (64 / L) * ld : 3 * K * 64;
 }
 
 // Block prologue of every y kernel: the DCT twiddle table and the
 // per-degree table into LDS.
-template <int L>
+template <int L, int K>
 __device__ __forceinline__ void load_tables(double2* smem, const YArgs& a) {
     for (int i = threadIdx.x; i < a.dct.n; i += YNT) smem[i] = a.dct.tw[i];
     double* tab = reinterpret_cast<double*>(smem + a.dct.n);
-    constexpr int NT = 4 * L;
+    constexpr int NT = K * L;
     for (int i = threadIdx.x; i < CHEB_TAB_ARRAYS * NT; i += YNT) {
         const int arr = i / NT, n = i - arr * NT;
         tab[i] = a.cheb_tab[arr * a.ntab + n];
@@ -206,35 +207,35 @@ __device__ __forceinline__ void load_tables(double2* smem, const YArgs& a) {
 
 // Lane-private LDS stash of a coefficient vector (each lane writes and reads
 // back only its own four slots, so no barrier is needed between the two).
-template <int L>
-__device__ __forceinline__ void stash(double2* slot, const Group<L>& g, const CVec& v) {
+template <int L, int K>
+__device__ __forceinline__ void stash(double2* slot, const Group<L, K>& g, const CVec<K>& v) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) slot[4 * g.t + e] = v.v[e];
+    for (int e = 0; e < K; ++e) slot[K * g.t + e] = v.v[e];
 }
-template <int L>
-__device__ __forceinline__ CVec unstash(const double2* slot, const Group<L>& g) {
-    CVec v;
+template <int L, int K>
+__device__ __forceinline__ CVec<K> unstash(const double2* slot, const Group<L, K>& g) {
+    CVec<K> v;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) v.v[e] = slot[4 * g.t + e];
+    for (int e = 0; e < K; ++e) v.v[e] = slot[K * g.t + e];
     return v;
 }
 
 // coefficient of degree n (broadcast to the group)
-template <int L>
-__device__ __forceinline__ double2 coef_at(const Group<L>& g, const CVec& a, int n) {
+template <int L, int K>
+__device__ __forceinline__ double2 coef_at(const Group<L, K>& g, const CVec<K>& a, int n) {
     double2 v = czero();
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-        if (4 * g.t + e == n) v = a.v[e];
+    for (int e = 0; e < K; ++e)
+        if (K * g.t + e == n) v = a.v[e];
     return g.sum(v);
 }
 
-template <int L>
-__device__ __forceinline__ void load_fac(const double* __restrict__ f, long long s, int Ny, const Group<L>& g,
-                                         double out[4]) {
+template <int L, int K>
+__device__ __forceinline__ void load_fac(const double* __restrict__ f, long long s, int Ny, const Group<L, K>& g,
+                                         double out[K]) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        int n = 4 * g.t + e;
+    for (int e = 0; e < K; ++e) {
+        int n = K * g.t + e;
         out[e] = (n < Ny) ? f[s * Ny + n] : 0.0;
     }
 }
@@ -251,47 +252,47 @@ __device__ __forceinline__ void load_fac(const double* __restrict__ f, long long
 //   divergence V                   (nonlinear.jl:146-167)
 // get(c) returns U component c: from registers in the stage kernel, from HBM
 // in the prologue (loaded where first needed, which shortens live ranges).
-template <int L, typename SC, typename GetU>
-__device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const Group<L>& g, const ModeInfo& mi,
+template <int L, int K, typename SC, typename GetU>
+__device__ __forceinline__ void inverse_six(Tile<L, K>& tile, const YArgs& a, const Group<L, K>& g, const ModeInfo& mi,
                                             GetU get) {
     SC* out6[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) out6[i] = reinterpret_cast<SC*>(a.out6[i]);
-    CVec V0 = get(0);
+    CVec<K> V0 = get(0);
     if (mi.mean && mi.valid) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             if (n < a.gd.Ny) V0.v[e] = cadd(V0.v[e], cmk(a.base[n], 0.0));
         }
     }
     if (a.form != FORM_ROTATING) {
         tile.dct_inverse(out6[0], a.dct, g, V0);
         if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[3], a.dct, g, cheb_deriv(g, V0));
-        const CVec V1 = get(1);
+        const CVec<K> V1 = get(1);
         tile.dct_inverse(out6[1], a.dct, g, V1);
         if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[4], a.dct, g, cheb_deriv(g, V1));
-        const CVec V2 = get(2);
+        const CVec<K> V2 = get(2);
         tile.dct_inverse(out6[2], a.dct, g, V2);
         if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[5], a.dct, g, cheb_deriv(g, V2));
         return;
     }
     // order chosen so each velocity component dies as early as possible
-    const CVec V1 = get(1);
-    CVec W;
+    const CVec<K> V1 = get(1);
+    CVec<K> W;
     tile.dct_inverse(out6[0], a.dct, g, V0);
     {
-        CVec d1 = cheb_deriv(g, V0);
+        CVec<K> d1 = cheb_deriv(g, V0);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) W.v[e] = csub(cmuli(cscale(V1.v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
+        for (int e = 0; e < K; ++e) W.v[e] = csub(cmuli(cscale(V1.v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
     }
     tile.dct_inverse(out6[5], a.dct, g, W);
-    const CVec V2 = get(2);
+    const CVec<K> V2 = get(2);
     tile.dct_inverse(out6[2], a.dct, g, V2);
     {
-        CVec d3 = cheb_deriv(g, V2);
+        CVec<K> d3 = cheb_deriv(g, V2);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V1.v[e], mi.kz)));  // dy V3 - i l V2
+        for (int e = 0; e < K; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V1.v[e], mi.kz)));  // dy V3 - i l V2
     }
     tile.dct_inverse(out6[3], a.dct, g, W);
     tile.dct_inverse(out6[1], a.dct, g, V1);
@@ -302,35 +303,35 @@ __device__ __forceinline__ void inverse_six(Tile<L>& tile, const YArgs& a, const
 // correction; see the derivation in the library documentation).  Returns
 // U[3], P.  mean/nyq modes handled separately by the caller.
 // ---------------------------------------------------------------------------
-template <int L>
-__device__ __forceinline__ void solve_nonzero(const Group<L>& g, const YArgs& a, const Bank& bk, const ModeInfo& mi,
+template <int L, int K>
+__device__ __forceinline__ void solve_nonzero(const Group<L, K>& g, const YArgs& a, const Bank& bk, const ModeInfo& mi,
                                               const double2* Rs0, const double2* Rs1, const double2* Rs2,
-                                              CVec Uout[3], CVec& Pout) {
+                                              CVec<K> Uout[3], CVec<K>& Pout) {
     const int P = a.gd.P, Ny = a.gd.Ny;
     const double* mc = bk.mconst + mi.s * MC_COUNT;
     const double shift = bk.lambda + bk.nu * mi.kappa2;
-    double fp[4], fv[4];
+    double fp[K], fv[K];
     load_fac(a.facP, mi.s, Ny, g, fp);
     // r = D Ry + i k Rx + i l Rz
-    CVec r;
+    CVec<K> r;
     {
-        CVec R1 = unstash(Rs1, g);
+        CVec<K> R1 = unstash(Rs1, g);
         r = cheb_deriv(g, R1);
-        CVec R0 = unstash(Rs0, g), R2 = unstash(Rs2, g);
+        CVec<K> R0 = unstash(Rs0, g), R2 = unstash(Rs2, g);
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int e = 0; e < K; ++e)
             r.v[e] = cadd(r.v[e], cadd(cmuli(cscale(R0.v[e], mi.kx)), cmuli(cscale(R2.v[e], mi.kz))));
     }
     double2 p1P, v1a, v1b, bp, bm;
     {
-        CVec p1 = helm_solve(g, P, mi.kappa2, fp, false, r, czero(), czero());
+        CVec<K> p1 = helm_solve(g, P, mi.kappa2, fp, false, r, czero(), czero());
         p1P = coef_at(g, p1, P);
-        CVec q = cheb_deriv(g, p1);
-        CVec R1 = unstash(Rs1, g);
+        CVec<K> q = cheb_deriv(g, p1);
+        CVec<K> R1 = unstash(Rs1, g);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R1.v[e]);
+        for (int e = 0; e < K; ++e) q.v[e] = csub(q.v[e], R1.v[e]);
         load_fac(bk.facV, mi.s, Ny, g, fv);
-        CVec v1 = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+        CVec<K> v1 = helm_solve(g, P, shift, fv, false, q, czero(), czero());
         bp = cscale(wall_deriv_plus(g, v1), -1.0);
         bm = cscale(wall_deriv_minus(g, v1), -1.0);
         v1a = coef_at(g, v1, P - 1);
@@ -340,7 +341,7 @@ __device__ __forceinline__ void solve_nonzero(const Group<L>& g, const YArgs& a,
     double2 dm = cadd(cscale(bp, mc[MC_A21]), cscale(bm, mc[MC_A22]));
     double2 ra, rb;
     {
-        CVec R1 = unstash(Rs1, g);
+        CVec<K> R1 = unstash(Rs1, g);
         ra = coef_at(g, R1, P - 1);
         rb = coef_at(g, R1, P);
     }
@@ -351,8 +352,8 @@ __device__ __forceinline__ void solve_nonzero(const Group<L>& g, const YArgs& a,
     double2 s2 = cscale(csub(cscale(v2b, shift), rb), 1.0 / (1.0 - mc[MC_SIG2]));
     // final pressure: rhs r + s1 D T_{P-1} + s2 D T_P, wall data per parity
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        int n = 4 * g.t + e;
+    for (int e = 0; e < K; ++e) {
+        int n = K * g.t + e;
         if (n % 2 == 0) {
             if (n < P - 1) r.v[e] = cfma(n == 0 ? (double)(P - 1) : 2.0 * (P - 1), s1, r.v[e]);
         } else {
@@ -363,47 +364,43 @@ __device__ __forceinline__ void solve_nonzero(const Group<L>& g, const YArgs& a,
     double2 dd = cadd(cscale(csub(dp, dm), 0.5), cscale(s2, 0.5 * (mc[MC_D0P] - mc[MC_D0M])));
     Pout = helm_solve(g, P, mi.kappa2, fp, false, r, de, dd);
     __builtin_amdgcn_sched_barrier(0);
-    CVec q;
+    // the three velocity solves share the operator: one fused solve
+    CVec<K> q[3];
     {
-        CVec R0 = unstash(Rs0, g);
+        const CVec<K> R0 = unstash(Rs0, g), R2 = unstash(Rs2, g);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kx)), R0.v[e]);
+        for (int e = 0; e < K; ++e) {
+            q[0].v[e] = csub(cmuli(cscale(Pout.v[e], mi.kx)), R0.v[e]);
+            q[2].v[e] = csub(cmuli(cscale(Pout.v[e], mi.kz)), R2.v[e]);
+        }
     }
-    Uout[0] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
-    __builtin_amdgcn_sched_barrier(0);
+    q[1] = cheb_deriv(g, Pout);
     {
-        CVec R2 = unstash(Rs2, g);
+        const CVec<K> R1 = unstash(Rs1, g);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) q.v[e] = csub(cmuli(cscale(Pout.v[e], mi.kz)), R2.v[e]);
+        for (int e = 0; e < K; ++e) q[1].v[e] = csub(q[1].v[e], R1.v[e]);
     }
-    Uout[2] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
-    __builtin_amdgcn_sched_barrier(0);
-    q = cheb_deriv(g, Pout);
-    {
-        CVec R1 = unstash(Rs1, g);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) q.v[e] = csub(q.v[e], R1.v[e]);
-    }
-    Uout[1] = helm_solve(g, P, shift, fv, false, q, czero(), czero());
+    const double2 zero3[3] = {czero(), czero(), czero()};
+    helm_solve_n<3>(g, P, shift, fv, false, q, zero3, zero3, Uout);
 }
 
 // mean mode (meanmode.jl:103-150); returns the new gradients
-template <int L>
-__device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, const Bank& bk, const double2* Rs0,
-                                           const double2* Rs1, const double2* Rs2, CVec Uout[3], CVec& Pout,
+template <int L, int K>
+__device__ __forceinline__ void solve_mean(const Group<L, K>& g, const YArgs& a, const Bank& bk, const double2* Rs0,
+                                           const double2* Rs1, const double2* Rs2, CVec<K> Uout[3], CVec<K>& Pout,
                                            double gout[2]) {
     const int P = a.gd.P, Ny = a.gd.Ny;
-    const CVec R[3] = {unstash(Rs0, g), unstash(Rs1, g), unstash(Rs2, g)};
+    const CVec<K> R[3] = {unstash(Rs0, g), unstash(Rs1, g), unstash(Rs2, g)};
     // pressure: integrate Ry through degree P-1, zero bulk mean
     {
-        double2 prev3 = g.up1(R[1].v[3]);
+        double2 prev3 = g.up1(R[1].v[K - 1]);
         double2 next0 = g.down1(R[1].v[0]);
-        CVec r = R[1];
+        CVec<K> r = R[1];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             double2 rm = (e == 0) ? prev3 : r.v[e - 1];  // r_{n-1}
-            double2 rp = (e == 3) ? next0 : r.v[e + 1];  // r_{n+1}
+            double2 rp = (e == K - 1) ? next0 : r.v[e + 1];  // r_{n+1}
             double2 lower = (n == 1) ? rm : cscale(rm, 0.5);
             double2 upper = (n + 1 < P) ? cscale(rp, 0.5) : czero();
             Pout.v[e] = (n >= 1 && n <= P) ? cscale(csub(lower, upper), 1.0 / n) : czero();
@@ -412,15 +409,15 @@ __device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, co
         if (g.t == 0) Pout.v[0] = cscale(m, -1.0);
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) Uout[1].v[e] = czero();
-    double fv[4];
+    for (int e = 0; e < K; ++e) Uout[1].v[e] = czero();
+    double fv[K];
     load_fac(bk.facV, 0, Ny, g, fv);
     for (int c = 0; c < 2; ++c) {
-        const CVec& src = R[c == 0 ? 0 : 2];
-        CVec f;
+        const CVec<K>& src = R[c == 0 ? 0 : 2];
+        CVec<K> f;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) f.v[e] = cscale(src.v[e], -1.0);
-        CVec u = helm_solve(g, P, bk.lambda, fv, false, f, czero(), czero());
+        for (int e = 0; e < K; ++e) f.v[e] = cscale(src.v[e], -1.0);
+        CVec<K> u = helm_solve(g, P, bk.lambda, fv, false, f, czero(), czero());
         double grad;
         if (a.bulk) {
             double target = c == 0 ? a.c0 : a.c1;
@@ -429,8 +426,8 @@ __device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, co
             grad = c == 0 ? a.c0 : a.c1;
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             if (n < Ny) u.v[e] = cadd(u.v[e], cmk(grad * bk.resp[n], 0.0));
         }
         Uout[c == 0 ? 0 : 2] = u;
@@ -438,18 +435,18 @@ __device__ __forceinline__ void solve_mean(const Group<L>& g, const YArgs& a, co
     }
 }
 
-template <int L>
-__device__ __forceinline__ void zero_vec(CVec& v) {
+template <int L, int K>
+__device__ __forceinline__ void zero_vec(CVec<K>& v) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) v.v[e] = czero();
+    for (int e = 0; e < K; ++e) v.v[e] = czero();
 }
 
 // Laplacian D^2 a - kappa^2 a
-template <int L>
-__device__ __forceinline__ CVec laplacian(const Group<L>& g, const CVec& a, double kappa2) {
-    CVec d = cheb_deriv(g, cheb_deriv(g, a));
+template <int L, int K>
+__device__ __forceinline__ CVec<K> laplacian(const Group<L, K>& g, const CVec<K>& a, double kappa2) {
+    CVec<K> d = cheb_deriv(g, cheb_deriv(g, a));
 #pragma unroll
-    for (int e = 0; e < 4; ++e) d.v[e] = cfma(-kappa2, a.v[e], d.v[e]);
+    for (int e = 0; e < K; ++e) d.v[e] = cfma(-kappa2, a.v[e], d.v[e]);
     return d;
 }
 
@@ -458,13 +455,13 @@ __device__ __forceinline__ CVec laplacian(const Group<L>& g, const CVec& a, doub
 // Stokes solve, state write, inverse transforms for the next stage.
 // ---------------------------------------------------------------------------
 // SC: storage of the y-physical fields (double2; float2 in the mixed step).
-template <int L, typename SC>
+template <int L, int K, typename SC>
 __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), block);
     if (a.order.on) tile.order_modes(a.order);
-    Group<L> g(tile.tab);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny;
     const bool meanmode = mi.mean && mi.valid;
@@ -472,48 +469,48 @@ __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     // Per component: forward DCT of the nonlinear term, Q = N (stage 1) or
     // A_j Q + N, R_c = lambda U_c + nu Lap U_c - d_c P + (B_j/C_j) Q_c.
     double gold0 = a.c0, gold1 = a.c1;
-    CVec Pold = tile.load(a.P, g);
-    CVec R[3];
+    CVec<K> Pold = tile.load(a.P, g);
+    CVec<K> R[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        CVec Qc;
+        CVec<K> Qc;
         if (a.Ncoef[c]) {
             Qc = tile.load(a.Ncoef[c], g);
         } else {
             Qc = tile.dct_forward(reinterpret_cast<const SC*>(a.Nphys[c]), a.dct, a.norm, g, mi.nyq);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Qc.v[e] = cscale(Qc.v[e], a.nl_sign);
+            for (int e = 0; e < K; ++e) Qc.v[e] = cscale(Qc.v[e], a.nl_sign);
         }
         if (a.Fcoef[c]) {  // forcing(tstage, U, N): N += F (cnrk2.jl:175)
-            const CVec F = tile.load(a.Fcoef[c], g);
+            const CVec<K> F = tile.load(a.Fcoef[c], g);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Qc.v[e] = cadd(Qc.v[e], F.v[e]);
+            for (int e = 0; e < K; ++e) Qc.v[e] = cadd(Qc.v[e], F.v[e]);
         }
         double bN = 0.0;
         if (a.bulk && meanmode && c != 1) bN = bulk_mean(g, Qc).x;
         if (a.stage != 0) {
-            CVec Qo = tile.load(a.Q[c], g);
+            CVec<K> Qo = tile.load(a.Q[c], g);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) Qc.v[e] = cfma(a.Aj, Qo.v[e], Qc.v[e]);
+            for (int e = 0; e < K; ++e) Qc.v[e] = cfma(a.Aj, Qo.v[e], Qc.v[e]);
         }
         if (a.store_Q) tile.store(a.Q[c], g, Qc);
-        CVec U = tile.load(a.U[c], g);
+        CVec<K> U = tile.load(a.U[c], g);
         if (a.bulk && meanmode && c != 1) {  // _bulkpressuregradient, cnrk2.jl:98-107
             double2 sh = csub(wall_deriv_plus(g, U), wall_deriv_minus(g, U));
             double gv = a.nu * sh.x / 2.0 + (c == 0 ? a.nu * a.curv_mean : 0.0) + bN;
             if (c == 0) gold0 = gv; else gold1 = gv;
         }
-        CVec lap = laplacian(g, U, mi.kappa2);
-        CVec dP;
+        CVec<K> lap = laplacian(g, U, mi.kappa2);
+        CVec<K> dP;
         if (c == 1) {
             dP = cheb_deriv(g, Pold);
         } else {
             double kk = (c == 0) ? mi.kx : mi.kz;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) dP.v[e] = cmuli(cscale(Pold.v[e], kk));
+            for (int e = 0; e < K; ++e) dP.v[e] = cmuli(cscale(Pold.v[e], kk));
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < K; ++e) {
             double2 x = cscale(U.v[e], a.lambda);
             x = cfma(a.nu, lap.v[e], x);
             x = csub(x, dP.v[e]);
@@ -526,8 +523,8 @@ __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     }
     if (meanmode) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             if (n < Ny) R[0].v[e] = cadd(R[0].v[e], cmk(2.0 * a.nu * a.curv[n], 0.0));
         }
         if (g.t == 0) {
@@ -537,29 +534,29 @@ __device__ __forceinline__ void ystage_body(const YArgs& a, unsigned block) {
     }
     double2* st = tile.stash;
     stash(st, g, R[0]);
-    stash(st + 4 * L, g, R[1]);
-    stash(st + 8 * L, g, R[2]);
-    CVec Un[3], Pn;
+    stash(st + K * L, g, R[1]);
+    stash(st + 2 * K * L, g, R[2]);
+    CVec<K> Un[3], Pn;
     if (meanmode) {
         double gnew[2] = {0.0, 0.0};
-        solve_mean(g, a, a.bank, st, st + 4 * L, st + 8 * L, Un, Pn, gnew);
+        solve_mean(g, a, a.bank, st, st + K * L, st + 2 * K * L, Un, Pn, gnew);
         if (g.t == 0 && a.grad_out) {
             a.grad_out[0] = gnew[0];
             a.grad_out[1] = gnew[1];
         }
     } else if (mi.nyq || !mi.valid) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
-        zero_vec<L>(Pn);
+        for (int c = 0; c < 3; ++c) zero_vec<L, K>(Un[c]);
+        zero_vec<L, K>(Pn);
     } else {
-        solve_nonzero(g, a, a.bank, mi, st, st + 4 * L, st + 8 * L, Un, Pn);
+        solve_nonzero(g, a, a.bank, mi, st, st + K * L, st + 2 * K * L, Un, Pn);
     }
     wave_sync();  // the stash aliases the wave's FFT lines reused below
     tile.store(a.U[0], g, Un[0]);
     tile.store(a.U[1], g, Un[1]);
     tile.store(a.U[2], g, Un[2]);
     tile.store(a.P, g, Pn);
-    if (a.out6[0]) inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return Un[c]; });
+    if (a.out6[0]) inverse_six<L, K, SC>(tile, a, g, mi, [&](int c) { return Un[c]; });
 }
 
 // Nyquist modes of a stage under the non-Nyquist-first order, elementwise
@@ -597,10 +594,12 @@ __device__ __forceinline__ void ystage_nyquist(const YArgs& a, unsigned blk) {
     }
 }
 
-template <int L, typename SC>
+template <int L, int K, typename SC>
 // two waves per SIMD (<= 256 registers per lane): one wave per SIMD leaves
-// the wave-serial scans and LDS FFTs unhidden
-__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) ystage_kernel(YArgs a) {
+// the wave-serial scans and LDS FFTs unhidden.  K = 8 (Ny above 256) keeps
+// twice the coefficients per lane and runs one wave per SIMD, whose larger
+// register budget holds them without scratch.
+__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(K == 8 ? 1 : 2))) ystage_kernel(YArgs a) {
     if (a.order.on) {  // block-uniform branches
         if (blockIdx.x >= a.order.nyq_first) {
             ystage_nyquist<SC>(a, blockIdx.x - a.order.nyq_first);
@@ -608,21 +607,21 @@ __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2))) y
         }
         // z rows of Nxh modes leave the blocks' row segments unaligned to
         // cache lines: neighbouring blocks share lines, so they run on one XCD
-        ystage_body<L, SC>(a, a.order.xcd ? xcd_logical(blockIdx.x, a.order.nyq_first) : blockIdx.x);
+        ystage_body<L, K, SC>(a, a.order.xcd ? xcd_logical(blockIdx.x, a.order.nyq_first) : blockIdx.x);
         return;
     }
-    ystage_body<L, SC>(a, blockIdx.x);
+    ystage_body<L, K, SC>(a, blockIdx.x);
 }
 
 // prologue: state U -> six y-physical fields (velocity and vorticity)
-template <int L, typename SC>
+template <int L, int K, typename SC>
 __device__ __forceinline__ void yprologue_body(const YArgs& a, unsigned block) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), block);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
-    inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
+    inverse_six<L, K, SC>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
 }
 
 // The single-problem prologue keeps its own copy of the body: passing the
@@ -631,73 +630,73 @@ __device__ __forceinline__ void yprologue_body(const YArgs& a, unsigned block) {
 // order (a.order.on) the launch has only the regular blocks: the periodic
 // passes never read the Nyquist modes' y-physical values, so none are
 // written.
-template <int L, typename SC>
+template <int L, int K, typename SC>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
+    load_tables<L, K>(smem, a);
     // the order's fields are read one by one (a reference to the byval
     // argument, or a copy of the whole struct, lands in scratch)
     const bool ordered = a.order.on != 0;
     const unsigned block = (ordered && a.order.xcd) ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), block);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), block);
     if (ordered) tile.order_modes(a.order.nreg, a.order.nxr, a.order.zskip);
-    Group<L> g(tile.tab);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     if constexpr (L == 4) {  // all loads first: the 4-lane build stays within 128 registers so
-        CVec U[3];
+        CVec<K> U[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
-        inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return U[c]; });
+        inverse_six<L, K, SC>(tile, a, g, mi, [&](int c) { return U[c]; });
     } else {
-        inverse_six<L, SC>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
+        inverse_six<L, K, SC>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
     }
 }
 
 // Batched ensemble launches: instance i owns blocks [start[i], start[i+1])
 // of the grid and its own argument block tab[i] (grid, factors, fields).
 // All instances of one launch share the group size L.
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(2)))
 ystage_batch_kernel(const YArgs* __restrict__ tab, const unsigned* __restrict__ start, int n) {
     const int i = batch_instance(start, n, blockIdx.x);
-    ystage_body<L, double2>(tab[i], blockIdx.x - start[i]);
+    ystage_body<L, K, double2>(tab[i], blockIdx.x - start[i]);
 }
 
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) yprologue_batch_kernel(const YArgs* __restrict__ tab,
                                                               const unsigned* __restrict__ start, int n) {
     const int i = batch_instance(start, n, blockIdx.x);
-    yprologue_body<L, double2>(tab[i], blockIdx.x - start[i]);
+    yprologue_body<L, K, double2>(tab[i], blockIdx.x - start[i]);
 }
 
 // Stokes solve! of a bank: R (3 spectral) -> U (3), P (stokes.jl:81-108)
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), blockIdx.x);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
-    CVec R[3];
+    CVec<K> R[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) R[c] = tile.load(a.Rin[c], g);
-    CVec Un[3], Pn;
+    CVec<K> Un[3], Pn;
     double gnew[2] = {0.0, 0.0};
     double2* st = tile.stash;
     stash(st, g, R[0]);
-    stash(st + 4 * L, g, R[1]);
-    stash(st + 8 * L, g, R[2]);
+    stash(st + K * L, g, R[1]);
+    stash(st + 2 * K * L, g, R[2]);
     if (mi.mean && mi.valid) {
-        solve_mean(g, a, a.bank, st, st + 4 * L, st + 8 * L, Un, Pn, gnew);
+        solve_mean(g, a, a.bank, st, st + K * L, st + 2 * K * L, Un, Pn, gnew);
         if (g.t == 0 && a.grad_out) {
             a.grad_out[0] = gnew[0];
             a.grad_out[1] = gnew[1];
         }
     } else if (mi.nyq || !mi.valid) {
-        for (int c = 0; c < 3; ++c) zero_vec<L>(Un[c]);
-        zero_vec<L>(Pn);
+        for (int c = 0; c < 3; ++c) zero_vec<L, K>(Un[c]);
+        zero_vec<L, K>(Pn);
     } else {
-        solve_nonzero(g, a, a.bank, mi, st, st + 4 * L, st + 8 * L, Un, Pn);
+        solve_nonzero(g, a, a.bank, mi, st, st + K * L, st + 2 * K * L, Un, Pn);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) tile.store(a.U[c], g, Un[c]);
@@ -705,44 +704,44 @@ __global__ void __launch_bounds__(YNT) ysolve_kernel(YArgs a) {
 }
 
 // R = -Laplacian(U) (projection source, pressure.jl:125)
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) yneglap_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), blockIdx.x);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < 3; ++c) {
-        CVec U = tile.load(a.U[c], g);
-        CVec lap = laplacian(g, U, mi.kappa2);
+        CVec<K> U = tile.load(a.U[c], g);
+        CVec<K> lap = laplacian(g, U, mi.kappa2);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) lap.v[e] = cscale(lap.v[e], -1.0);
+        for (int e = 0; e < K; ++e) lap.v[e] = cscale(lap.v[e], -1.0);
         tile.store(a.Rout[c], g, lap);
     }
 }
 
 // pressure reconstruction (_pressure_poisson, pressure.jl:196-285):
 // A (3 spectral accelerations), v = U[1] -> P
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), blockIdx.x);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int P = a.gd.P, Ny = a.gd.Ny;
-    CVec A[3];
+    CVec<K> A[3];
     for (int c = 0; c < 3; ++c) A[c] = tile.load(a.Rin[c], g);
-    CVec v = tile.load(a.U[1], g);
-    CVec Pn;
+    CVec<K> v = tile.load(a.U[1], g);
+    CVec<K> Pn;
     if (mi.mean && mi.valid) {
-        double2 prev3 = g.up1(A[1].v[3]);
+        double2 prev3 = g.up1(A[1].v[K - 1]);
         double2 next0 = g.down1(A[1].v[0]);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             double2 rm = (e == 0) ? prev3 : A[1].v[e - 1];
-            double2 rp = (e == 3) ? next0 : A[1].v[e + 1];
+            double2 rp = (e == K - 1) ? next0 : A[1].v[e + 1];
             double2 lower = (n == 1) ? rm : cscale(rm, 0.5);
             double2 upper = (n + 1 < Ny - 1) ? cscale(rp, 0.5) : czero();
             Pn.v[e] = (n >= 1 && n <= P) ? cscale(csub(lower, upper), 1.0 / n) : czero();
@@ -750,18 +749,18 @@ __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
         double2 m = bulk_mean(g, Pn);
         if (g.t == 0) Pn.v[0] = cscale(m, -1.0);
     } else if (mi.nyq || !mi.valid) {
-        zero_vec<L>(Pn);
+        zero_vec<L, K>(Pn);
     } else {
         // div A
-        CVec r = cheb_deriv(g, A[1]);
+        CVec<K> r = cheb_deriv(g, A[1]);
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int e = 0; e < K; ++e)
             r.v[e] = cadd(r.v[e], cadd(cmuli(cscale(A[0].v[e], mi.kx)), cmuli(cscale(A[2].v[e], mi.kz))));
         // wall data P_y(+-1) = A_y(+-1) + nu v_yy(+-1)
         double2 up = czero(), lo = czero();
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             if (n < Ny) {
                 double nn = n;
                 double2 term = cfma(a.nu * (nn * nn * (nn * nn - 1.0) / 3.0), v.v[e], A[1].v[e]);
@@ -771,7 +770,7 @@ __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
         }
         up = g.sum(up);
         lo = g.sum(lo);
-        double fp[4];
+        double fp[K];
         load_fac(a.facPN, mi.s, Ny, g, fp);
         // Neumann parity data: even (g+ - g-)/2, odd (g+ + g-)/2
         Pn = helm_solve(g, P, mi.kappa2, fp, true, r, cscale(csub(up, lo), 0.5), cscale(cadd(up, lo), 0.5));
@@ -782,21 +781,21 @@ __global__ void __launch_bounds__(YNT) ypressure_kernel(YArgs a) {
 // Batched Chebyshev transforms of spectral fields (no group logic).
 // dir < 0: y-physical values -> normalised coefficients (normalize_forward!)
 // dir > 0: coefficients -> values at the Lobatto points
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), blockIdx.x);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     for (int c = 0; c < a.nfields; ++c) {
         if (dir < 0) {
-            CVec v = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
+            CVec<K> v = tile.dct_forward(a.Nphys[c], a.dct, a.norm, g, mi.nyq);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) v.v[e] = cscale(v.v[e], a.nl_sign);
+            for (int e = 0; e < K; ++e) v.v[e] = cscale(v.v[e], a.nl_sign);
             tile.store(a.Rout[c], g, v);
         } else {
-            CVec v = tile.load(a.Rin[c], g);
+            CVec<K> v = tile.load(a.Rin[c], g);
             tile.dct_inverse(a.out6[c], a.dct, g, v);
         }
     }
@@ -805,29 +804,29 @@ __global__ void __launch_bounds__(YNT) ydct_kernel(YArgs a, int dir) {
 // Divergence form (nonlinear.jl:146-167, div! of the flux tensor): the six
 // y-physical products T = (VxVx, VxVy, VxVz, VyVy, VyVz, VzVz) are forward
 // transformed and N_i = -(i k T_ix + d/dy T_iy + i l T_iz) is written to Rout.
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) ydivergence_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), blockIdx.x);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
-    CVec N[3];
+    CVec<K> N[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) zero_vec<L>(N[c]);
+    for (int c = 0; c < 3; ++c) zero_vec<L, K>(N[c]);
     // tensor entry (i, j) for products p = 0..5
     const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-        CVec T = tile.dct_forward(a.Nphys[p], a.dct, a.norm, g, mi.nyq);
-        CVec dT = cheb_deriv(g, T);
+        CVec<K> T = tile.dct_forward(a.Nphys[p], a.dct, a.norm, g, mi.nyq);
+        CVec<K> dT = cheb_deriv(g, T);
         // contribution to N_i from d_j T_ij and, off the diagonal, to N_j from d_i T_ij
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             if (side == 1 && pi[p] == pj[p]) continue;
             const int i = side == 0 ? pi[p] : pj[p], j = side == 0 ? pj[p] : pi[p];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < K; ++e) {
                 double2 d = (j == 1) ? dT.v[e] : cmuli(cscale(T.v[e], j == 0 ? mi.kx : mi.kz));
                 N[i].v[e] = csub(N[i].v[e], d);
             }
@@ -844,21 +843,21 @@ __global__ void __launch_bounds__(YNT) ydivergence_kernel(YArgs a) {
 // excluded.  Y_DOT: sum over the three components of U (U) and V (Rin).
 // Y_DISSIPATION: u = v = curl U (operators.jl:172-190); the caller scales by nu.
 // Each block writes its partial sum to partial[blockIdx.x].
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) ydot_kernel(YArgs a, int curl) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), blockIdx.x);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny;
     double2* vline = tile.stash;  // this group's copy of v, indexed by degree
-    CVec W0, W1, W2;
+    CVec<K> W0, W1, W2;
     if (curl) {
-        CVec U0 = tile.load(a.U[0], g), U1 = tile.load(a.U[1], g), U2 = tile.load(a.U[2], g);
-        CVec d2 = cheb_deriv(g, U2), d0 = cheb_deriv(g, U0);
+        CVec<K> U0 = tile.load(a.U[0], g), U1 = tile.load(a.U[1], g), U2 = tile.load(a.U[2], g);
+        CVec<K> d2 = cheb_deriv(g, U2), d0 = cheb_deriv(g, U0);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < K; ++e) {
             W0.v[e] = csub(d2.v[e], cmuli(cscale(U1.v[e], mi.kz)));
             W1.v[e] = cmuli(csub(cscale(U0.v[e], mi.kz), cscale(U2.v[e], mi.kx)));
             W2.v[e] = csub(cmuli(cscale(U1.v[e], mi.kx)), d0.v[e]);
@@ -867,7 +866,7 @@ __global__ void __launch_bounds__(YNT) ydot_kernel(YArgs a, int curl) {
     double acc = 0.0;
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
-        CVec u, v;
+        CVec<K> u, v;
         if (curl) {
             u = (c == 0) ? W0 : (c == 1) ? W1 : W2;
             v = u;
@@ -878,8 +877,8 @@ __global__ void __launch_bounds__(YNT) ydot_kernel(YArgs a, int curl) {
         stash(vline, g, v);
         wave_sync();
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int m = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            const int m = K * g.t + e;
             double2 s = czero();
             if (m < Ny)
                 for (int n = m & 1; n < Ny; n += 2) {
@@ -913,30 +912,30 @@ __global__ void __launch_bounds__(YNT) ydot_kernel(YArgs a, int curl) {
 // w_k Re(a(k,l,y) conj(b(k,l,y))) with w = 1 (k = 0), 2 (k > 0), Nyquist
 // planes excluded.  partial[block * Ny * 9 + y * 9 + q] (PLANE_STATS_Q
 // quantities: means u v w, then uu vv ww uv uw vw).
-template <int L>
+template <int L, int K>
 __global__ void __launch_bounds__(YNT) ystats_kernel(YArgs a) {
     extern __shared__ double2 smem[];
-    load_tables<L>(smem, a);
-    Tile<L> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L>(a.ld), blockIdx.x);
-    Group<L> g(tile.tab);
+    load_tables<L, K>(smem, a);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), blockIdx.x);
+    Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
     const int Ny = a.gd.Ny, P = a.gd.P;
     constexpr int M = YNT / L, W = 64 / L, Q = PLANE_STATS_Q;
-    double2 vals[3][4];  // values at y = 4t+e (group layout after the inverse transform)
+    double2 vals[3][K];  // values at y = Kt+e (group layout after the inverse transform)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        CVec U = tile.load(a.U[c], g);
+        CVec<K> U = tile.load(a.U[c], g);
         if (c == 0 && mi.mean && mi.valid) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                int n = 4 * g.t + e;
+            for (int e = 0; e < K; ++e) {
+                int n = K * g.t + e;
                 if (n < Ny) U.v[e] = cadd(U.v[e], cmk(a.base[n], 0.0));
             }
         }
         // inverse Chebyshev transform in the wave's lines, result read back in group layout
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             if (n <= P) {
                 double2 x = U.v[e];
                 tile.bufA[tile.mw * tile.ld + n] = (n == 0 || n == P) ? cscale(x, 2.0) : x;
@@ -946,13 +945,13 @@ __global__ void __launch_bounds__(YNT) ystats_kernel(YArgs a) {
         wave_sync();
         double2* res = lds_fft<true>(tile.bufA, tile.bufB, a.dct, tile.tw, W, tile.ld, -1, tile.lane, 64);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int n = 4 * g.t + e;
+        for (int e = 0; e < K; ++e) {
+            int n = K * g.t + e;
             vals[c][e] = (n <= P) ? cscale(res[tile.mw * tile.ld + n], 0.5) : czero();
         }
         wave_sync();
     }
-    // this mode's contributions at each y of the lane: 9 quantities x 4 points
+    // this mode's contributions at each y of the lane: 9 quantities x K points
     const bool use = mi.valid && !mi.nyq;
     const double wk = (mi.ix == 0) ? 1.0 : 2.0;
     const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
@@ -965,8 +964,8 @@ __global__ void __launch_bounds__(YNT) ystats_kernel(YArgs a) {
     for (int m = 0; m < M; ++m) {
         if (mloc == m) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int y = 4 * g.t + e;
+            for (int e = 0; e < K; ++e) {
+                const int y = K * g.t + e;
                 if (y >= Ny) continue;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) red[y * Q + c] += (use && mi.mean) ? vals[c][e].x : 0.0;
@@ -1017,6 +1016,7 @@ hipError_t launch_reduce(const double* part, int n, double* out, hipStream_t st)
 // host launchers
 // ---------------------------------------------------------------------------
 int y_group_lanes(int Ny);
+int y_group_k(int Ny);
 size_t y_lds_bytes(int Ny, int ld);
 
 template <typename K>
@@ -1052,47 +1052,47 @@ static unsigned y_stage_order(YArgs& a) {
     return o.nyq_first + o.nyq_blocks;
 }
 
-template <int L>
+template <int L, int K>
 static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     constexpr int M = YNT / L;
     dim3 grid((unsigned)((a.gd.B + M - 1) / M));
     size_t lds = y_lds_bytes(a.gd.Ny, a.ld);
-    allow_lds(ystage_kernel<L, double2>, lds);
-    allow_lds(yprologue_kernel<L, double2>, lds);
-    allow_lds(ystage_kernel<L, float2>, lds);
-    allow_lds(yprologue_kernel<L, float2>, lds);
-    allow_lds(ysolve_kernel<L>, lds);
-    allow_lds(yneglap_kernel<L>, lds);
-    allow_lds(ypressure_kernel<L>, lds);
-    allow_lds(ydct_kernel<L>, lds);
-    allow_lds(ydivergence_kernel<L>, lds);
-    allow_lds(ydot_kernel<L>, lds);
-    allow_lds(ystats_kernel<L>, lds);
+    allow_lds(ystage_kernel<L, K, double2>, lds);
+    allow_lds(yprologue_kernel<L, K, double2>, lds);
+    allow_lds(ystage_kernel<L, K, float2>, lds);
+    allow_lds(yprologue_kernel<L, K, float2>, lds);
+    allow_lds(ysolve_kernel<L, K>, lds);
+    allow_lds(yneglap_kernel<L, K>, lds);
+    allow_lds(ypressure_kernel<L, K>, lds);
+    allow_lds(ydct_kernel<L, K>, lds);
+    allow_lds(ydivergence_kernel<L, K>, lds);
+    allow_lds(ydot_kernel<L, K>, lds);
+    allow_lds(ystats_kernel<L, K>, lds);
     switch (which) {
         case Y_STAGE: {
             YArgs b = a;
             const unsigned nb = y_stage_order<L>(b);
             const dim3 g2 = nb ? dim3(nb) : grid;
-            if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, float2>), g2, dim3(YNT), lds, st, b);
-            else hipLaunchKernelGGL((ystage_kernel<L, double2>), g2, dim3(YNT), lds, st, b);
+            if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
+            else hipLaunchKernelGGL((ystage_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
             break;
         }
         case Y_PROLOGUE: {
             YArgs b = a;  // regular (non-Nyquist) blocks only, see yprologue_kernel
             const dim3 g2 = y_stage_order<L>(b) ? dim3(b.order.nyq_first) : grid;
-            if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, float2>), g2, dim3(YNT), lds, st, b);
-            else hipLaunchKernelGGL((yprologue_kernel<L, double2>), g2, dim3(YNT), lds, st, b);
+            if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
+            else hipLaunchKernelGGL((yprologue_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
             break;
         }
-        case Y_SOLVE: hipLaunchKernelGGL(ysolve_kernel<L>, grid, dim3(YNT), lds, st, a); break;
-        case Y_NEGLAP: hipLaunchKernelGGL(yneglap_kernel<L>, grid, dim3(YNT), lds, st, a); break;
-        case Y_PRESSURE: hipLaunchKernelGGL(ypressure_kernel<L>, grid, dim3(YNT), lds, st, a); break;
-        case Y_DCT_FWD: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, -1); break;
-        case Y_DCT_INV: hipLaunchKernelGGL(ydct_kernel<L>, grid, dim3(YNT), lds, st, a, +1); break;
-        case Y_DIVERGENCE: hipLaunchKernelGGL(ydivergence_kernel<L>, grid, dim3(YNT), lds, st, a); break;
-        case Y_DOT: hipLaunchKernelGGL(ydot_kernel<L>, grid, dim3(YNT), lds, st, a, 0); break;
-        case Y_DISSIPATION: hipLaunchKernelGGL(ydot_kernel<L>, grid, dim3(YNT), lds, st, a, 1); break;
-        case Y_PLANE_STATS: hipLaunchKernelGGL(ystats_kernel<L>, grid, dim3(YNT), lds, st, a); break;
+        case Y_SOLVE: hipLaunchKernelGGL((ysolve_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
+        case Y_NEGLAP: hipLaunchKernelGGL((yneglap_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
+        case Y_PRESSURE: hipLaunchKernelGGL((ypressure_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
+        case Y_DCT_FWD: hipLaunchKernelGGL((ydct_kernel<L, K>), grid, dim3(YNT), lds, st, a, -1); break;
+        case Y_DCT_INV: hipLaunchKernelGGL((ydct_kernel<L, K>), grid, dim3(YNT), lds, st, a, +1); break;
+        case Y_DIVERGENCE: hipLaunchKernelGGL((ydivergence_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
+        case Y_DOT: hipLaunchKernelGGL((ydot_kernel<L, K>), grid, dim3(YNT), lds, st, a, 0); break;
+        case Y_DISSIPATION: hipLaunchKernelGGL((ydot_kernel<L, K>), grid, dim3(YNT), lds, st, a, 1); break;
+        case Y_PLANE_STATS: hipLaunchKernelGGL((ystats_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1105,6 +1105,8 @@ unsigned y_blocks(const GridDims& gd) {
     return (unsigned)((gd.B + M - 1) / M);
 }
 
+// Group layout of a wall-normal size: K = 4 coefficients per lane and the
+// smallest L with 4 L >= Ny; above 256 points K = 8 with L = 64 (up to 512).
 int y_group_lanes(int Ny) {
     int need = (Ny + 3) / 4;
     if (need <= 4) return 4;
@@ -1112,33 +1114,39 @@ int y_group_lanes(int Ny) {
     if (need <= 16) return 16;
     if (need <= 32) return 32;
     if (need <= 64) return 64;
+    if (Ny <= 512) return 64;
     return 0;
 }
+int y_group_k(int Ny) { return Ny <= 256 ? 4 : (Ny <= 512 ? 8 : 0); }
 
 // DCT twiddle table (2P entries) followed by one region per wavefront
 size_t y_lds_bytes(int Ny, int ld) {
-    int L = y_group_lanes(Ny);
+    const int L = y_group_lanes(Ny), K = y_group_k(Ny);
     int region = 0;
-    switch (L) {
-        case 4: region = y_region_slots<4>(ld); break;
-        case 8: region = y_region_slots<8>(ld); break;
-        case 16: region = y_region_slots<16>(ld); break;
-        case 32: region = y_region_slots<32>(ld); break;
-        case 64: region = y_region_slots<64>(ld); break;
-        default: return 0;
+    if (K == 8) {
+        region = y_region_slots<64, 8>(ld);
+    } else {
+        switch (L) {
+            case 4: region = y_region_slots<4, 4>(ld); break;
+            case 8: region = y_region_slots<8, 4>(ld); break;
+            case 16: region = y_region_slots<16, 4>(ld); break;
+            case 32: region = y_region_slots<32, 4>(ld); break;
+            case 64: region = y_region_slots<64, 4>(ld); break;
+            default: return 0;
+        }
     }
-    return (2 * (size_t)(Ny - 1) + (size_t)CHEB_TAB_ARRAYS * 4 * L / 2 + (YNT / 64) * (size_t)region) * sizeof(double2);
+    return (2 * (size_t)(Ny - 1) + (size_t)CHEB_TAB_ARRAYS * K * L / 2 + (YNT / 64) * (size_t)region) * sizeof(double2);
 }
 
 template <int L>
 static hipError_t launch_y_batch(int which, const YArgs* tab, const unsigned* start, int n, unsigned nblocks, size_t lds,
                                  hipStream_t st) {
     if (which == Y_STAGE) {
-        allow_lds(ystage_batch_kernel<L>, lds);
-        hipLaunchKernelGGL(ystage_batch_kernel<L>, dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
+        allow_lds(ystage_batch_kernel<L, 4>, lds);
+        hipLaunchKernelGGL((ystage_batch_kernel<L, 4>), dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
     } else if (which == Y_PROLOGUE) {
-        allow_lds(yprologue_batch_kernel<L>, lds);
-        hipLaunchKernelGGL(yprologue_batch_kernel<L>, dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
+        allow_lds(yprologue_batch_kernel<L, 4>, lds);
+        hipLaunchKernelGGL((yprologue_batch_kernel<L, 4>), dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1158,12 +1166,13 @@ hipError_t launch_ykernel_batch(int which, int L, const YArgs* tab, const unsign
 }
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
+    if (y_group_k(a.gd.Ny) == 8) return launch_y<64, 8>(which, a, st);
     switch (y_group_lanes(a.gd.Ny)) {
-        case 4: return launch_y<4>(which, a, st);
-        case 8: return launch_y<8>(which, a, st);
-        case 16: return launch_y<16>(which, a, st);
-        case 32: return launch_y<32>(which, a, st);
-        case 64: return launch_y<64>(which, a, st);
+        case 4: return launch_y<4, 4>(which, a, st);
+        case 8: return launch_y<8, 4>(which, a, st);
+        case 16: return launch_y<16, 4>(which, a, st);
+        case 32: return launch_y<32, 4>(which, a, st);
+        case 64: return launch_y<64, 4>(which, a, st);
         default: return hipErrorInvalidValue;
     }
 }

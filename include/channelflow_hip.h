@@ -203,9 +203,10 @@ int cf_set_nonlinear(cf_problem* h, cf_forcing_fn fn, void* user, double* N0, do
 
 /* Tile autotuner (no counterpart in the reference; a performance setting):
  * times the nonlinear term's z passes for each tile width and its x pass for
- * each wavefront count on the problem's workspaces, on `stream`, and keeps
- * the fastest for the FP64 step.  Results of every call are identical for
- * any tiling.  Synchronises the stream. */
+ * each variant and wavefront count on the problem's workspaces, on `stream`,
+ * and keeps the fastest for the FP64 step (and, for a mixed-precision
+ * problem, the fastest FP32 x pass of its step).  Results of every call are
+ * identical for any tiling.  Synchronises the stream. */
 int cf_problem_tune(cf_problem* h, void* stream);
 /* current tiles: out[0] z tile width, out[1] z in-place points per thread
  * (0: two buffers), out[2] x wavefronts per block */

@@ -56,7 +56,7 @@ def test_create_rejects_bad_configs_without_device():
     assert make(vals=(float("nan"), 0.0))[0] == _lib.CF_ERR_ARGUMENT  # problem.jl:86-87
     assert make(form=3)[0] == _lib.CF_ERR_ARGUMENT          # forms: rotating, convective, divergence
     assert b"form" in L.cf_last_error()
-    assert make(Ny=257)[0] == _lib.CF_ERR_UNSUPPORTED      # a mode's column must fit one wavefront
+    assert make(Ny=513)[0] == _lib.CF_ERR_UNSUPPORTED      # a mode's column must fit one wavefront (8 x 64)
     assert make(precision=2)[0] == _lib.CF_ERR_ARGUMENT
     assert make(precision=1, form=2)[0] == _lib.CF_ERR_UNSUPPORTED  # mixed: no divergence form
     assert make(precision=1, Nx=7)[0] == _lib.CF_ERR_UNSUPPORTED    # mixed: odd Nxp (11) rejected

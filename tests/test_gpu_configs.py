@@ -61,7 +61,7 @@ def test_big_nonlinear_and_stokes(big):
     U, P = U.cpu().numpy(), P.cpu().numpy()
     Ur, Pr, gr = O.stokes(go, NU3, lam, R, baseflow=basec, **kw)
     env = roundoff_envelope(lambda R_: O.stokes(go, NU3, lam, R_, baseflow=basec, **kw)[:2], [R], [Ur, Pr])
-    eg, eo = exact_errors(go, NU3, lam, R, [U, P], [Ur, Pr], max_modes=12)
+    eg, eo = exact_errors(go, NU3, lam, R, [(U, P), (Ur, Pr)], max_modes=12)
     for i in range(2):
         say(f"256x193x256 stokes[{i}] vs exact: hip {eg[i]:.3e} oracle {eo[i]:.3e}")
         assert eg[i] <= max(1e-12, 4.0 * eo[i])

@@ -94,7 +94,7 @@ def test_plane_statistics_match_physical_grid(gpu, dims):
     assert np.abs(gcv - pc).max() < 1e-12 * np.abs(pc).max()
 
 
-@pytest.mark.parametrize("dims", [(8, 17, 8), (16, 33, 12), (12, 65, 10), (4, 255, 3), (10, 97, 12)])
+@pytest.mark.parametrize("dims", [(8, 17, 8), (16, 33, 12), (12, 65, 10), (4, 255, 3), (10, 97, 12), (4, 257, 3)])
 def test_velocity_gradient_matches_oracle(gpu, dims):
     """cf_velocity_gradient: dV_i/dx_j of V = U + Ub e_x at the padded points
     against the oracle's ddx1/ddx2/ddx3 followed by its inverse transform;

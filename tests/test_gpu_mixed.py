@@ -12,7 +12,7 @@ from test_gpu_parity import say
 pytestmark = pytest.mark.gpu
 
 # even padded Nx (the mixed path's requirement); group sizes L = 8, 16, 32, 64
-MIXED_GRIDS = [(8, 17, 8), (16, 33, 12), (32, 33, 32), (12, 65, 10), (4, 255, 3), (16, 97, 16)]
+MIXED_GRIDS = [(8, 17, 8), (16, 33, 12), (32, 33, 32), (12, 65, 10), (4, 255, 3), (16, 97, 16), (4, 257, 4)]
 
 
 def _pair(cf, dims, form=None, **kw):
