@@ -446,19 +446,23 @@ X_VARIANTS = {"six": 0, "split": 1, "async": 2}  # x pass variants of the rotati
 
 def get_tiles(problem):
     """(z tile width, z points per thread, x pass variant, x wavefronts per
-    block) of the problem's FP64 step (cf_problem_get_tiles)."""
-    out = (C.c_int * 4)()
+    block, wall-normal coefficients per lane) of the problem's step
+    (cf_problem_get_tiles)."""
+    out = (C.c_int * 5)()
     _lib.check(_lib.load().cf_problem_get_tiles(problem.scheme.h, out))
     return tuple(out)
 
 
-def set_tiles(problem, z_width, x_variant, x_waves):
-    """Set the FP64 step's z tile width, x pass variant (0/"six", 1/"split",
-    2/"async") and x wavefronts per block (cf_problem_set_tiles); an
-    unavailable choice raises and leaves the tiling as it was.  Results of the
-    step are identical for any valid tiling."""
+def set_tiles(problem, z_width, x_variant, x_waves, y_coefs=None):
+    """Set the step's z tile width, x pass variant (0/"six", 1/"split",
+    2/"async"), x wavefronts per block and (optionally) the wall-normal
+    kernels' coefficients per lane, 4 or 8 (cf_problem_set_tiles); an
+    unavailable choice raises and leaves the tiling as it was.  The x and z
+    choices give identical results; the wall-normal layout agrees to
+    roundoff."""
     v = X_VARIANTS.get(x_variant, x_variant)
-    arr = (C.c_int * 4)(int(z_width), 0, int(v), int(x_waves))
+    yk = get_tiles(problem)[4] if y_coefs is None else int(y_coefs)
+    arr = (C.c_int * 5)(int(z_width), 0, int(v), int(x_waves), yk)
     _lib.check(_lib.load().cf_problem_set_tiles(problem.scheme.h, arr))
     return get_tiles(problem)
 

@@ -118,6 +118,9 @@ struct cf_problem {
     // block of the six-line and the split variant in xmaxw32)
     int xsplit32 = 0, xppts32 = 0, xwavess32 = 0;
     int xmaxw32[2] = {0, 0};
+    // group layout (lanes, coefficients per lane) of the step's wall-normal
+    // stage and prologue kernels
+    int yl = 0, yk = 0;
     // one-step graphs of the plain step (no host callbacks), one per field
     // pointer set and constraint, least recently used dropped past GRAPH_CACHE;
     // cleared when the tiling changes
@@ -379,6 +382,8 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         h->LPB = (int)std::max<size_t>(1, std::min<size_t>(8, LDS_MAX / per_line));
     }
     if (y_lds_bytes(gd.Ny, h->yld) > 160 * 1024) return bail(fail(CF_ERR_UNSUPPORTED, "wall-normal tile exceeds LDS"));
+    h->yl = y_group_lanes(gd.Ny);
+    h->yk = y_group_k(gd.Ny);
     if (gd.Nxp % 2 == 0) {
         XArgs xa;
         std::memset(&xa, 0, sizeof(xa));
@@ -628,6 +633,8 @@ static int enqueue_nonlinear_coef(cf_problem* h, double2* const Ncoef[3], hipStr
 
 static YArgs prologue_args(cf_problem* h, double2* const U[3], bool mixed) {
     YArgs a = base_yargs(h);
+    a.yl = h->yl;
+    a.yk = h->yk;
     a.mixed = mixed ? 1 : 0;
     for (int i = 0; i < 3; ++i) a.U[i] = U[i];
     for (int i = 0; i < 6; ++i) a.out6[i] = six_field(h, i, mixed);
@@ -690,6 +697,8 @@ static YArgs stage_args(cf_problem* h, double2* const U[3], double2* P, int j, c
                         const DevBank* bank, bool mixed, double2* const Ncoef[3], bool forced = false) {
     const GridDims& gd = h->gd;
     YArgs a = base_yargs(h);
+    a.yl = h->yl;
+    a.yk = h->yk;
     a.mixed = mixed ? 1 : 0;
     a.nl_sign = (h->form == CF_FORM_CONVECTIVE) ? -1.0 : 1.0;
     a.stage = j;
@@ -1564,6 +1573,42 @@ extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
         h->xwavess = bestW[1];
         h->xwavesa = bestW[2];
     }
+    // group layout of the step's stage and prologue kernels: the default or,
+    // where built (K = 8 on half the lanes), twice the coefficients per lane;
+    // timed on stage-2 launches over the workspaces (U in R, P in the z
+    // buffer; no gradient output)
+    if (!rc) {
+        const int L0 = y_group_lanes(h->gd.Ny), K0 = y_group_k(h->gd.Ny);
+        int cand[2][2] = {{L0, K0}, {L0 / 2, 8}};
+        const int ncand = (K0 == 4 && L0 >= 32) ? 2 : 1;
+        const int keepL = h->yl, keepK = h->yk;
+        int bestL = keepL, bestK = keepK;
+        best = 1e30f;
+        const bool mixed = h->mixed != 0;
+        for (int ci = 0; ci < ncand && !rc; ++ci) {
+            const size_t lds = y_lds_bytes_layout(h->gd.Ny, h->yld, cand[ci][0], cand[ci][1]);
+            if (lds == 0 || lds > 160 * 1024) continue;
+            h->yl = cand[ci][0];
+            h->yk = cand[ci][1];
+            double2* U[3] = {fld(h->R, h->gd.S, 0), fld(h->R, h->gd.S, 1), fld(h->R, h->gd.S, 2)};
+            double2* Ncoef[3] = {fld(h->Q, h->gd.S, 0), fld(h->Q, h->gd.S, 1), fld(h->Q, h->gd.S, 2)};
+            const Constraint c0{0, 0.0, 0.0};
+            YArgs a = stage_args(h, U, h->zbuf, 2, c0, h->dt, &h->stage[2], mixed, Ncoef);
+            a.grad_out = nullptr;
+            float ms;
+            rc = timed([&] {
+                HIPTRY(launch_ykernel(Y_STAGE, a, st));
+                return 0;
+            }, &ms);
+            if (!rc && ms < best) {
+                best = ms;
+                bestL = h->yl;
+                bestK = h->yk;
+            }
+        }
+        h->yl = rc ? keepL : bestL;
+        h->yk = rc ? keepK : bestK;
+    }
     // mixed step: its FP32 x pass (six-line or split, wavefronts per block)
     if (!rc && h->mixed && h->gd.Nxp % 2 == 0) {
         const int keepSplit = h->xsplit32;
@@ -1605,16 +1650,17 @@ extern "C" int cf_problem_tiles(const cf_problem* h, int out[3]) {
     return CF_OK;
 }
 
-extern "C" int cf_problem_get_tiles(const cf_problem* h, int tiles[4]) {
+extern "C" int cf_problem_get_tiles(const cf_problem* h, int tiles[5]) {
     if (!h || !tiles) return fail(CF_ERR_ARGUMENT, "null argument");
     tiles[0] = h->KT;
     tiles[1] = h->zppt;
     tiles[2] = h->xsplit;
     tiles[3] = h->gd.Nxp % 2 ? 0 : h->xsplit == 2 ? h->xwavesa : h->xsplit ? h->xwavess : h->xwaves;
+    tiles[4] = h->yk;
     return CF_OK;
 }
 
-extern "C" int cf_problem_set_tiles(cf_problem* h, const int tiles[4]) {
+extern "C" int cf_problem_set_tiles(cf_problem* h, const int tiles[5]) {
     if (!h || !tiles) return fail(CF_ERR_ARGUMENT, "null argument");
     // z tile: an in-place width (one LDS buffer) or, for plans that need two
     // buffers, a width whose two buffers fit 64 KB
@@ -1636,6 +1682,19 @@ extern "C" int cf_problem_set_tiles(cf_problem* h, const int tiles[4]) {
     } else if (var != 0) {
         return fail(CF_ERR_ARGUMENT, "odd padded Nx has one x pass");
     }
+    // wall-normal layout: K coefficients per lane on the fewest lanes covering Ny
+    const int yk = tiles[4];
+    int yl = 0;
+    if (yk == y_group_k(h->gd.Ny)) {
+        yl = y_group_lanes(h->gd.Ny);
+    } else if (yk == 8) {
+        yl = 16;
+        while (yl * 8 < h->gd.Ny) yl *= 2;
+        const size_t lds = y_lds_bytes_layout(h->gd.Ny, h->yld, yl, 8);
+        if (lds == 0 || lds > 160 * 1024) return fail(CF_ERR_ARGUMENT, "wall-normal layout unavailable for this Ny");
+    } else {
+        return fail(CF_ERR_ARGUMENT, "wall-normal coefficients per lane must be 4 or 8");
+    }
     DeviceGuard dg(h->device);
     if (int rc = invalidate_graphs(h)) return rc;
     h->KT = kt;
@@ -1644,6 +1703,8 @@ extern "C" int cf_problem_set_tiles(cf_problem* h, const int tiles[4]) {
         h->xsplit = var;
         (var == 2 ? h->xwavesa : var == 1 ? h->xwavess : h->xwaves) = w;
     }
+    h->yl = yl;
+    h->yk = yk;
     return CF_OK;
 }
 

@@ -70,6 +70,9 @@ struct YArgs {
     int nfields;
     double* partial;   // Y_DOT / Y_DISSIPATION: one partial sum per block (device)
     YOrder order;      // Y_STAGE: set by the launcher (zero: plain order)
+    // Y_STAGE / Y_PROLOGUE: group layout, L lanes of K coefficients per mode
+    // (yk = 0: the default layout of Ny; other kernels always use it)
+    int yl, yk;
 };
 
 struct ZArgs {
@@ -157,6 +160,9 @@ hipError_t launch_gaussian(double* out3, long long n, double eps, unsigned long 
 constexpr int PLANE_STATS_Q = 9;  // per y: means of u, v, w; covariances uu vv ww uv uw vw
 int y_group_lanes(int Ny);  // lanes L per mode of the wall-normal kernels (0: Ny unsupported)
 int y_group_k(int Ny);      // coefficients K per lane (4, or 8 above 256 points)
+// LDS bytes of a y launch in layout (L, K); 0 when (L, K) is not built or does
+// not cover Ny coefficients
+size_t y_lds_bytes_layout(int Ny, int ld, int L, int K);
 size_t y_lds_bytes(int Ny, int ld);
 hipError_t launch_zpass(int dir, const ZArgs& a, hipStream_t st);
 size_t z_lds_bytes(const ZArgs& a);

@@ -1053,14 +1053,43 @@ static unsigned y_stage_order(YArgs& a) {
 }
 
 template <int L, int K>
+static size_t y_lds_lk(int Ny, int ld) {
+    return (2 * (size_t)(Ny - 1) + (size_t)CHEB_TAB_ARRAYS * K * L / 2 + (YNT / 64) * (size_t)y_region_slots<L, K>(ld)) *
+           sizeof(double2);
+}
+
+template <int L, int K>
+static hipError_t launch_y_step(int which, const YArgs& a, hipStream_t st) {
+    constexpr int M = YNT / L;
+    const dim3 grid((unsigned)((a.gd.B + M - 1) / M));
+    const size_t lds = y_lds_lk<L, K>(a.gd.Ny, a.ld);
+    YArgs b = a;
+    if (which == Y_STAGE) {
+        allow_lds(ystage_kernel<L, K, double2>, lds);
+        allow_lds(ystage_kernel<L, K, float2>, lds);
+        const unsigned nb = y_stage_order<L>(b);
+        const dim3 g2 = nb ? dim3(nb) : grid;
+        if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
+        else hipLaunchKernelGGL((ystage_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
+    } else if (which == Y_PROLOGUE) {
+        allow_lds(yprologue_kernel<L, K, double2>, lds);
+        allow_lds(yprologue_kernel<L, K, float2>, lds);
+        // regular (non-Nyquist) blocks only, see yprologue_kernel
+        const dim3 g2 = y_stage_order<L>(b) ? dim3(b.order.nyq_first) : grid;
+        if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
+        else hipLaunchKernelGGL((yprologue_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int L, int K>
 static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
+    if (which == Y_STAGE || which == Y_PROLOGUE) return launch_y_step<L, K>(which, a, st);
     constexpr int M = YNT / L;
     dim3 grid((unsigned)((a.gd.B + M - 1) / M));
-    size_t lds = y_lds_bytes(a.gd.Ny, a.ld);
-    allow_lds(ystage_kernel<L, K, double2>, lds);
-    allow_lds(yprologue_kernel<L, K, double2>, lds);
-    allow_lds(ystage_kernel<L, K, float2>, lds);
-    allow_lds(yprologue_kernel<L, K, float2>, lds);
+    size_t lds = y_lds_lk<L, K>(a.gd.Ny, a.ld);
     allow_lds(ysolve_kernel<L, K>, lds);
     allow_lds(yneglap_kernel<L, K>, lds);
     allow_lds(ypressure_kernel<L, K>, lds);
@@ -1069,21 +1098,6 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     allow_lds(ydot_kernel<L, K>, lds);
     allow_lds(ystats_kernel<L, K>, lds);
     switch (which) {
-        case Y_STAGE: {
-            YArgs b = a;
-            const unsigned nb = y_stage_order<L>(b);
-            const dim3 g2 = nb ? dim3(nb) : grid;
-            if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
-            else hipLaunchKernelGGL((ystage_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
-            break;
-        }
-        case Y_PROLOGUE: {
-            YArgs b = a;  // regular (non-Nyquist) blocks only, see yprologue_kernel
-            const dim3 g2 = y_stage_order<L>(b) ? dim3(b.order.nyq_first) : grid;
-            if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
-            else hipLaunchKernelGGL((yprologue_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
-            break;
-        }
         case Y_SOLVE: hipLaunchKernelGGL((ysolve_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
         case Y_NEGLAP: hipLaunchKernelGGL((yneglap_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
         case Y_PRESSURE: hipLaunchKernelGGL((ypressure_kernel<L, K>), grid, dim3(YNT), lds, st, a); break;
@@ -1121,21 +1135,33 @@ int y_group_k(int Ny) { return Ny <= 256 ? 4 : (Ny <= 512 ? 8 : 0); }
 
 // DCT twiddle table (2P entries) followed by one region per wavefront
 size_t y_lds_bytes(int Ny, int ld) {
-    const int L = y_group_lanes(Ny), K = y_group_k(Ny);
-    int region = 0;
-    if (K == 8) {
-        region = y_region_slots<64, 8>(ld);
-    } else {
+    return y_lds_bytes_layout(Ny, ld, y_group_lanes(Ny), y_group_k(Ny));
+}
+
+// Layouts built: K = 4 with L = 4..64 (the defaults), and K = 8 with L = 16,
+// 32, 64 (the only layout above 256 points; below it an alternative the
+// tuner may pick for the step's stage and prologue kernels).
+size_t y_lds_bytes_layout(int Ny, int ld, int L, int K) {
+    if (K * L < Ny) return 0;
+    if (K == 4) {
         switch (L) {
-            case 4: region = y_region_slots<4, 4>(ld); break;
-            case 8: region = y_region_slots<8, 4>(ld); break;
-            case 16: region = y_region_slots<16, 4>(ld); break;
-            case 32: region = y_region_slots<32, 4>(ld); break;
-            case 64: region = y_region_slots<64, 4>(ld); break;
+            case 4: return y_lds_lk<4, 4>(Ny, ld);
+            case 8: return y_lds_lk<8, 4>(Ny, ld);
+            case 16: return y_lds_lk<16, 4>(Ny, ld);
+            case 32: return y_lds_lk<32, 4>(Ny, ld);
+            case 64: return y_lds_lk<64, 4>(Ny, ld);
             default: return 0;
         }
     }
-    return (2 * (size_t)(Ny - 1) + (size_t)CHEB_TAB_ARRAYS * K * L / 2 + (YNT / 64) * (size_t)region) * sizeof(double2);
+    if (K == 8) {
+        switch (L) {
+            case 16: return y_lds_lk<16, 8>(Ny, ld);
+            case 32: return y_lds_lk<32, 8>(Ny, ld);
+            case 64: return y_lds_lk<64, 8>(Ny, ld);
+            default: return 0;
+        }
+    }
+    return 0;
 }
 
 template <int L>
@@ -1166,6 +1192,14 @@ hipError_t launch_ykernel_batch(int which, int L, const YArgs* tab, const unsign
 }
 
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
+    if ((which == Y_STAGE || which == Y_PROLOGUE) && a.yk != 0 &&
+        !(a.yk == y_group_k(a.gd.Ny) && a.yl == y_group_lanes(a.gd.Ny))) {
+        if (!y_lds_bytes_layout(a.gd.Ny, a.ld, a.yl, a.yk)) return hipErrorInvalidValue;
+        if (a.yk == 8 && a.yl == 16) return launch_y_step<16, 8>(which, a, st);
+        if (a.yk == 8 && a.yl == 32) return launch_y_step<32, 8>(which, a, st);
+        if (a.yk == 8 && a.yl == 64) return launch_y_step<64, 8>(which, a, st);
+        return hipErrorInvalidValue;
+    }
     if (y_group_k(a.gd.Ny) == 8) return launch_y<64, 8>(which, a, st);
     switch (y_group_lanes(a.gd.Ny)) {
         case 4: return launch_y<4, 4>(which, a, st);

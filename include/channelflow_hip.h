@@ -219,10 +219,13 @@ int cf_problem_tiles(const cf_problem* h, int out[3]);
  * of the rotating form with even padded Nx (0 six LDS lines, 1 split: three
  * lines, velocity held in registers, 2 six lines with direct-to-LDS loads;
  * other forms: 0), tiles[3] x wavefronts per block (1..the variant's LDS
- * limit).  Results are identical for every valid choice; an unavailable one
- * returns CF_ERR_ARGUMENT and changes nothing. */
-int cf_problem_get_tiles(const cf_problem* h, int tiles[4]);
-int cf_problem_set_tiles(cf_problem* h, const int tiles[4]);
+ * limit), tiles[4] Chebyshev coefficients per lane of the wall-normal stage
+ * and prologue kernels (4 or 8 where Ny <= 256 and Ny > 64; 8 above 256).
+ * The x and z choices give identical results; the wall-normal layout changes
+ * the summation order of its scans (results agree to roundoff).  An
+ * unavailable choice returns CF_ERR_ARGUMENT and changes nothing. */
+int cf_problem_get_tiles(const cf_problem* h, int tiles[5]);
+int cf_problem_set_tiles(cf_problem* h, const int tiles[5]);
 
 /* Uniform pressure gradient (dPdx, dPdz) returned by the last Stokes solve of
  * the last step (stokes.jl:101-107).  Synchronises the stream; HOST output. */

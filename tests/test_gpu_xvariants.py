@@ -92,3 +92,39 @@ def test_set_tiles_rejects(gpu):
     conv = cf.ChannelFlowProblem(g, lambda y: y, 1 / 400, 0.01, form=cf.ConvectiveForm)
     with pytest.raises(ValueError):
         cf.set_tiles(conv, cf.get_tiles(conv)[0], 1, 1)
+
+
+@pytest.mark.parametrize("shape", [(16, 97, 12), (8, 193, 6), (24, 65, 16)])
+def test_wall_normal_layouts(gpu, shape):
+    """The step's stage and prologue kernels with 4 and with 8 Chebyshev
+    coefficients per lane (half the lanes per mode): three steps of each
+    agree with the oracle within its roundoff envelope (the scans sum in a
+    different order, so the two agree to roundoff, not bit for bit)."""
+    import torch
+    from oracle import oracle as O
+    from test_gpu_parity import _split, assert_parity, roundoff_envelope
+    cf = gpu
+    box = (2 * math.pi, math.pi)
+    prof = lambda y: 1.5 * (1 - y ** 2)  # noqa: E731
+    g = cf.Grid(*shape, *box)
+    prob = cf.ChannelFlowProblem(g, prof, 1 / 2800, 0.005, bulkvelocity=(1.0, 0.0))
+    st = cf.perturbation_state(prob, 0.05, seed=11)
+    U0, P0 = st.velocity.cpu().numpy(), st.stagepressure.cpu().numpy()
+    go = O.Grid(*shape, *box)
+    ref = O.Problem(go, prof, 1 / 2800, 0.005, bulkvelocity=(1.0, 0.0))
+    U1, P1 = ref.step(U0, P0, 3)
+    env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, 3)), [U0, P0], _split((U1, P1)), trials=8)
+    keep = cf.get_tiles(prob)
+    try:
+        for yk in (4, 8):
+            got = cf.set_tiles(prob, keep[0], keep[2], keep[3], y_coefs=yk)
+            assert got[4] == yk
+            U = torch.as_tensor(U0, device="cuda")
+            P = torch.as_tensor(P0, device="cuda")
+            cf.step_(prob.scheme, prob.nlterm, U, P, 0.0, nsteps=3, **prob.constraint)
+            say(f"{shape}: {yk} coefficients per lane")
+            assert_parity(f"{shape} K={yk}", _split((U.cpu().numpy(), P.cpu().numpy())), _split((U1, P1)), env)
+    finally:
+        cf.set_tiles(prob, keep[0], keep[2], keep[3], y_coefs=keep[4])
+    with pytest.raises(ValueError):
+        cf.set_tiles(prob, keep[0], keep[2], keep[3], y_coefs=6)
