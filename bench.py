@@ -43,18 +43,18 @@ LAUNCHES_PER_STEP = [1, 3, 3, 3, 3]
 def algorithmic_bytes(Nx, Ny, Nz):
     """Per-launch algorithmic HBM bytes (one read of every input, one write of
     every output, complex128) of each kernel class, and the canonical
-    per-step figure 16(90 S + 54 Sz) used for the step-level roofline."""
+    per-step figure 16(87 S + 54 Sz) used for the step-level roofline."""
     Nxh, Nzp = Nx // 2 + 1, (3 * Nz + 1) // 2
     S = Nxh * Nz * Ny
     Sz = Nxh * Nzp * Ny
     per_launch = {
-        "y_prologue": 16 * 9 * S,             # U (3) -> V, Omega (6)
-        "z_inverse": 16 * (6 * S + 6 * Sz),   # 6 fields (k,l,y) -> (k,zp,y)
+        "y_prologue": 16 * 8 * S,             # U (3) -> V, Omega_x, Omega_z (5)
+        "z_inverse": 16 * (5 * S + 6 * Sz),   # 5 fields (k,l,y) -> 6 (k,zp,y) (V_x also as dV_x/dz)
         "x_pass": 16 * 9 * Sz,                # 6 in, 3 out
         "z_forward": 16 * (3 * Sz + 3 * S),   # 3 fields (k,zp,y) -> (k,l,y)
-        "y_stage": 16 * 19 * S,               # stage mean of (7+13, 10+13, 10+4) S
+        "y_stage": 16 * 55 * S / 3,           # stage mean of (7+12, 10+12, 10+4) S
     }
-    step_canonical = 16 * (90 * S + 54 * Sz)
+    step_canonical = 16 * (87 * S + 54 * Sz)
     return per_launch, step_canonical, S, Sz
 
 
@@ -64,11 +64,11 @@ MIXED = dict(N=(256, 193, 256), L=(4 * math.pi, 4 * math.pi / 3), nu=1 / 2800, d
 
 
 def mixed_bytes(Nx, Ny, Nz):
-    """Algorithmic bytes of the mixed step: FP64 state traffic 16*36 S, FP32
-    y-physical and z-padded traffic 8*(54 S + 54 Sz)."""
+    """Algorithmic bytes of the mixed step: FP64 state traffic 16*39 S, FP32
+    y-physical and z-padded traffic 8*(48 S + 54 Sz); and of the FP64 step."""
     Nxh, Nzp = Nx // 2 + 1, (3 * Nz + 1) // 2
     S, Sz = Nxh * Nz * Ny, Nxh * Nzp * Ny
-    return 16 * 36 * S + 8 * (54 * S + 54 * Sz), 16 * (90 * S + 54 * Sz)
+    return 16 * 39 * S + 8 * (48 * S + 54 * Sz), 16 * (87 * S + 54 * Sz)
 
 
 def wall_shear(g, U, nu, base):
