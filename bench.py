@@ -230,9 +230,13 @@ def bench_closure(cf, snapshots, every, eval_steps, epochs, sample_every=10):
         if (k + 1) % 250 == 0:
             torch.cuda.synchronize()
             progress(f"closure: {k + 1}/{snapshots} snapshots")
-    ntest = max(1, snapshots // 10)
-    Xtr, Ttr = torch.cat(Xs[:-ntest]), torch.cat(Ts[:-ntest])
-    Xte, Tte = torch.cat(Xs[-ntest:]), torch.cat(Ts[-ntest:])
+    # interleaved split: every tenth snapshot is held out, so the held-out
+    # set spans the whole sampled window (a final block would see only the
+    # run's last, possibly different, state)
+    test_ids = [k for k in range(snapshots) if k % 10 == 9] or [snapshots - 1]
+    train_ids = [k for k in range(snapshots) if k not in set(test_ids)] or [0]
+    Xtr, Ttr = torch.cat([Xs[k] for k in train_ids]), torch.cat([Ts[k] for k in train_ids])
+    Xte, Tte = torch.cat([Xs[k] for k in test_ids]), torch.cat([Ts[k] for k in test_ids])
     del Xs, Ts
     torch.cuda.synchronize()
     data_s = time.perf_counter() - t0
