@@ -819,6 +819,9 @@ static int graph_steps(cf_problem* h, double2* const U[3], double2* P, int nstep
 // inside the step, else launched one by one
 static int run_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
                      double t0, bool graph) {
+    // a caller capturing its own graph on `st` records the kernels themselves
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (graph && st && hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) graph = false;
     if (graph && nsteps > 0 && !h->force_fn && !h->nl_fn) return graph_steps(h, U, P, nsteps, c, st);
     return enqueue_steps(h, U, P, nsteps, c, st, nullptr, 0.0, nullptr, t0);
 }

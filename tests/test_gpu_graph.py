@@ -110,3 +110,33 @@ def test_graph_mixed(gpu):
     ref = eager(cf, prob, st, 3, **prob.constraint)
     U, P = graphed(cf, prob, st, 3, **prob.constraint)
     assert same((U.cpu().numpy(), P.cpu().numpy()), ref)
+
+
+def test_step_under_caller_capture(setup):
+    """A caller capturing its own graph around step_ records the step's
+    kernels (the library replays no graph of its own inside a capture).
+    Every replay, with eager copies and synchronisations between them,
+    equals the eager steps."""
+    import torch
+    cf, prob, st = setup
+    ref = eager(cf, prob, st, 2, **prob.constraint)
+    U, P = st.velocity.clone(), st.stagepressure.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):  # first use of these buffers outside the capture
+        cf.step_(prob.scheme, prob.nlterm, U, P, 0.0, nsteps=1, **prob.constraint)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, capture_error_mode="relaxed"):
+        cf.step_(prob.scheme, prob.nlterm, U, P, 0.0, nsteps=2, **prob.constraint)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        U.copy_(st.velocity)
+        P.copy_(st.stagepressure)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        ok = same((U.cpu().numpy(), P.cpu().numpy()), ref)
+        say(f"caller graph replay {rep}: identical {ok}")
+        assert ok
