@@ -159,8 +159,9 @@ struct Tile {
     }
     // inverse: coefficient registers -> values at y_j (endpoints doubled,
     // REDFT00, x1/2: InverseFFT!'s chebyshev + copy_to_padded! scale).
-    // Computed in FP64; SC = float2 rounds the stored values.
-    template <typename SC>
+    // Computed in FP64; SC = float2 rounds the stored values.  INPLACE: one
+    // LDS line per mode, transformed in place (8 points per lane).
+    template <typename SC, bool INPLACE = false>
     __device__ __forceinline__ void dct_inverse(SC* __restrict__ dst, const FFTPlan& plan, const Group<L, K>& g,
                                                 const CVec<K>& a) {
         const int P = gd.P;
@@ -174,7 +175,9 @@ struct Tile {
             }
         }
         wave_sync();
-        double2* res = lds_fft<true, true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
+        double2* res = bufA;
+        if constexpr (INPLACE) lds_fft_inplace<8, true>(bufA, plan, tw, W, ld, -1, lane, 64);
+        else res = lds_fft<true, true>(bufA, bufB, plan, tw, W, ld, -1, lane, 64);
         for (int idx = lane; idx < (P + 1) * W; idx += 64) {
             int n = idx / W, m = idx - n * W;
             long long sm = sw0 + m;
@@ -252,7 +255,7 @@ __device__ __forceinline__ void load_fac(const double* __restrict__ f, long long
 //   divergence V                   (nonlinear.jl:146-167)
 // get(c) returns U component c: from registers in the stage kernel, from HBM
 // in the prologue (loaded where first needed, which shortens live ranges).
-template <int L, int K, typename SC, typename GetU>
+template <int L, int K, typename SC, bool INPLACE = false, typename GetU>
 __device__ __forceinline__ void inverse_six(Tile<L, K>& tile, const YArgs& a, const Group<L, K>& g, const ModeInfo& mi,
                                             GetU get) {
     SC* out6[6];
@@ -267,35 +270,35 @@ __device__ __forceinline__ void inverse_six(Tile<L, K>& tile, const YArgs& a, co
         }
     }
     if (a.form != FORM_ROTATING) {
-        tile.dct_inverse(out6[0], a.dct, g, V0);
-        if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[3], a.dct, g, cheb_deriv(g, V0));
+        tile.template dct_inverse<SC, INPLACE>(out6[0], a.dct, g, V0);
+        if (a.form == FORM_CONVECTIVE) tile.template dct_inverse<SC, INPLACE>(out6[3], a.dct, g, cheb_deriv(g, V0));
         const CVec<K> V1 = get(1);
-        tile.dct_inverse(out6[1], a.dct, g, V1);
-        if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[4], a.dct, g, cheb_deriv(g, V1));
+        tile.template dct_inverse<SC, INPLACE>(out6[1], a.dct, g, V1);
+        if (a.form == FORM_CONVECTIVE) tile.template dct_inverse<SC, INPLACE>(out6[4], a.dct, g, cheb_deriv(g, V1));
         const CVec<K> V2 = get(2);
-        tile.dct_inverse(out6[2], a.dct, g, V2);
-        if (a.form == FORM_CONVECTIVE) tile.dct_inverse(out6[5], a.dct, g, cheb_deriv(g, V2));
+        tile.template dct_inverse<SC, INPLACE>(out6[2], a.dct, g, V2);
+        if (a.form == FORM_CONVECTIVE) tile.template dct_inverse<SC, INPLACE>(out6[5], a.dct, g, cheb_deriv(g, V2));
         return;
     }
     // order chosen so each velocity component dies as early as possible
     const CVec<K> V1 = get(1);
     CVec<K> W;
-    tile.dct_inverse(out6[0], a.dct, g, V0);
+    tile.template dct_inverse<SC, INPLACE>(out6[0], a.dct, g, V0);
     {
         CVec<K> d1 = cheb_deriv(g, V0);
 #pragma unroll
         for (int e = 0; e < K; ++e) W.v[e] = csub(cmuli(cscale(V1.v[e], mi.kx)), d1.v[e]);  // i k V2 - dy V1
     }
-    tile.dct_inverse(out6[5], a.dct, g, W);
+    tile.template dct_inverse<SC, INPLACE>(out6[5], a.dct, g, W);
     const CVec<K> V2 = get(2);
-    tile.dct_inverse(out6[2], a.dct, g, V2);
+    tile.template dct_inverse<SC, INPLACE>(out6[2], a.dct, g, V2);
     {
         CVec<K> d3 = cheb_deriv(g, V2);
 #pragma unroll
         for (int e = 0; e < K; ++e) W.v[e] = csub(d3.v[e], cmuli(cscale(V1.v[e], mi.kz)));  // dy V3 - i l V2
     }
-    tile.dct_inverse(out6[3], a.dct, g, W);
-    tile.dct_inverse(out6[1], a.dct, g, V1);
+    tile.template dct_inverse<SC, INPLACE>(out6[3], a.dct, g, W);
+    tile.template dct_inverse<SC, INPLACE>(out6[1], a.dct, g, V1);
 }
 
 // ---------------------------------------------------------------------------
@@ -630,7 +633,10 @@ __device__ __forceinline__ void yprologue_body(const YArgs& a, unsigned block) {
 // order (a.order.on) the launch has only the regular blocks: the periodic
 // passes never read the Nyquist modes' y-physical values, so none are
 // written.
-template <int L, int K, typename SC>
+// INPLACE: the DCT lines are transformed in place (one line per mode, half
+// the LDS of the two-buffer form, so more blocks per CU); the host picks it
+// where the plan fits 8 points per lane (fft_inplace_fits)
+template <int L, int K, typename SC, bool INPLACE = false>
 __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     load_tables<L, K>(smem, a);
@@ -638,7 +644,7 @@ __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
     // argument, or a copy of the whole struct, lands in scratch)
     const bool ordered = a.order.on != 0;
     const unsigned block = (ordered && a.order.xcd) ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
-    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, y_region_slots<L, K>(a.ld), block);
+    Tile<L, K> tile(a.gd, smem, a.dct.n, a.ld, INPLACE ? (64 / L) * a.ld : y_region_slots<L, K>(a.ld), block);
     if (ordered) tile.order_modes(a.order.nreg, a.order.nxr, a.order.zskip);
     Group<L, K> g(tile.tab);
     ModeInfo mi = mode_info(a.gd, tile.s);
@@ -646,9 +652,9 @@ __global__ void __launch_bounds__(YNT) yprologue_kernel(YArgs a) {
         CVec<K> U[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) U[c] = tile.load(a.U[c], g);
-        inverse_six<L, K, SC>(tile, a, g, mi, [&](int c) { return U[c]; });
+        inverse_six<L, K, SC, INPLACE>(tile, a, g, mi, [&](int c) { return U[c]; });
     } else {
-        inverse_six<L, K, SC>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
+        inverse_six<L, K, SC, INPLACE>(tile, a, g, mi, [&](int c) { return tile.load(a.U[c], g); });
     }
 }
 
@@ -1072,10 +1078,19 @@ static hipError_t launch_y_step(int which, const YArgs& a, hipStream_t st) {
         if (a.mixed) hipLaunchKernelGGL((ystage_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
         else hipLaunchKernelGGL((ystage_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
     } else if (which == Y_PROLOGUE) {
-        allow_lds(yprologue_kernel<L, K, double2>, lds);
-        allow_lds(yprologue_kernel<L, K, float2>, lds);
         // regular (non-Nyquist) blocks only, see yprologue_kernel
         const dim3 g2 = y_stage_order<L>(b) ? dim3(b.order.nyq_first) : grid;
+        if (fft_inplace_fits(a.dct, 64 / L, 8, 64)) {
+            const size_t ldsi = (2 * (size_t)(a.gd.Ny - 1) + (size_t)CHEB_TAB_ARRAYS * K * L / 2 +
+                                 (YNT / 64) * (size_t)(64 / L) * a.ld) * sizeof(double2);
+            allow_lds(yprologue_kernel<L, K, double2, true>, ldsi);
+            allow_lds(yprologue_kernel<L, K, float2, true>, ldsi);
+            if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, K, float2, true>), g2, dim3(YNT), ldsi, st, b);
+            else hipLaunchKernelGGL((yprologue_kernel<L, K, double2, true>), g2, dim3(YNT), ldsi, st, b);
+            return hipGetLastError();
+        }
+        allow_lds(yprologue_kernel<L, K, double2>, lds);
+        allow_lds(yprologue_kernel<L, K, float2>, lds);
         if (a.mixed) hipLaunchKernelGGL((yprologue_kernel<L, K, float2>), g2, dim3(YNT), lds, st, b);
         else hipLaunchKernelGGL((yprologue_kernel<L, K, double2>), g2, dim3(YNT), lds, st, b);
     } else {
