@@ -427,7 +427,7 @@ def main():
         print(json.dumps({"copy_bytes_each": src.numel() * 8, "copies": args.copy_calibration}))
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (requested --gpus {args.gpus}); "
+        print(f"bench.py: this library measures one GPU only (requested --gpus {args.gpus}); "
               "the multi-GPU pencil decomposition is a design note, not built.", file=sys.stderr)
         return 2
 

@@ -1,4 +1,4 @@
-"""The north-star workloads beyond the headline, each exercised at its own
+"""The benchmark workloads beyond the headline, each exercised at its own
 shape against the oracle (the headline 128x97x128 has tests/test_gpu_headline.py):
 
   256x193x256 plane Poiseuille (Lx = 4 pi, Lz = 4 pi/3, nu = 1/2800,
