@@ -138,6 +138,66 @@ ENSEMBLE = dict(n=64, grids=[(16, 17, 16), (24, 33, 24), (32, 33, 32), (48, 49, 
                      "(seed 1000+i); one graph-captured ensemble step, autotuned tiles")
 
 
+# The reference's published complete-step timings (benchmarks/results/final/
+# cpu-1.csv and gpu.csv, min over 100 samples; docs/src/benchmarks.md:38-44):
+# N -> (Xeon Gold 6336Y 1 thread, A100 80GB PCIe) seconds, grid (N, N+1, N).
+REFERENCE_STEP_S = {8: (0.000261355, 0.002532956), 16: (0.002208649, 0.002612679), 32: (0.016417331, 0.002889341),
+                    64: (0.169632803, 0.005745282), 128: (1.62903821, 0.019938163),
+                    192: (6.581443052, 0.052922028), 256: (15.013734888, 0.115872041)}
+
+
+def bench_reference_grids(cf, sizes, samples=100, warmup=5):
+    """The reference's own step benchmark on this device: plane Couette,
+    rotating form, zero pressure gradient, nu = 1/400, dt = 0.002,
+    Lx = Lz = 2pi, grid (N, N+1, N), the smooth initial field of
+    benchmarks/initial.jl:2-13 (projected, with its pressure). Protocol of
+    benchmarks/step.jl:31-53: restore the initial state, then time one step
+    (host clock around the call and a device synchronisation); five warm-up
+    samples, then the minimum and mean over `samples`."""
+    import numpy as np
+    import torch
+    rows = []
+    for N in sizes:
+        g = cf.Grid(N, N + 1, N, 2 * math.pi, 2 * math.pi)
+        prob = cf.CouetteFlow(g, 1 / 400, 0.002)
+        cf.tune(prob)
+        y, z, x = g.points()
+        one = np.ones((g.Ny, g.Nzp, g.Nxp))
+        u = np.stack([0.1 * (1 - y ** 2) * np.cos(x) * np.sin(z) * one,
+                      0.05 * (1 - y ** 2) ** 2 * np.cos(z) * one,
+                      0.2 * y * (1 - y ** 2) * np.sin(z) * one])
+        U0 = cf.project_(cf.FFT(prob, torch.from_numpy(u).cuda()), prob)
+        P0 = cf.pressure(U0, prob)
+        U, P = U0.clone(), P0.clone()
+        times = []
+        for k in range(warmup + samples):
+            U.copy_(U0)
+            P.copy_(P0)
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            cf.step_(prob.scheme, prob.nlterm, U, P, 0.0, **prob.constraint)
+            torch.cuda.synchronize()
+            if k >= warmup:
+                times.append(time.perf_counter() - a)
+        assert bool(torch.isfinite(torch.view_as_real(U)).all()), "non-finite velocity"
+        tmin, tmean = min(times), sum(times) / len(times)
+        row = {"grid": [N, N + 1, N], "min_ms": round(1e3 * tmin, 4), "mean_ms": round(1e3 * tmean, 4)}
+        if N in REFERENCE_STEP_S:
+            cpu_s, a100_s = REFERENCE_STEP_S[N]
+            row.update({"reference_cpu_1thread_ms": round(1e3 * cpu_s, 3), "reference_a100_ms": round(1e3 * a100_s, 3),
+                        "speedup_vs_a100": round(a100_s / tmin, 2), "speedup_vs_cpu_1thread": round(cpu_s / tmin, 1)})
+        rows.append(row)
+        progress(f"reference grid {N}x{N + 1}x{N}: min {1e3 * tmin:.3f} ms, mean {1e3 * tmean:.3f} ms")
+        del prob, U0, P0, U, P
+        torch.cuda.empty_cache()
+    return {"workload": "the reference's step benchmark: plane Couette, rotating form, zero pressure gradient, "
+                        "nu=1/400, dt=0.002, Lx=Lz=2pi, grid (N, N+1, N), smooth initial field; min and mean of "
+                        f"{samples} single timed steps after {warmup} warm-ups, each from the restored initial state",
+            "reference_hardware": "Xeon Gold 6336Y 1 thread (FFTW) and A100 80GB PCIe (CUDA.jl), "
+                                  "docs/src/benchmarks.md:38-44",
+            "rows": rows}
+
+
 def bench_ensemble(cf, steps, plain_steps=20):
     """Batched ensemble (one graph launch per ensemble step) against the plain
     path (each problem advanced by its own step_ calls): instance-steps/s."""
@@ -414,6 +474,8 @@ def main():
     ap.add_argument("--closure-every", type=int, default=5, help="fine steps between snapshots")
     ap.add_argument("--closure-eval-steps", type=int, default=5000)
     ap.add_argument("--closure-epochs", type=int, default=2)
+    ap.add_argument("--reference-grids", default="8,16,32,64,128,192,256",
+                    help="N list of the reference's (N, N+1, N) Couette step benchmark ('' : skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run N device copies of 256 MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -509,6 +571,12 @@ def main():
             forms["batched"] = bench_ensemble(cf, args.ensemble_steps)
         except Exception as e:
             forms["batched"] = {"error": f"{type(e).__name__}: {e}"}
+    if args.reference_grids:
+        torch.cuda.empty_cache()
+        try:
+            forms["reference_grids"] = bench_reference_grids(cf, [int(n) for n in args.reference_grids.split(",")])
+        except Exception as e:
+            forms["reference_grids"] = {"error": f"{type(e).__name__}: {e}"}
     if args.closure_snapshots > 0:
         torch.cuda.empty_cache()
         try:

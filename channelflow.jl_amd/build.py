@@ -17,8 +17,9 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcfhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels_y.hip", "kernels_xz.hip", "kernels_rng.hip", "kernels_closure.hip", "scratch.hip", "setup.cpp",
-           "api.cpp"]
+# heaviest translation units first (the wall-normal layouts, then the x/z passes)
+SOURCES = [*[f"kernels_y_p{p}.hip" for p in (6, 4, 3, 2, 1, 5)], "kernels_xz.hip", "kernels_closure.hip", "kernels_y.hip",
+           "kernels_rng.hip", "scratch.hip", "setup.cpp", "api.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -66,7 +67,7 @@ def build(verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "6"))))
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", str(os.cpu_count() or 6)))))
     with ThreadPoolExecutor(jobs) as pool:
         results = list(pool.map(lambda s: _compile(s, verbose), srcs))
     stderr = "\n".join(r[1] for r in results)

@@ -14,6 +14,14 @@
 #include "cf_device.h"
 #include "cf_kernels.h"
 
+// This file is compiled as several translation units so that the layouts
+// build in parallel: part 0 (this file on its own) holds the dispatchers and
+// the reduction kernels, parts 1-6 (kernels_y_p<n>.hip) each instantiate the
+// kernels of some (L, K) layouts; the list is at the end of the file.
+#ifndef CF_Y_PART
+#define CF_Y_PART 0
+#endif
+
 namespace cf {
 
 constexpr int YNT = 256;  // threads per block of every y kernel
@@ -987,6 +995,7 @@ __global__ void __launch_bounds__(YNT) ystats_kernel(YArgs a) {
     for (int idx = threadIdx.x; idx < Ny * Q; idx += YNT) a.partial[(size_t)blockIdx.x * Ny * Q + idx] = red[idx];
 }
 
+#if CF_Y_PART == 0
 __global__ void __launch_bounds__(256) reduce_columns_kernel(const double* part, int nblocks, int nq, double* out) {
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= nq) return;
@@ -1017,6 +1026,7 @@ hipError_t launch_reduce(const double* part, int n, double* out, hipStream_t st)
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, st, part, n, out);
     return hipGetLastError();
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // host launchers
@@ -1065,7 +1075,16 @@ static size_t y_lds_lk(int Ny, int ld) {
 }
 
 template <int L, int K>
-static hipError_t launch_y_step(int which, const YArgs& a, hipStream_t st) {
+hipError_t launch_y_step(int which, const YArgs& a, hipStream_t st);
+template <int L, int K>
+hipError_t launch_y(int which, const YArgs& a, hipStream_t st);
+template <int L>
+hipError_t launch_y_batch(int which, const YArgs* tab, const unsigned* start, int n, unsigned nblocks, size_t lds,
+                          hipStream_t st);
+
+#if CF_Y_PART != 0
+template <int L, int K>
+hipError_t launch_y_step(int which, const YArgs& a, hipStream_t st) {
     constexpr int M = YNT / L;
     const dim3 grid((unsigned)((a.gd.B + M - 1) / M));
     const size_t lds = y_lds_lk<L, K>(a.gd.Ny, a.ld);
@@ -1100,7 +1119,7 @@ static hipError_t launch_y_step(int which, const YArgs& a, hipStream_t st) {
 }
 
 template <int L, int K>
-static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
+hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     if (which == Y_STAGE || which == Y_PROLOGUE) return launch_y_step<L, K>(which, a, st);
     constexpr int M = YNT / L;
     dim3 grid((unsigned)((a.gd.B + M - 1) / M));
@@ -1127,6 +1146,24 @@ static hipError_t launch_y(int which, const YArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int L>
+hipError_t launch_y_batch(int which, const YArgs* tab, const unsigned* start, int n, unsigned nblocks, size_t lds,
+                                 hipStream_t st) {
+    if (which == Y_STAGE) {
+        allow_lds(ystage_batch_kernel<L, 4>, lds);
+        hipLaunchKernelGGL((ystage_batch_kernel<L, 4>), dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
+    } else if (which == Y_PROLOGUE) {
+        allow_lds(yprologue_batch_kernel<L, 4>, lds);
+        hipLaunchKernelGGL((yprologue_batch_kernel<L, 4>), dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+#endif  // CF_Y_PART != 0
+
+#if CF_Y_PART == 0
 unsigned y_blocks(const GridDims& gd) {
     const int L = y_group_lanes(gd.Ny);
     if (!L) return 0;
@@ -1179,21 +1216,6 @@ size_t y_lds_bytes_layout(int Ny, int ld, int L, int K) {
     return 0;
 }
 
-template <int L>
-static hipError_t launch_y_batch(int which, const YArgs* tab, const unsigned* start, int n, unsigned nblocks, size_t lds,
-                                 hipStream_t st) {
-    if (which == Y_STAGE) {
-        allow_lds(ystage_batch_kernel<L, 4>, lds);
-        hipLaunchKernelGGL((ystage_batch_kernel<L, 4>), dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
-    } else if (which == Y_PROLOGUE) {
-        allow_lds(yprologue_batch_kernel<L, 4>, lds);
-        hipLaunchKernelGGL((yprologue_batch_kernel<L, 4>), dim3(nblocks), dim3(YNT), lds, st, tab, start, n);
-    } else {
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_ykernel_batch(int which, int L, const YArgs* tab, const unsigned* start, int n, unsigned nblocks,
                                 size_t lds, hipStream_t st) {
     switch (L) {
@@ -1225,5 +1247,33 @@ hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
         default: return hipErrorInvalidValue;
     }
 }
+
+#endif  // CF_Y_PART == 0
+
+// Layouts and the translation unit that instantiates each: the default
+// layouts (K = 4, L = 4..64, and K = 8 on L = 64 above 256 points) with every
+// kernel, the alternative K = 8 layouts on 16 and 32 lanes with the step's
+// two kernels only, and the ensemble's batched kernels on K = 4.
+#if CF_Y_PART == 1
+template hipError_t launch_y<4, 4>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y<8, 4>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_batch<4>(int, const YArgs*, const unsigned*, int, unsigned, size_t, hipStream_t);
+template hipError_t launch_y_batch<8>(int, const YArgs*, const unsigned*, int, unsigned, size_t, hipStream_t);
+#elif CF_Y_PART == 2
+template hipError_t launch_y<16, 4>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_step<16, 8>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_batch<16>(int, const YArgs*, const unsigned*, int, unsigned, size_t, hipStream_t);
+#elif CF_Y_PART == 3
+template hipError_t launch_y<32, 4>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_batch<32>(int, const YArgs*, const unsigned*, int, unsigned, size_t, hipStream_t);
+#elif CF_Y_PART == 4
+template hipError_t launch_y<64, 4>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_batch<64>(int, const YArgs*, const unsigned*, int, unsigned, size_t, hipStream_t);
+#elif CF_Y_PART == 5
+template hipError_t launch_y_step<32, 8>(int, const YArgs&, hipStream_t);
+#elif CF_Y_PART == 6
+template hipError_t launch_y<64, 8>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_step<64, 8>(int, const YArgs&, hipStream_t);
+#endif
 
 }  // namespace cf
