@@ -101,12 +101,18 @@ def _scratch_accesses(kernels, srcs, results, verbose):
     touches it.  The compiler can reserve a frame slot (for instance a
     register-scavenging slot of SGPR spills) that no instruction uses; such a
     kernel runs without scratch traffic and is accepted.  Decided from the
-    device assembly of the sources that define the kernels."""
-    used = []
+    device assembly of the sources that define the kernels (compiled in
+    parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    todo = []
     for src, (_, err, _) in zip(srcs, results):
         mine = [k for k in kernels if ("Function Name: " + k) in err]
-        if not mine:
-            continue
+        if mine:
+            todo.append((src, mine))
+
+    def check(item):
+        src, mine = item
         asm = os.path.join(OBJDIR, os.path.basename(src) + ".s")
         cmd = [HIPCC, *FLAGS, "--cuda-device-only", "-S", "-I", CSRC, src, "-o", asm]
         if verbose:
@@ -114,14 +120,23 @@ def _scratch_accesses(kernels, srcs, results, verbose):
         subprocess.run(cmd, check=True, capture_output=True)
         text = open(asm).read()
         os.remove(asm)
+        used, notes = [], []
         for k in mine:
             start = text.find("\n" + k + ":")
             end = text.find(".Lfunc_end", start)
             body = text[start:end] if start >= 0 else ""
             if start < 0 or re.search(r"\n\s+(scratch_|buffer_|s_swappc)", body):
                 used.append(k)
-            elif verbose:
-                print(f"{k}: reserved private frame, no scratch access in its code", flush=True)
+            else:
+                notes.append(f"{k}: reserved private frame, no scratch access in its code")
+        return used, notes
+
+    used = []
+    with ThreadPoolExecutor(max(1, len(todo))) as pool:
+        for u, notes in pool.map(check, todo):
+            used += u
+            if verbose and notes:
+                print("\n".join(notes), flush=True)
     return used
 
 
