@@ -50,17 +50,21 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
 
 // ---------------------------------------------------------------------------
 // Subgrid-force MLP: 10 -> 64 -> 64 -> 64 -> 3, GELU (erf form) after the
-// hidden layers.  Weights and activations bfloat16 on the matrix cores
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulation), biases and master weights
-// FP32.  theta layout: [W1 (64x16) | b1 | W2 (64x64) | b2 | W3 | b3 | W4 (16x64) | b4 (16)];
+// hidden layers.  Weights and activations bfloat16 on the matrix cores with
+// FP32 accumulation: v_mfma_f32_16x16x32_bf16 (gfx950) for every product
+// whose reduction length is a multiple of 32, v_mfma_f32_16x16x16_bf16 for
+// the 16-wide ones (the padded input layer and the output-layer gradient);
+// biases and master weights FP32.  theta layout: [W1 (64x16) | b1 | W2 (64x64) | b2 | W3 | b3 | W4 (16x64) | b4 (16)];
 // padded input columns 10..15 and output rows 3..15 stay zero.
 //
-// MFMA 16x16x16 lane layout (lane l, lr = l % 16, lg = l / 16):
-//   A (M x K): A[lr][4 lg + j], j < 4      B (K x N): B[4 lg + j][lr]
-//   D (M x N): D[4 lg + r][lr], r < 4
+// MFMA lane layouts (lane l, lr = l % 16, lg = l / 16):
+//   16x16x16: A (M x K): A[lr][4 lg + j], j < 4      B (K x N): B[4 lg + j][lr]
+//   16x16x32: A (M x K): A[lr][8 lg + j], j < 8      B (K x N): B[8 lg + j][lr]
+//   D (M x N): D[4 lg + r][lr], r < 4 (both)
 // ---------------------------------------------------------------------------
 namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MIN = 10, MK = 16, MH = 64, MOUT = 3, MKO = 16;
@@ -80,7 +84,19 @@ __device__ __forceinline__ float gelu_grad(float z) {
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 __device__ __forceinline__ bf16x4 ld4(const unsigned short* p) { return *reinterpret_cast<const bf16x4*>(p); }
+// 8 consecutive bf16 (16-byte aligned: every row stride and offset used is a
+// multiple of 8 elements)
+__device__ __forceinline__ bf16x8 ld8(const unsigned short* p) { return *reinterpret_cast<const bf16x8*>(p); }
+__device__ __forceinline__ bf16x8 ld8s(const unsigned short* p, int stride) {
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (short)p[j * stride];
+    return v;
+}
 __device__ __forceinline__ bf16x4 ld4s(const unsigned short* p, int stride) {
     bf16x4 v;
     v[0] = (short)p[0];
@@ -114,9 +130,17 @@ __device__ __forceinline__ void hidden_layer(const unsigned short* in, const uns
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob) {
             f32x4 c = {0.f, 0.f, 0.f, 0.f};
+            if constexpr (KS % 2 == 0) {
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-                c = mfma16(ld4(in + (rb * 16 + lr) * K + ks * 16 + 4 * lg), ld4(W + (ob * 16 + lr) * K + ks * 16 + 4 * lg), c);
+                for (int ks = 0; ks < KS / 2; ++ks)
+                    c = mfma32(ld8(in + (rb * 16 + lr) * K + ks * 32 + 8 * lg), ld8(W + (ob * 16 + lr) * K + ks * 32 + 8 * lg),
+                               c);
+            } else {
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks)
+                    c = mfma16(ld4(in + (rb * 16 + lr) * K + ks * 16 + 4 * lg),
+                               ld4(W + (ob * 16 + lr) * K + ks * 16 + 4 * lg), c);
+            }
             const float bias = b[ob * 16 + lr];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -160,8 +184,8 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(MLPInferArgs a) {
     for (int rb = 0; rb < 4; ++rb) {
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-            c = mfma16(ld4(bufB + (rb * 16 + lr) * MH + ks * 16 + 4 * lg), ld4(W + OW4 + lr * MH + ks * 16 + 4 * lg), c);
+        for (int ks = 0; ks < 2; ++ks)
+            c = mfma32(ld8(bufB + (rb * 16 + lr) * MH + ks * 32 + 8 * lg), ld8(W + OW4 + lr * MH + ks * 32 + 8 * lg), c);
         if (lr < MOUT) {
             const float bias = a.theta[OB4 + lr], scale = a.norm[2 * MIN + lr];
 #pragma unroll
@@ -239,8 +263,8 @@ __global__ void __launch_bounds__(128) mlp_train_kernel(MLPTrainArgs a) {
     for (int rb = 0; rb < 2; ++rb) {
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-            c = mfma16(ld4(H3 + (rb * 16 + lr) * MH + ks * 16 + 4 * lg), ld4(W + OW4 + lr * MH + ks * 16 + 4 * lg), c);
+        for (int ks = 0; ks < 2; ++ks)
+            c = mfma32(ld8(H3 + (rb * 16 + lr) * MH + ks * 32 + 8 * lg), ld8(W + OW4 + lr * MH + ks * 32 + 8 * lg), c);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = rb * 16 + 4 * lg + r;
@@ -273,9 +297,7 @@ __global__ void __launch_bounds__(128) mlp_train_kernel(MLPTrainArgs a) {
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) {
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks)
-            c = mfma16(ld4s(G + (ks * 16 + 4 * lg) * MKO + lr, MKO), ld4s(H3 + (ks * 16 + 4 * lg) * MH + nb * 16 + lr, MH), c);
+        c = mfma32(ld8s(G + 8 * lg * MKO + lr, MKO), ld8s(H3 + 8 * lg * MH + nb * 16 + lr, MH), c);
 #pragma unroll
         for (int r = 0; r < 4; ++r) part[OW4 + (4 * lg + r) * MH + nb * 16 + lr] = c[r];
     }
@@ -294,9 +316,12 @@ __global__ void __launch_bounds__(128) mlp_train_kernel(MLPTrainArgs a) {
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) {
                 f32x4 c = {0.f, 0.f, 0.f, 0.f};
-                for (int ks = 0; ks < KN / 16; ++ks)
-                    c = mfma16(ld4(G + (rb * 16 + lr) * KN + ks * 16 + 4 * lg),
-                               ld4s(Wl[L] + (ks * 16 + 4 * lg) * MH + nb * 16 + lr, MH), c);
+                if (KN == MKO)
+                    c = mfma16(ld4(G + (rb * 16 + lr) * KN + 4 * lg), ld4s(Wl[L] + 4 * lg * MH + nb * 16 + lr, MH), c);
+                else
+                    for (int ks = 0; ks < KN / 32; ++ks)
+                        c = mfma32(ld8(G + (rb * 16 + lr) * KN + ks * 32 + 8 * lg),
+                                   ld8s(Wl[L] + (ks * 32 + 8 * lg) * MH + nb * 16 + lr, MH), c);
                 dh[rb][nb] = c;
             }
         wave_sync();  // every lane has read G before it is overwritten
@@ -324,10 +349,7 @@ __global__ void __launch_bounds__(128) mlp_train_kernel(MLPTrainArgs a) {
         for (int ob = 0; ob < 4; ++ob)
             for (int nb = 0; nb < KI / 16; ++nb) {
                 f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int ks = 0; ks < 2; ++ks)
-                    c = mfma16(ld4s(G + (ks * 16 + 4 * lg) * MH + ob * 16 + lr, MH),
-                               ld4s(Hin[L] + (ks * 16 + 4 * lg) * KI + nb * 16 + lr, KI), c);
+                c = mfma32(ld8s(G + 8 * lg * MH + ob * 16 + lr, MH), ld8s(Hin[L] + 8 * lg * KI + nb * 16 + lr, KI), c);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) part[OWl[L] + (ob * 16 + 4 * lg + r) * KI + nb * 16 + lr] = c[r];
             }
