@@ -314,9 +314,12 @@ def bench_closure(cf, snapshots, every, eval_steps, epochs, sample_every=10):
     torch.cuda.empty_cache()
     # a posteriori: filtered DNS and coarse runs from the same filtered state
     Ub = cf.box_filter(fine, coarse, st.velocity)
+    # the closure as a device-side forcing (graph-replayed step), and the same
+    # closure called back from the host at every stage (same kernels, same result)
     runs = {"plain": (Ub.clone(), None), "closure": (Ub.clone(), net.forcing(coarse)),
-            "closure_mean": (Ub.clone(), net.forcing(coarse, modes="mean"))}
-    stats = {k: cf.ProfileStatistics(gc.Ny) for k in ("dns", "plain", "closure", "closure_mean")}
+            "closure_mean": (Ub.clone(), net.forcing(coarse, modes="mean")),
+            "closure_host": (Ub.clone(), net.forcing(coarse, device=False))}
+    stats = {k: cf.ProfileStatistics(gc.Ny) for k in ("dns", "plain", "closure", "closure_mean", "closure_host")}
     ms = {}
     for name, (U, force) in runs.items():
         P = cf.pressure(U, coarse)
@@ -357,6 +360,8 @@ def bench_closure(cf, snapshots, every, eval_steps, epochs, sample_every=10):
         "coarse_plain_steps_per_s": round(1000.0 / ms["plain"], 1),
         "coarse_closure_steps_per_s": round(1000.0 / ms["closure"], 1),
         "coarse_closure_mean_steps_per_s": round(1000.0 / ms["closure_mean"], 1),
+        "coarse_closure_host_callback_steps_per_s": round(1000.0 / ms["closure_host"], 1),
+        "closure_device_equals_host_callback": bool(torch.equal(runs["closure"][0], runs["closure_host"][0])),
         "mean_profile_rel_err": {"plain": round(em_p, 5), "closure": round(em_c, 5),
                                  "closure_mean": round(em_m, 5)},
         "reynolds_stress_rel_err": {"plain": dict(zip(names, [round(e, 4) for e in ec_p])),

@@ -323,6 +323,14 @@ class _Forcing:
         return False
 
 
+def _forcing_hook(scheme, U, forcing):
+    """Context installing `forcing` on the scheme's handle for one call: the
+    forcing's own installer when it has one (the learned closure registers
+    itself as a device-side forcing), else the host callback."""
+    install = getattr(forcing, "_install", None)
+    return install(scheme, U) if install is not None else _Forcing(scheme, U, forcing)
+
+
 class _Nonlinear:
     """nonlinear(tstage, U, N) (cnrk2.jl:121-126,172) given by the caller in
     place of the problem's NonLinearTerm, installed for one call: at every
@@ -391,7 +399,7 @@ def step_(scheme, nlterm, U, P, t, forcing=None, pressuregradient=None, bulkvelo
         if custom is not None:
             hooks.enter_context(_Nonlinear(scheme, U, custom))
         if forcing is not None:
-            hooks.enter_context(_Forcing(scheme, U, forcing))
+            hooks.enter_context(_forcing_hook(scheme, U, forcing))
         run()
     return tout.value
 
@@ -422,7 +430,7 @@ class Flow:
         if p.forcing is None:  # forcing=channel.forcing (ext/ChannelFlowFlowsExt.jl:48-49)
             run()
         else:
-            with _Forcing(p.scheme, U, p.forcing):
+            with _forcing_hook(p.scheme, U, p.forcing):
                 run()
         return state
 

@@ -41,7 +41,7 @@ SYMBOLS = ["cf_problem_create", "cf_problem_destroy", "cf_last_error", "cf_probl
            "cf_step_timed",
            "cf_last_gradient", "cf_forward", "cf_inverse", "cf_nonlinear", "cf_stokes_solve", "cf_project",
            "cf_pressure", "cf_dot", "cf_dissipation", "cf_power_input", "cf_plane_statistics", "cf_advance_to", "cf_advance_to_noalloc", "cf_gaussian_noise", "cf_random_state",
-           "cf_ensemble_create", "cf_ensemble_advance", "cf_ensemble_info", "cf_ensemble_destroy", "cf_set_forcing", "cf_set_nonlinear",
+           "cf_ensemble_create", "cf_ensemble_advance", "cf_ensemble_info", "cf_ensemble_destroy", "cf_set_forcing", "cf_set_closure", "cf_set_nonlinear",
            "cf_velocity_gradient", "cf_box_filter", "cf_mlp_features", "cf_mlp_infer", "cf_mlp_work_floats",
            "cf_mlp_grad", "cf_mlp_adam", "cf_problem_tune", "cf_problem_tiles", "cf_problem_get_tiles", "cf_problem_set_tiles",
            "cf_forward3", "cf_inverse3"]
@@ -87,6 +87,7 @@ def load(path=LIB_PATH):
     L.cf_ensemble_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.cf_ensemble_destroy.argtypes = [vp]
     L.cf_set_forcing.argtypes = [vp, FORCING_FN, vp, vp, vp, vp]
+    L.cf_set_closure.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i]
     L.cf_set_nonlinear.argtypes = [vp, FORCING_FN, vp, vp, vp, vp]
     L.cf_velocity_gradient.argtypes = [vp, vp, vp, vp, vp, vp]
     L.cf_box_filter.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, d, d, vp]

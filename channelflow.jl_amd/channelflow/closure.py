@@ -166,35 +166,17 @@ class SubgridMLP:
                                             g.Nxp, g.Nzp, g.Ny, cf._ptr(out), cf._stream()))
         return out
 
-    def forcing(self, problem, modes="all"):
+    def forcing(self, problem, modes="all", device=True):
         """forcing(t, U, F) adding the predicted subgrid force (spectral) to F.
         modes="mean" keeps only its plane-mean part (the (0,0) Fourier mode,
-        a function of y), the part that sets the mean profile."""
+        a function of y), the part that sets the mean profile.  Passed to
+        step_ or a Flow, device=True registers it as a device-side forcing
+        (cf_set_closure: no host callback per stage, the step stays
+        graph-replayed); device=False calls it back from the host at every
+        stage.  Both enqueue the same kernels and give the same result."""
         if modes not in ("all", "mean"):
             raise ValueError("modes must be 'all' or 'mean'")
-        cf = _cf()
-        torch = cf._torch()
-        g = problem.grid
-        # the step's nonlinear term is live in the problem's own workspaces
-        # while the callback runs: gradients and transforms use a helper
-        # problem (same grid, base flow and form) with workspaces of its own
-        helper = cf.ChannelFlowProblem(g, problem.baseflow, problem.scheme.nu, problem.scheme.dt, form=problem.form)
-        gbuf = torch.empty((3, 3) + g.phys_shape, dtype=torch.float64, device="cuda")
-        fbuf = torch.empty((3,) + g.phys_shape, dtype=torch.float64, device="cuda")
-        spec = g.zeros_spec(3)
-        h = helper.scheme.h
-
-        def force(t, U, F):
-            self.predict(helper, U, grad=gbuf, out=fbuf)
-            _lib.check(_lib.load().cf_forward3(h, cf._ptr(fbuf), cf._ptr(spec[0]), cf._ptr(spec[1]), cf._ptr(spec[2]),
-                                               cf._stream()))
-            if modes == "mean":
-                F[:, :, 0, 0] += spec[:, :, 0, 0]
-            else:
-                F.add_(spec)
-
-        force.helper = helper  # keeps the helper's handle alive with the callable
-        return force
+        return ClosureForcing(self, problem, modes, device)
 
     # -- persistence ----------------------------------------------------------
     def state_dict(self):
@@ -206,6 +188,63 @@ class SubgridMLP:
         self.wbf.copy_(bf16_bits(self.theta))
         self.norm.copy_(torch.as_tensor(np.asarray(d["norm"], np.float32)))
         self.steps = int(d.get("steps", 0))
+
+
+class ClosureForcing:
+    """The closure's forcing(t, U, F) on one problem's grid.  The stage's
+    nonlinear term is live in the stepping problem's workspaces while the
+    force is computed, so gradients and transforms run on a helper problem
+    (same grid, base flow and form) with workspaces of its own."""
+
+    def __init__(self, net, problem, modes, device):
+        cf = _cf()
+        torch = cf._torch()
+        g = problem.grid
+        self.net, self.modes, self.device = net, modes, device
+        self.helper = cf.ChannelFlowProblem(g, problem.baseflow, problem.scheme.nu, problem.scheme.dt,
+                                            form=problem.form)
+        self.gbuf = torch.empty((3, 3) + g.phys_shape, dtype=torch.float64, device="cuda")
+        self.fbuf = torch.empty((3,) + g.phys_shape, dtype=torch.float64, device="cuda")
+        self.spec = g.zeros_spec(3)
+
+    def __call__(self, t, U, F):
+        cf = _cf()
+        self.net.predict(self.helper, U, grad=self.gbuf, out=self.fbuf)
+        spec = self.spec
+        _lib.check(_lib.load().cf_forward3(self.helper.scheme.h, cf._ptr(self.fbuf), cf._ptr(spec[0]),
+                                           cf._ptr(spec[1]), cf._ptr(spec[2]), cf._stream()))
+        if self.modes == "mean":
+            F[:, :, 0, 0] += spec[:, :, 0, 0]
+        else:
+            F.add_(spec)
+
+    def _install(self, scheme, U):
+        cf = _cf()
+        return _DeviceClosure(self, scheme) if self.device else cf._Forcing(scheme, U, self)
+
+
+class _DeviceClosure:
+    """Registers a ClosureForcing on a scheme's handle for one call
+    (cf_set_closure) and removes it afterwards.  Its buffers, the network's
+    tensors and the scheme's forcing fields keep their addresses from call to
+    call, so the step's cached graph is replayed."""
+
+    def __init__(self, force, scheme):
+        cf = _cf()
+        self.force, self.scheme = force, scheme
+        self.F = cf._scheme_buffer(scheme, "_forcing_buffer")
+
+    def __enter__(self):
+        cf = _cf()
+        f, net, F = self.force, self.force.net, self.F
+        _lib.check(_lib.load().cf_set_closure(self.scheme.h, f.helper.scheme.h, cf._ptr(net.theta), cf._ptr(net.wbf),
+                                              cf._ptr(net.norm), cf._ptr(f.gbuf), cf._ptr(f.fbuf), cf._ptr(F[0]),
+                                              cf._ptr(F[1]), cf._ptr(F[2]), 1 if f.modes == "mean" else 0))
+        return self
+
+    def __exit__(self, *exc):
+        _lib.load().cf_set_closure(self.scheme.h, None, None, None, None, None, None, None, None, None, 0)
+        return False
 
 
 def subgrid_force(fine, coarse, U, widths=None):

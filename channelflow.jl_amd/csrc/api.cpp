@@ -16,6 +16,7 @@
 #include <utility>
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <list>
 #include <vector>
 
@@ -121,12 +122,34 @@ struct cf_problem {
     // group layout (lanes, coefficients per lane) of the step's wall-normal
     // stage and prologue kernels
     int yl = 0, yk = 0;
+    unsigned long long serial = 0;  // creation number, unique in the process
+    // learned closure registered as a device-side forcing (cf_set_closure):
+    // the helper problem whose workspaces compute it, the network and the
+    // caller's buffers; all null when none is registered
+    struct ClosureCfg {
+        cf_problem* helper = nullptr;
+        unsigned long long helper_serial = 0;  // tells a helper from a later one at the same address
+        const float* theta = nullptr;
+        const unsigned short* wbf = nullptr;
+        const float* norm = nullptr;
+        double* grad9 = nullptr;
+        double* force3 = nullptr;
+        double2* F[3] = {nullptr, nullptr, nullptr};
+        int mean_only = 0;
+        bool active() const { return theta != nullptr; }
+        bool operator==(const ClosureCfg& o) const {
+            return helper == o.helper && helper_serial == o.helper_serial && theta == o.theta && wbf == o.wbf &&
+                   norm == o.norm && grad9 == o.grad9 && force3 == o.force3 && F[0] == o.F[0] && F[1] == o.F[1] &&
+                   F[2] == o.F[2] && mean_only == o.mean_only;
+        }
+    } closure;
     // one-step graphs of the plain step (no host callbacks), one per field
     // pointer set and constraint, least recently used dropped past GRAPH_CACHE;
     // cleared when the tiling changes
     struct StepGraph {
         const void* f[4];
         Constraint c;
+        ClosureCfg cl;  // the closure the graph computes (none: all null)
         hipGraphExec_t exec;
         unsigned long long used;
     };
@@ -293,6 +316,8 @@ extern "C" int cf_problem_create(const cf_config* cfg, cf_problem** out) {
         return fail(CF_ERR_UNSUPPORTED, "grid too large for one device");
 
     cf_problem* h = new cf_problem();
+    static std::atomic<unsigned long long> created{0};
+    h->serial = ++created;
     h->device = cfg->device;
     DeviceGuard dg(h->device);
     GridDims& gd = h->gd;
@@ -726,13 +751,16 @@ static YArgs stage_args(cf_problem* h, double2* const U[3], double2* P, int j, c
 
 static const double STAGE_T[3] = {0.0, 1.0 / 3.0, 3.0 / 4.0};  // stage times, cnrk2.jl:170
 
+static int enqueue_closure(cf_problem* h, double2* const U[3], hipStream_t st);
+static int forward_fields(cf_problem* h, int nf, const double* u, double* const* U, hipStream_t st);
+
 // nsteps CNRK2 steps of length dt with the stage solver banks `banks`
 // (default: the problem's dt and its three stage banks)
 static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st,
                          Timer* tm = nullptr, double dt = 0.0, DevBank* const* banks = nullptr, double t0 = 0.0) {
     const GridDims& gd = h->gd;
     const bool mixed = h->mixed != 0;
-    const bool forced = h->force_fn != nullptr;
+    const bool forced = h->force_fn != nullptr || h->closure.active();
     const bool custom = h->nl_fn != nullptr;  // caller's nonlinear(tstage, U, N) in coefficients
     if (!(dt > 0.0)) dt = h->dt;
     for (int step = 0; step < nsteps; ++step) {
@@ -753,10 +781,12 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
                 if (h->form == CF_FORM_DIVERGENCE)
                     if (int rc = enqueue_nonlinear_coef(h, Ncoef, st)) return rc;
             }
-            if (forced) {
+            if (h->force_fn) {
                 if (h->force_fn(h->force_user, ts, (const double*)U[0], (const double*)U[1], (const double*)U[2],
                                 (double*)h->F[0], (double*)h->F[1], (double*)h->F[2], st) != 0)
                     return fail(CF_ERR_ARGUMENT, "forcing callback failed");
+            } else if (h->closure.active()) {
+                if (int rc = enqueue_closure(h, U, st)) return rc;
             }
             YArgs a = stage_args(h, U, P, j, c, dt, banks ? banks[j] : &h->stage[j], mixed, Ncoef, forced);
             if (custom) {  // N arrives in coefficients; no y-physical fields are needed
@@ -773,14 +803,17 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
 // Plain steps (no host callbacks) replayed from a cached one-step graph: one
 // hipGraphLaunch per step instead of the step's 13 kernel launches.  The graph
 // holds kernel nodes only (no memset or copy nodes) with the arguments of
-// these field pointers and this constraint.
+// these field pointers, this constraint and the registered closure (whose
+// kernels it includes).
 static constexpr size_t GRAPH_CACHE = 4;
 
 static int graph_steps(cf_problem* h, double2* const U[3], double2* P, int nsteps, const Constraint& c, hipStream_t st) {
     const void* key[4] = {U[0], U[1], U[2], P};
     cf_problem::StepGraph* g = nullptr;
     for (auto& e : h->graphs)
-        if (std::equal(key, key + 4, e.f) && e.c.bulk == c.bulk && e.c.c0 == c.c0 && e.c.c1 == c.c1) g = &e;
+        if (std::equal(key, key + 4, e.f) && e.c.bulk == c.bulk && e.c.c0 == c.c0 && e.c.c1 == c.c1 &&
+            e.cl == h->closure)
+            g = &e;
     if (!g) {
         if (!h->cap && hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking) != hipSuccess)
             return fail(CF_ERR_DEVICE, "hipStreamCreate");
@@ -806,6 +839,7 @@ static int graph_steps(cf_problem* h, double2* const U[3], double2* P, int nstep
         cf_problem::StepGraph e;
         std::copy(key, key + 4, e.f);
         e.c = c;
+        e.cl = h->closure;
         e.exec = exec;
         h->graphs.push_back(e);
         g = &h->graphs.back();
@@ -933,6 +967,8 @@ extern "C" int cf_advance_to_noalloc(cf_problem* h, double* U0, double* U1, doub
     return advance_to(h, U0, U1, U2, P, t, T, constraint, values, t_out, stream, true);
 }
 
+static int velocity_gradient_on(cf_problem* h, double2* const U[3], double* out, hipStream_t st);
+
 // Velocity-gradient tensor of the total velocity V = U + Ub e_x on the padded
 // grid: the convective prologue (V, dV/dy), the z pass with dV/dz, and the
 // gradient x pass (dV/dx = i alpha k V) writing 9 real fields.
@@ -944,8 +980,13 @@ extern "C" int cf_velocity_gradient(cf_problem* h, const double* U0, const doubl
     const GridDims& gd = h->gd;
     if (gd.Nxp % 2 != 0 || h->xwavesg < 1) return fail(CF_ERR_UNSUPPORTED, "velocity gradient needs an even padded Nx");
     DeviceGuard dg(h->device);
-    hipStream_t st = (hipStream_t)stream;
     double2* U[3] = {(double2*)U0, (double2*)U1, (double2*)U2};
+    return velocity_gradient_on(h, U, out, (hipStream_t)stream);
+}
+
+// the gradient's three launches on h's workspaces (checked by the caller)
+static int velocity_gradient_on(cf_problem* h, double2* const U[3], double* out, hipStream_t st) {
+    const GridDims& gd = h->gd;
     YArgs a = prologue_args(h, U, false);
     a.form = FORM_CONVECTIVE;
     HIPTRY(launch_ykernel(Y_PROLOGUE, a, st));
@@ -1073,8 +1114,64 @@ extern "C" int cf_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v
     return CF_OK;
 }
 
+// The registered closure's force of one stage into F, on the helper's
+// workspaces: velocity gradient of U, the network at every padded point, the
+// forward transform of its three outputs, and for a plane-mean closure every
+// mode but (0, 0) zeroed.  The same kernels, in the same order, as the host
+// forcing that calls cf_velocity_gradient, cf_mlp_infer and cf_forward3.
+static int enqueue_closure(cf_problem* h, double2* const U[3], hipStream_t st) {
+    const cf_problem::ClosureCfg& c = h->closure;
+    cf_problem* hp = c.helper;
+    const GridDims& gd = hp->gd;
+    if (int rc = velocity_gradient_on(hp, U, c.grad9, st)) return rc;
+    MLPInferArgs a = mlp_args(c.theta, c.wbf, c.norm, c.grad9, gd.Nxp, gd.Nzp, gd.Ny);
+    for (int i = 0; i < 3; ++i) a.out[i] = c.force3 + (size_t)a.npts * i;
+    HIPTRY(launch_mlp_infer(a, st));
+    double* Fs[3] = {(double*)c.F[0], (double*)c.F[1], (double*)c.F[2]};
+    if (int rc = forward_fields(hp, 3, c.force3, Fs, st)) return rc;
+    if (c.mean_only) HIPTRY(launch_keep_mean(c.F[0], c.F[1], c.F[2], gd.S, gd.B, st));
+    return 0;
+}
+
+extern "C" int cf_set_closure(cf_problem* h, cf_problem* helper, const float* theta, const unsigned short* wbf,
+                              const float* norm, double* grad9, double* force3, double* F0, double* F1, double* F2,
+                              int mean_only) {
+    if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    cf_problem::ClosureCfg c;
+    if (theta) {
+        if (h->force_fn) return fail(CF_ERR_ARGUMENT, "a host forcing is registered on this problem");
+        if (!helper || helper == h) return fail(CF_ERR_ARGUMENT, "the closure needs a helper problem of its own");
+        if (!wbf || !norm) return fail(CF_ERR_ARGUMENT, "null network argument");
+        const void* f[5] = {grad9, force3, F0, F1, F2};
+        if (int rc = check_fields(f, 5)) return rc;
+        const GridDims &a = h->gd, &b = helper->gd;
+        if (helper->device != h->device) return fail(CF_ERR_ARGUMENT, "the helper must share the problem's device");
+        if (a.Nx != b.Nx || a.Ny != b.Ny || a.Nz != b.Nz || h->Lx != helper->Lx || h->Lz != helper->Lz)
+            return fail(CF_ERR_DIMENSION, "the helper must have the problem's grid");
+        if (b.Nxp % 2 != 0 || helper->xwavesg < 1)
+            return fail(CF_ERR_UNSUPPORTED, "the closure's velocity gradient needs an even padded Nx");
+        if (int rc = mlp_dims_ok(b.Nxp, b.Nzp, b.Ny)) return rc;
+        c.helper = helper;
+        c.helper_serial = helper->serial;
+        c.theta = theta;
+        c.wbf = wbf;
+        c.norm = norm;
+        c.grad9 = grad9;
+        c.force3 = force3;
+        c.F[0] = (double2*)F0;
+        c.F[1] = (double2*)F1;
+        c.F[2] = (double2*)F2;
+        c.mean_only = mean_only ? 1 : 0;
+    }
+    h->closure = c;
+    if (!h->force_fn)
+        for (int i = 0; i < 3; ++i) h->F[i] = c.F[i];
+    return CF_OK;
+}
+
 extern "C" int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, double* F0, double* F1, double* F2) {
     if (!h) return fail(CF_ERR_ARGUMENT, "null problem");
+    if (fn && h->closure.active()) return fail(CF_ERR_ARGUMENT, "a closure is registered on this problem");
     if (fn) {
         const void* f[3] = {F0, F1, F2};
         if (int rc = check_fields(f, 3)) return rc;
@@ -1977,6 +2074,7 @@ extern "C" int cf_ensemble_create(cf_problem* const* problems, double* const* fi
         if (h->device != problems[0]->device) return fail(CF_ERR_ARGUMENT, "ensemble problems must share a device");
         if (h->mixed) return fail(CF_ERR_UNSUPPORTED, "ensembles run the FP64 step");
         if (h->force_fn) return fail(CF_ERR_UNSUPPORTED, "a forced problem cannot join an ensemble (host callback)");
+        if (h->closure.active()) return fail(CF_ERR_UNSUPPORTED, "a problem with a closure cannot join an ensemble");
         if (h->nl_fn) return fail(CF_ERR_UNSUPPORTED, "a problem with a nonlinear callback cannot join an ensemble");
         if (h->form == CF_FORM_DIVERGENCE || h->gd.Nxp % 2 != 0)
             return fail(CF_ERR_UNSUPPORTED, "ensembles need the rotating or convective form and an even padded Nx");

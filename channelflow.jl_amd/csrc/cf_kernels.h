@@ -149,6 +149,10 @@ hipError_t launch_mlp_grad(const MLPTrainArgs& a, float* grad, hipStream_t st); 
 hipError_t launch_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v, const float* grad, float lr, int step,
                            hipStream_t st);
 
+// zeroes every Fourier mode of three spectral fields but (k, l) = (0, 0):
+// S coefficients per field, B modes per y (the (0,0) mode sits at n B)
+hipError_t launch_keep_mean(double2* F0, double2* F1, double2* F2, long long S, long long B, hipStream_t st);
+
 hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st);
 unsigned y_blocks(const GridDims& gd);  // blocks of a y launch (partials of Y_DOT)
 // out[0] = sum of part[0..n) in a fixed order (one block)

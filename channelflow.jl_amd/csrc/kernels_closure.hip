@@ -437,4 +437,20 @@ hipError_t launch_mlp_adam(float* theta, unsigned short* wbf, float* m, float* v
     return hipGetLastError();
 }
 
+// Plane-mean part of a spectral force: every mode but (k, l) = (0, 0) zeroed
+__global__ void __launch_bounds__(256) keep_mean_kernel(double2* F0, double2* F1, double2* F2, long long S, long long B) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= S || i % B == 0) return;
+    const double2 z = make_double2(0.0, 0.0);
+    F0[i] = z;
+    F1[i] = z;
+    F2[i] = z;
+}
+
+hipError_t launch_keep_mean(double2* F0, double2* F1, double2* F2, long long S, long long B, hipStream_t st) {
+    if (S <= 0 || B <= 0 || (S + 255) / 256 > 0x7FFFFFFFll) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(keep_mean_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st, F0, F1, F2, S, B);
+    return hipGetLastError();
+}
+
 }  // namespace cf

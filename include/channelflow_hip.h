@@ -186,6 +186,25 @@ typedef int (*cf_forcing_fn)(void* user, double tstage, const double* U0, const 
                              double* F0, double* F1, double* F2, void* stream);
 int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, double* F0, double* F1, double* F2);
 
+/* The learned closure as a device-side forcing (no counterpart in the
+ * reference, whose forcing is a host callable; for callers that want the
+ * closure's step replayed from a graph, without a host callback per stage).
+ * While registered, every stage of cf_step / cf_advance / cf_advance_to /
+ * cf_step_timed computes F0..F2 on the device before the stage kernel, with
+ * the kernels a host forcing would enqueue: cf_velocity_gradient of the stage
+ * velocity into grad9 (9 padded physical fields), cf_mlp_infer into force3
+ * (3 padded physical fields) and cf_forward3 of force3 into F0..F2 (spectral),
+ * all on `helper`'s workspaces; mean_only != 0 then zeroes every mode but
+ * (k, l) = (0, 0).  helper: another problem on the same grid and device
+ * (the stepping problem's workspaces hold the stage's nonlinear term); it
+ * and every buffer must stay alive while registered.  theta, wbf, norm as
+ * for cf_mlp_infer; they are read at every stage, so training between steps
+ * takes effect.  The plain step stays graph-replayed (one graph per field
+ * set, constraint and closure).  theta = NULL removes the closure.  Exclusive
+ * with cf_set_forcing; a problem with a closure cannot join an ensemble. */
+int cf_set_closure(cf_problem* h, cf_problem* helper, const float* theta, const unsigned short* wbf, const float* norm,
+                   double* grad9, double* force3, double* F0, double* F1, double* F2, int mean_only);
+
 /* nonlinear(tstage, U, N) (cnrk2.jl:121-126,172): the step's nonlinear term
  * supplied by the caller in place of the problem's form.  At each stage of
  * cf_step / cf_advance / cf_advance_to / cf_step_timed the library calls
