@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libcfhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # heaviest translation units first (the wall-normal layouts, then the x/z passes)
-SOURCES = [*[f"kernels_y_p{p}.hip" for p in (6, 4, 3, 2, 1, 5)], "kernels_xz.hip", "kernels_closure.hip", "kernels_y.hip",
+SOURCES = [*[f"kernels_y_p{p}.hip" for p in (6, 7, 4, 3, 2, 1, 5)], "kernels_xz.hip", "kernels_closure.hip", "kernels_y.hip",
            "kernels_rng.hip", "scratch.hip", "setup.cpp", "api.cpp"]
 
 FLAGS = [

@@ -139,7 +139,7 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 // one row; L = 32 is a row pair, L = 64 the whole wave.  Every lane of a group
 // must call a group operation together (values of other groups are never used).
 //
-// Each lane holds K consecutive Chebyshev coefficients (K = 4 or 8, even:
+// Each lane holds K consecutive Chebyshev coefficients (K = 4, 6 or 8, even:
 // element e of a lane has the parity of e), so a group covers degrees below
 // K L.  tab: per-degree constants (see build_cheb_table), NT = K L entries
 // per array.

@@ -163,7 +163,7 @@ hipError_t launch_reduce_columns(const double* part, int nblocks, int nq, double
 hipError_t launch_gaussian(double* out3, long long n, double eps, unsigned long long seed, hipStream_t st);
 constexpr int PLANE_STATS_Q = 9;  // per y: means of u, v, w; covariances uu vv ww uv uw vw
 int y_group_lanes(int Ny);  // lanes L per mode of the wall-normal kernels (0: Ny unsupported)
-int y_group_k(int Ny);      // coefficients K per lane (4, or 8 above 256 points)
+int y_group_k(int Ny);      // coefficients K per lane (4; 6 above 256 points, 8 above 384)
 // LDS bytes of a y launch in layout (L, K); 0 when (L, K) is not built or does
 // not cover Ny coefficients
 size_t y_lds_bytes_layout(int Ny, int ld, int L, int K);

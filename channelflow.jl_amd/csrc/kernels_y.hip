@@ -2,8 +2,8 @@
 //
 // A block owns M = NT/L consecutive Fourier modes s = ix + Nxh*iz of the
 // reference (k,l,n) layout; a group of L lanes owns one mode, lane t holding
-// Chebyshev coefficients n = Kt..Kt+K-1 in registers (K = 4, or 8 when Ny
-// exceeds 256).  HBM rows (fixed n, M
+// Chebyshev coefficients n = Kt..Kt+K-1 in registers (K = 4, or 6 / 8 when
+// Ny exceeds 256 / 384).  HBM rows (fixed n, M
 // consecutive modes) are moved through LDS, which also stages the DCT-I
 // (REDFT00) as a length-2P Stockham FFT of the even extension.
 //
@@ -16,7 +16,7 @@
 
 // This file is compiled as several translation units so that the layouts
 // build in parallel: part 0 (this file on its own) holds the dispatchers and
-// the reduction kernels, parts 1-6 (kernels_y_p<n>.hip) each instantiate the
+// the reduction kernels, parts 1-7 (kernels_y_p<n>.hip) each instantiate the
 // kernels of some (L, K) layouts; the list is at the end of the file.
 #ifndef CF_Y_PART
 #define CF_Y_PART 0
@@ -607,10 +607,11 @@ __device__ __forceinline__ void ystage_nyquist(const YArgs& a, unsigned blk) {
 
 template <int L, int K, typename SC>
 // two waves per SIMD (<= 256 registers per lane): one wave per SIMD leaves
-// the wave-serial scans and LDS FFTs unhidden.  K = 8 (Ny above 256) keeps
-// twice the coefficients per lane and runs one wave per SIMD, whose larger
-// register budget holds them without scratch.
-__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(K == 8 ? 1 : 2))) ystage_kernel(YArgs a) {
+// the wave-serial scans and LDS FFTs unhidden.  K = 6 and 8 (Ny above 256)
+// keep 1.5 and 2 times the coefficients per lane and run one wave per SIMD,
+// whose larger register budget holds them without scratch (at two waves
+// K = 6 spills 160 bytes per lane).
+__global__ void __launch_bounds__(YNT) __attribute__((amdgpu_waves_per_eu(K >= 6 ? 1 : 2))) ystage_kernel(YArgs a) {
     if (a.order.on) {  // block-uniform branches
         if (blockIdx.x >= a.order.nyq_first) {
             ystage_nyquist<SC>(a, blockIdx.x - a.order.nyq_first);
@@ -1172,7 +1173,8 @@ unsigned y_blocks(const GridDims& gd) {
 }
 
 // Group layout of a wall-normal size: K = 4 coefficients per lane and the
-// smallest L with 4 L >= Ny; above 256 points K = 8 with L = 64 (up to 512).
+// smallest L with 4 L >= Ny; above 256 points L = 64 with K = 6 (up to 384)
+// or K = 8 (up to 512).
 int y_group_lanes(int Ny) {
     int need = (Ny + 3) / 4;
     if (need <= 4) return 4;
@@ -1183,16 +1185,17 @@ int y_group_lanes(int Ny) {
     if (Ny <= 512) return 64;
     return 0;
 }
-int y_group_k(int Ny) { return Ny <= 256 ? 4 : (Ny <= 512 ? 8 : 0); }
+int y_group_k(int Ny) { return Ny <= 256 ? 4 : (Ny <= 384 ? 6 : (Ny <= 512 ? 8 : 0)); }
 
 // DCT twiddle table (2P entries) followed by one region per wavefront
 size_t y_lds_bytes(int Ny, int ld) {
     return y_lds_bytes_layout(Ny, ld, y_group_lanes(Ny), y_group_k(Ny));
 }
 
-// Layouts built: K = 4 with L = 4..64 (the defaults), and K = 8 with L = 16,
-// 32, 64 (the only layout above 256 points; below it an alternative the
-// tuner may pick for the step's stage and prologue kernels).
+// Layouts built: K = 4 with L = 4..64 (the defaults up to 256 points), K = 6
+// with L = 64 (the default up to 384), and K = 8 with L = 16, 32, 64 (the
+// default up to 512; below it an alternative the tuner may pick for the
+// step's stage and prologue kernels).
 size_t y_lds_bytes_layout(int Ny, int ld, int L, int K) {
     if (K * L < Ny) return 0;
     if (K == 4) {
@@ -1205,6 +1208,7 @@ size_t y_lds_bytes_layout(int Ny, int ld, int L, int K) {
             default: return 0;
         }
     }
+    if (K == 6) return L == 64 ? y_lds_lk<64, 6>(Ny, ld) : 0;
     if (K == 8) {
         switch (L) {
             case 16: return y_lds_lk<16, 8>(Ny, ld);
@@ -1237,6 +1241,7 @@ hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
         if (a.yk == 8 && a.yl == 64) return launch_y_step<64, 8>(which, a, st);
         return hipErrorInvalidValue;
     }
+    if (y_group_k(a.gd.Ny) == 6) return launch_y<64, 6>(which, a, st);
     if (y_group_k(a.gd.Ny) == 8) return launch_y<64, 8>(which, a, st);
     switch (y_group_lanes(a.gd.Ny)) {
         case 4: return launch_y<4, 4>(which, a, st);
@@ -1251,9 +1256,10 @@ hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
 #endif  // CF_Y_PART == 0
 
 // Layouts and the translation unit that instantiates each: the default
-// layouts (K = 4, L = 4..64, and K = 8 on L = 64 above 256 points) with every
-// kernel, the alternative K = 8 layouts on 16 and 32 lanes with the step's
-// two kernels only, and the ensemble's batched kernels on K = 4.
+// layouts (K = 4 on L = 4..64; K = 6 and 8 on L = 64 above 256 and 384
+// points) with every kernel, the alternative K = 8 layouts on 16 and 32
+// lanes with the step's two kernels only, and the ensemble's batched
+// kernels on K = 4.
 #if CF_Y_PART == 1
 template hipError_t launch_y<4, 4>(int, const YArgs&, hipStream_t);
 template hipError_t launch_y<8, 4>(int, const YArgs&, hipStream_t);
@@ -1274,6 +1280,8 @@ template hipError_t launch_y_step<32, 8>(int, const YArgs&, hipStream_t);
 #elif CF_Y_PART == 6
 template hipError_t launch_y<64, 8>(int, const YArgs&, hipStream_t);
 template hipError_t launch_y_step<64, 8>(int, const YArgs&, hipStream_t);
+#elif CF_Y_PART == 7
+template hipError_t launch_y<64, 6>(int, const YArgs&, hipStream_t);
 #endif
 
 }  // namespace cf

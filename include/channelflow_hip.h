@@ -239,7 +239,8 @@ int cf_problem_tiles(const cf_problem* h, int out[3]);
  * lines, velocity held in registers, 2 six lines with direct-to-LDS loads;
  * other forms: 0), tiles[3] x wavefronts per block (1..the variant's LDS
  * limit), tiles[4] Chebyshev coefficients per lane of the wall-normal stage
- * and prologue kernels (4 or 8 where Ny <= 256 and Ny > 64; 8 above 256).
+ * and prologue kernels (4 or 8 where 64 < Ny <= 256; 6 or 8 for
+ * 256 < Ny <= 384; 8 above 384).
  * The x and z choices give identical results; the wall-normal layout changes
  * the summation order of its scans (results agree to roundoff).  An
  * unavailable choice returns CF_ERR_ARGUMENT and changes nothing. */

@@ -94,12 +94,14 @@ def test_set_tiles_rejects(gpu):
         cf.set_tiles(conv, cf.get_tiles(conv)[0], 1, 1)
 
 
-@pytest.mark.parametrize("shape", [(16, 97, 12), (8, 193, 6), (24, 65, 16)])
-def test_wall_normal_layouts(gpu, shape):
-    """The step's stage and prologue kernels with 4 and with 8 Chebyshev
-    coefficients per lane (half the lanes per mode): three steps of each
-    agree with the oracle within its roundoff envelope (the scans sum in a
-    different order, so the two agree to roundoff, not bit for bit)."""
+@pytest.mark.parametrize("shape,coefs,bad", [((16, 97, 12), (4, 8), 6), ((8, 193, 6), (4, 8), 6),
+                                             ((24, 65, 16), (4, 8), 6), ((6, 257, 4), (6, 8), 4)])
+def test_wall_normal_layouts(gpu, shape, coefs, bad):
+    """The step's stage and prologue kernels with their default Chebyshev
+    coefficients per lane (4; 6 above 256 points) and with 8 (on half the
+    lanes per mode, or on the same 64 lanes above 256 points): three steps of
+    each agree with the oracle within its roundoff envelope (the scans sum in
+    a different order, so the layouts agree to roundoff, not bit for bit)."""
     import torch
     from oracle import oracle as O
     from test_gpu_parity import _split, assert_parity, roundoff_envelope
@@ -116,7 +118,8 @@ def test_wall_normal_layouts(gpu, shape):
     env = roundoff_envelope(lambda a, b: _split(ref.step(a, b, 3)), [U0, P0], _split((U1, P1)), trials=8)
     keep = cf.get_tiles(prob)
     try:
-        for yk in (4, 8):
+        assert keep[4] == coefs[0]
+        for yk in coefs:
             got = cf.set_tiles(prob, keep[0], keep[2], keep[3], y_coefs=yk)
             assert got[4] == yk
             U = torch.as_tensor(U0, device="cuda")
@@ -127,4 +130,4 @@ def test_wall_normal_layouts(gpu, shape):
     finally:
         cf.set_tiles(prob, keep[0], keep[2], keep[3], y_coefs=keep[4])
     with pytest.raises(ValueError):
-        cf.set_tiles(prob, keep[0], keep[2], keep[3], y_coefs=6)
+        cf.set_tiles(prob, keep[0], keep[2], keep[3], y_coefs=bad)
