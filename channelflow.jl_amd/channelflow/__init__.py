@@ -464,9 +464,9 @@ def get_tiles(problem):
 def set_tiles(problem, z_width, x_variant, x_waves, y_coefs=None):
     """Set the step's z tile width, x pass variant (0/"six", 1/"split",
     2/"async"), x wavefronts per block and (optionally) the wall-normal
-    kernels' coefficients per lane: the default (4; 6 above 256 points) or 8
-    (cf_problem_set_tiles); an unavailable choice raises and leaves the
-    tiling as it was.  The x and z
+    kernels' coefficients per lane: the default (4; 6 above 256 points) or
+    an alternative built for the size, 6 or 8 (cf_problem_set_tiles); an
+    unavailable choice raises and leaves the tiling as it was.  The x and z
     choices give identical results; the wall-normal layout agrees to
     roundoff."""
     v = X_VARIANTS.get(x_variant, x_variant)

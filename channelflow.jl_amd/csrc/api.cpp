@@ -1675,13 +1675,15 @@ extern "C" int cf_problem_tune(cf_problem* h, void* stream) {
     }
     // group layout of the step's stage and prologue kernels: the default or,
     // where built, K = 8 (on half the lanes where the default is K = 4, on
-    // the same 64 lanes where it is K = 6);
+    // the same 64 lanes where it is K = 6) or K = 6 on 32 lanes (where the
+    // default is 64 lanes of 4 and Ny <= 192);
     // timed on stage-2 launches over the workspaces (U in R, P in the z
     // buffer; no gradient output)
     if (!rc) {
         const int L0 = y_group_lanes(h->gd.Ny), K0 = y_group_k(h->gd.Ny);
-        int cand[2][2] = {{L0, K0}, {K0 == 6 ? L0 : L0 / 2, 8}};
-        const int ncand = ((K0 == 4 && L0 >= 32) || K0 == 6) ? 2 : 1;
+        int cand[3][2] = {{L0, K0}, {K0 == 6 ? L0 : L0 / 2, 8}, {32, 6}};
+        int ncand = ((K0 == 4 && L0 >= 32) || K0 == 6) ? 2 : 1;
+        if (K0 == 4 && L0 == 64 && h->gd.Ny <= 192) ncand = 3;  // 32 lanes of 6 also cover Ny
         const int keepL = h->yl, keepK = h->yk;
         int bestL = keepL, bestK = keepK;
         best = 1e30f;
@@ -1788,13 +1790,13 @@ extern "C" int cf_problem_set_tiles(cf_problem* h, const int tiles[5]) {
     int yl = 0;
     if (yk == y_group_k(h->gd.Ny)) {
         yl = y_group_lanes(h->gd.Ny);
-    } else if (yk == 8) {
+    } else if (yk == 6 || yk == 8) {
         yl = 16;
-        while (yl * 8 < h->gd.Ny) yl *= 2;
-        const size_t lds = y_lds_bytes_layout(h->gd.Ny, h->yld, yl, 8);
+        while (yl * yk < h->gd.Ny) yl *= 2;
+        const size_t lds = yl <= 64 ? y_lds_bytes_layout(h->gd.Ny, h->yld, yl, yk) : 0;
         if (lds == 0 || lds > 160 * 1024) return fail(CF_ERR_ARGUMENT, "wall-normal layout unavailable for this Ny");
     } else {
-        return fail(CF_ERR_ARGUMENT, "wall-normal coefficients per lane must be the default (4, or 6 above 256 points) or 8");
+        return fail(CF_ERR_ARGUMENT, "wall-normal coefficients per lane must be 4, 6 or 8");
     }
     DeviceGuard dg(h->device);
     if (int rc = invalidate_graphs(h)) return rc;

@@ -1193,9 +1193,9 @@ size_t y_lds_bytes(int Ny, int ld) {
 }
 
 // Layouts built: K = 4 with L = 4..64 (the defaults up to 256 points), K = 6
-// with L = 64 (the default up to 384), and K = 8 with L = 16, 32, 64 (the
-// default up to 512; below it an alternative the tuner may pick for the
-// step's stage and prologue kernels).
+// with L = 64 (the default up to 384) and L = 32, and K = 8 with L = 16, 32,
+// 64 (the default up to 512); the non-default ones are alternatives the tuner
+// may pick for the step's stage and prologue kernels.
 size_t y_lds_bytes_layout(int Ny, int ld, int L, int K) {
     if (K * L < Ny) return 0;
     if (K == 4) {
@@ -1208,7 +1208,7 @@ size_t y_lds_bytes_layout(int Ny, int ld, int L, int K) {
             default: return 0;
         }
     }
-    if (K == 6) return L == 64 ? y_lds_lk<64, 6>(Ny, ld) : 0;
+    if (K == 6) return L == 64 ? y_lds_lk<64, 6>(Ny, ld) : (L == 32 ? y_lds_lk<32, 6>(Ny, ld) : 0);
     if (K == 8) {
         switch (L) {
             case 16: return y_lds_lk<16, 8>(Ny, ld);
@@ -1236,6 +1236,8 @@ hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
     if ((which == Y_STAGE || which == Y_PROLOGUE) && a.yk != 0 &&
         !(a.yk == y_group_k(a.gd.Ny) && a.yl == y_group_lanes(a.gd.Ny))) {
         if (!y_lds_bytes_layout(a.gd.Ny, a.ld, a.yl, a.yk)) return hipErrorInvalidValue;
+        if (a.yk == 6 && a.yl == 32) return launch_y_step<32, 6>(which, a, st);
+        if (a.yk == 6 && a.yl == 64) return launch_y_step<64, 6>(which, a, st);
         if (a.yk == 8 && a.yl == 16) return launch_y_step<16, 8>(which, a, st);
         if (a.yk == 8 && a.yl == 32) return launch_y_step<32, 8>(which, a, st);
         if (a.yk == 8 && a.yl == 64) return launch_y_step<64, 8>(which, a, st);
@@ -1257,9 +1259,9 @@ hipError_t launch_ykernel(int which, const YArgs& a, hipStream_t st) {
 
 // Layouts and the translation unit that instantiates each: the default
 // layouts (K = 4 on L = 4..64; K = 6 and 8 on L = 64 above 256 and 384
-// points) with every kernel, the alternative K = 8 layouts on 16 and 32
-// lanes with the step's two kernels only, and the ensemble's batched
-// kernels on K = 4.
+// points) with every kernel, the alternative layouts (K = 8 on 16 and 32
+// lanes, K = 6 on 32) with the step's two kernels only, and the ensemble's
+// batched kernels on K = 4.
 #if CF_Y_PART == 1
 template hipError_t launch_y<4, 4>(int, const YArgs&, hipStream_t);
 template hipError_t launch_y<8, 4>(int, const YArgs&, hipStream_t);
@@ -1282,6 +1284,8 @@ template hipError_t launch_y<64, 8>(int, const YArgs&, hipStream_t);
 template hipError_t launch_y_step<64, 8>(int, const YArgs&, hipStream_t);
 #elif CF_Y_PART == 7
 template hipError_t launch_y<64, 6>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_step<64, 6>(int, const YArgs&, hipStream_t);
+template hipError_t launch_y_step<32, 6>(int, const YArgs&, hipStream_t);
 #endif
 
 }  // namespace cf

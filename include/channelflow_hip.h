@@ -239,8 +239,10 @@ int cf_problem_tiles(const cf_problem* h, int out[3]);
  * lines, velocity held in registers, 2 six lines with direct-to-LDS loads;
  * other forms: 0), tiles[3] x wavefronts per block (1..the variant's LDS
  * limit), tiles[4] Chebyshev coefficients per lane of the wall-normal stage
- * and prologue kernels (4 or 8 where 64 < Ny <= 256; 6 or 8 for
- * 256 < Ny <= 384; 8 above 384).
+ * and prologue kernels: the default (4 up to Ny = 256, 6 up to 384, 8
+ * above) or an alternative where built: 8 on fewer lanes for
+ * 64 < Ny <= 384, 6 on 32 lanes for 97 <= Ny <= 192 and on 64 lanes for
+ * 193 <= Ny <= 256.
  * The x and z choices give identical results; the wall-normal layout changes
  * the summation order of its scans (results agree to roundoff).  An
  * unavailable choice returns CF_ERR_ARGUMENT and changes nothing. */
