@@ -506,6 +506,7 @@ def main():
     w = WORKLOADS[args.workload]
     g, prof, kw, prob = make_problem(cf, w)
     tiles = cf.tune(prob)  # tile shapes by timing (setup; results do not depend on them)
+    full_tiles = cf.get_tiles(prob)
     state = cf.perturbation_state(prob, w["amp"], seed=42)
     U, P = state.velocity, state.stagepressure
     U0, P0 = U.cpu().numpy(), P.cpu().numpy()
@@ -605,7 +606,9 @@ def main():
         "data": "synthetic (seeded divergence-free perturbation, kappa^-4 spectrum, amplitude 0.05)",
         "config": {"workload": w["desc"], "grid": list(w["N"]), "form": "rotational",
                    "constraint": "bulkvelocity (1.0, 0.0)" if w["bulk"] else "pressuregradient (0, 0)",
-                   "tiles": {"z_tile_k": tiles[0], "z_points_per_thread": tiles[1], "x_waves_per_block": tiles[2]},
+                   "tiles": {"z_tile_k": tiles[0], "z_points_per_thread": tiles[1], "x_waves_per_block": tiles[2],
+                             "x_variant": ["six", "split", "async"][full_tiles[2]],
+                             "y_coefs_per_lane": full_tiles[4]},
                    "step_algorithmic_bytes": step_bytes,
                    "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
                    "kernel_breakdown": breakdown},
