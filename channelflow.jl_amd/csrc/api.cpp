@@ -156,6 +156,11 @@ struct cf_problem {
     std::vector<StepGraph> graphs;
     unsigned long long graph_clock = 0;
     hipStream_t cap = nullptr;  // capture stream (created with the first graph)
+    // the registered closure's kernels run on `side`, forked from the step's
+    // stream after each stage's inputs are final and joined back before the
+    // stage kernel (created with the first closure registration)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 struct DeviceGuard {
@@ -283,6 +288,11 @@ static void free_all(cf_problem* h) {
     drop_graphs(h);
     if (h->cap) (void)hipStreamDestroy(h->cap);
     h->cap = nullptr;
+    if (h->side) (void)hipStreamDestroy(h->side);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    h->side = nullptr;
+    h->ev_fork = h->ev_join = nullptr;
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
 }
@@ -762,6 +772,10 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
     const bool mixed = h->mixed != 0;
     const bool forced = h->force_fn != nullptr || h->closure.active();
     const bool custom = h->nl_fn != nullptr;  // caller's nonlinear(tstage, U, N) in coefficients
+    // a registered closure (exclusive with a host forcing) computes F on the
+    // side stream while this stream computes the nonlinear term: in a captured
+    // graph the two form parallel branches of each stage
+    const bool closure = h->closure.active() && !h->force_fn;
     if (!(dt > 0.0)) dt = h->dt;
     for (int step = 0; step < nsteps; ++step) {
         if (int rc = tmark(tm, -1, st)) return rc;
@@ -771,6 +785,12 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
         for (int j = 0; j < 3; ++j) {
             const double ts = t0 + (double)step * dt + STAGE_T[j] * dt;
             double2* Ncoef[3] = {fld(h->R, gd.S, 0), fld(h->R, gd.S, 1), fld(h->R, gd.S, 2)};
+            if (closure) {  // fork: U of this stage is final on `st`
+                HIPTRY(hipEventRecord(h->ev_fork, st));
+                HIPTRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+                if (int rc = enqueue_closure(h, U, h->side)) return rc;
+                HIPTRY(hipEventRecord(h->ev_join, h->side));
+            }
             if (custom) {  // nonlinear(tstage, U, N) overwrites N (cnrk2.jl:121-126,173)
                 for (int i = 0; i < 3; ++i) Ncoef[i] = h->NL[i];
                 if (h->nl_fn(h->nl_user, ts, (const double*)U[0], (const double*)U[1], (const double*)U[2],
@@ -785,8 +805,8 @@ static int enqueue_steps(cf_problem* h, double2* const U[3], double2* P, int nst
                 if (h->force_fn(h->force_user, ts, (const double*)U[0], (const double*)U[1], (const double*)U[2],
                                 (double*)h->F[0], (double*)h->F[1], (double*)h->F[2], st) != 0)
                     return fail(CF_ERR_ARGUMENT, "forcing callback failed");
-            } else if (h->closure.active()) {
-                if (int rc = enqueue_closure(h, U, st)) return rc;
+            } else if (closure) {  // join: F is final before the stage kernel reads it
+                HIPTRY(hipStreamWaitEvent(st, h->ev_join, 0));
             }
             YArgs a = stage_args(h, U, P, j, c, dt, banks ? banks[j] : &h->stage[j], mixed, Ncoef, forced);
             if (custom) {  // N arrives in coefficients; no y-physical fields are needed
@@ -1162,6 +1182,13 @@ extern "C" int cf_set_closure(cf_problem* h, cf_problem* helper, const float* th
         c.F[1] = (double2*)F1;
         c.F[2] = (double2*)F2;
         c.mean_only = mean_only ? 1 : 0;
+        DeviceGuard dg(h->device);
+        if (!h->side && hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess)
+            return fail(CF_ERR_DEVICE, "hipStreamCreate");
+        if (!h->ev_fork && hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess)
+            return fail(CF_ERR_DEVICE, "hipEventCreate");
+        if (!h->ev_join && hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+            return fail(CF_ERR_DEVICE, "hipEventCreate");
     }
     h->closure = c;
     if (!h->force_fn)

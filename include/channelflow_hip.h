@@ -195,7 +195,9 @@ int cf_set_forcing(cf_problem* h, cf_forcing_fn fn, void* user, double* F0, doub
  * velocity into grad9 (9 padded physical fields), cf_mlp_infer into force3
  * (3 padded physical fields) and cf_forward3 of force3 into F0..F2 (spectral),
  * all on `helper`'s workspaces; mean_only != 0 then zeroes every mode but
- * (k, l) = (0, 0).  helper: another problem on the same grid and device
+ * (k, l) = (0, 0).  These kernels run on a stream of the problem's own,
+ * forked from the step's stream and joined back before each stage kernel
+ * (a parallel branch of the step's graph).  helper: another problem on the same grid and device
  * (the stepping problem's workspaces hold the stage's nonlinear term); it
  * and every buffer must stay alive while registered.  theta, wbf, norm as
  * for cf_mlp_infer; they are read at every stage, so training between steps
