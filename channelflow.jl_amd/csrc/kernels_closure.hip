@@ -153,8 +153,13 @@ __device__ __forceinline__ void hidden_layer(const unsigned short* in, const uns
 }
 }  // namespace
 
-// Inference: a wave owns 64 consecutive padded points; block = 4 waves.
-__global__ void __launch_bounds__(256) mlp_infer_kernel(MLPInferArgs a) {
+// Inference: a wave owns 64 consecutive padded points; a block holds the
+// bf16 weights (21 KB) and MLP_INFER_WAVES waves of two 64 x 64 activation
+// buffers (16 KB each): two waves per block fit three blocks in a CU's LDS
+// (four waves per block fit one, so that launches of up to a few hundred
+// blocks ran two rounds).
+constexpr int MLP_INFER_WAVES = 2;
+__global__ void __launch_bounds__(64 * MLP_INFER_WAVES) mlp_infer_kernel(MLPInferArgs a) {
     extern __shared__ double2 smem_raw[];
     unsigned short* W = reinterpret_cast<unsigned short*>(smem_raw);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
@@ -162,7 +167,7 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(MLPInferArgs a) {
     unsigned short* bufB = bufA + 64 * MH;
     copy_weights(W, a.wbf);
     __syncthreads();
-    const long long p0 = ((long long)blockIdx.x * 4 + wave) * 64;
+    const long long p0 = ((long long)blockIdx.x * MLP_INFER_WAVES + wave) * 64;
     {
         float x[MIN];
         const long long p = p0 + lane;
@@ -391,7 +396,7 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(float* theta, unsigned sh
     wbf[j] = f2bf(th);
 }
 
-size_t mlp_infer_lds() { return (size_t)(NPAR + 4) * 2 + 4 * 2 * 64 * MH * 2; }
+size_t mlp_infer_lds() { return (size_t)(NPAR + 4) * 2 + MLP_INFER_WAVES * 2 * 64 * MH * 2; }
 size_t mlp_train_lds() {
     const size_t wave = 3 * 32 * MH * 4 + (3 * 32 * MH + 32 * MK + 32 * MH) * 2;
     return ((size_t)(NPAR * 2 + 15) / 16) * 16 + 2 * wave;
@@ -400,8 +405,9 @@ size_t mlp_train_lds() {
 hipError_t launch_mlp_infer(const MLPInferArgs& a, hipStream_t st) {
     const size_t lds = mlp_infer_lds();
     (void)hipFuncSetAttribute((const void*)mlp_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const unsigned blocks = (unsigned)((a.npts + 255) / 256);
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), lds, st, a);
+    constexpr int pts = 64 * MLP_INFER_WAVES;
+    const unsigned blocks = (unsigned)((a.npts + pts - 1) / pts);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(pts), lds, st, a);
     return hipGetLastError();
 }
 
