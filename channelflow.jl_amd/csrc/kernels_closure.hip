@@ -153,22 +153,25 @@ __device__ __forceinline__ void hidden_layer(const unsigned short* in, const uns
 }
 }  // namespace
 
-// Inference: a wave owns 64 consecutive padded points; a block holds the
-// bf16 weights (21 KB) and MLP_INFER_WAVES waves of two 64 x 64 activation
-// buffers (16 KB each): two waves per block fit three blocks in a CU's LDS
-// (four waves per block fit one, so that launches of up to a few hundred
-// blocks ran two rounds).
-constexpr int MLP_INFER_WAVES = 2;
+// Inference: a wave owns MLP_INFER_RB row blocks of 16 consecutive padded
+// points; a block holds the bf16 weights (21 KB) and MLP_INFER_WAVES waves of
+// two activation buffers (16 RB x 64 bf16 each).  32 points per wave in
+// blocks of four waves (53 KB) fit three blocks, twelve waves, in a CU: each
+// wave's layers are a serial chain of matrix products and GELUs, so more and
+// shorter waves run faster (64 points per wave in blocks of four, 86 KB,
+// fit one block in a CU and ran 67 us at 32x33x32, 48 us in blocks of two).
+constexpr int MLP_INFER_RB = 2, MLP_INFER_WAVES = 4, MLP_INFER_PW = 16 * MLP_INFER_RB;
 __global__ void __launch_bounds__(64 * MLP_INFER_WAVES) mlp_infer_kernel(MLPInferArgs a) {
     extern __shared__ double2 smem_raw[];
+    constexpr int RB = MLP_INFER_RB, PW = MLP_INFER_PW;
     unsigned short* W = reinterpret_cast<unsigned short*>(smem_raw);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
-    unsigned short* bufA = W + NPAR + (NPAR & 3 ? 4 - (NPAR & 3) : 0) + wave * 2 * 64 * MH;
-    unsigned short* bufB = bufA + 64 * MH;
+    unsigned short* bufA = W + NPAR + (NPAR & 3 ? 4 - (NPAR & 3) : 0) + wave * 2 * PW * MH;
+    unsigned short* bufB = bufA + PW * MH;
     copy_weights(W, a.wbf);
     __syncthreads();
-    const long long p0 = ((long long)blockIdx.x * MLP_INFER_WAVES + wave) * 64;
-    {
+    const long long p0 = ((long long)blockIdx.x * MLP_INFER_WAVES + wave) * PW;
+    if (lane < PW) {
         float x[MIN];
         const long long p = p0 + lane;
         if (p < a.npts) point_features(a.grad, p, a.plane, a.Ny, a.norm, x);
@@ -178,15 +181,15 @@ __global__ void __launch_bounds__(64 * MLP_INFER_WAVES) mlp_infer_kernel(MLPInfe
         for (int c = 0; c < MK; ++c) bufA[lane * MK + c] = c < MIN ? f2bf(x[c]) : (unsigned short)0;
     }
     wave_sync();
-    hidden_layer<4, 1>(bufA, W + OW1, a.theta + OB1, nullptr, bufB, lr, lg);
+    hidden_layer<RB, 1>(bufA, W + OW1, a.theta + OB1, nullptr, bufB, lr, lg);
     wave_sync();
-    hidden_layer<4, 4>(bufB, W + OW2, a.theta + OB2, nullptr, bufA, lr, lg);
+    hidden_layer<RB, 4>(bufB, W + OW2, a.theta + OB2, nullptr, bufA, lr, lg);
     wave_sync();
-    hidden_layer<4, 4>(bufA, W + OW3, a.theta + OB3, nullptr, bufB, lr, lg);
+    hidden_layer<RB, 4>(bufA, W + OW3, a.theta + OB3, nullptr, bufB, lr, lg);
     wave_sync();
     // output layer: column lr < 3 is a force component
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
+    for (int rb = 0; rb < RB; ++rb) {
         f32x4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks)
@@ -396,7 +399,7 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(float* theta, unsigned sh
     wbf[j] = f2bf(th);
 }
 
-size_t mlp_infer_lds() { return (size_t)(NPAR + 4) * 2 + MLP_INFER_WAVES * 2 * 64 * MH * 2; }
+size_t mlp_infer_lds() { return (size_t)(NPAR + 4) * 2 + MLP_INFER_WAVES * 2 * MLP_INFER_PW * MH * 2; }
 size_t mlp_train_lds() {
     const size_t wave = 3 * 32 * MH * 4 + (3 * 32 * MH + 32 * MK + 32 * MH) * 2;
     return ((size_t)(NPAR * 2 + 15) / 16) * 16 + 2 * wave;
@@ -405,9 +408,9 @@ size_t mlp_train_lds() {
 hipError_t launch_mlp_infer(const MLPInferArgs& a, hipStream_t st) {
     const size_t lds = mlp_infer_lds();
     (void)hipFuncSetAttribute((const void*)mlp_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    constexpr int pts = 64 * MLP_INFER_WAVES;
+    constexpr int pts = MLP_INFER_PW * MLP_INFER_WAVES;
     const unsigned blocks = (unsigned)((a.npts + pts - 1) / pts);
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(pts), lds, st, a);
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(64 * MLP_INFER_WAVES), lds, st, a);
     return hipGetLastError();
 }
 
