@@ -155,12 +155,14 @@ __device__ __forceinline__ void hidden_layer(const unsigned short* in, const uns
 
 // Inference: a wave owns MLP_INFER_RB row blocks of 16 consecutive padded
 // points; a block holds the bf16 weights (21 KB) and MLP_INFER_WAVES waves of
-// two activation buffers (16 RB x 64 bf16 each).  32 points per wave in
-// blocks of four waves (53 KB) fit three blocks, twelve waves, in a CU: each
-// wave's layers are a serial chain of matrix products and GELUs, so more and
-// shorter waves run faster (64 points per wave in blocks of four, 86 KB,
-// fit one block in a CU and ran 67 us at 32x33x32, 48 us in blocks of two).
-constexpr int MLP_INFER_RB = 2, MLP_INFER_WAVES = 4, MLP_INFER_PW = 16 * MLP_INFER_RB;
+// two activation buffers (16 RB x 64 bf16 each).  Each wave's layers are a
+// serial chain of matrix products and GELUs, so more and shorter waves run
+// faster: 16 points per wave in blocks of eight waves (53 KB) fit three
+// blocks, 24 waves, in a CU.  Measured at 32x33x32 (76 k points): 64 points
+// per wave in blocks of four (86 KB, one block per CU) 67 us; in blocks of
+// two 48 us; 32 points per wave in blocks of four about 25 us less per step
+// than that; 16 points per wave in blocks of eight, 8 us less again.
+constexpr int MLP_INFER_RB = 1, MLP_INFER_WAVES = 8, MLP_INFER_PW = 16 * MLP_INFER_RB;
 __global__ void __launch_bounds__(64 * MLP_INFER_WAVES) mlp_infer_kernel(MLPInferArgs a) {
     extern __shared__ double2 smem_raw[];
     constexpr int RB = MLP_INFER_RB, PW = MLP_INFER_PW;
